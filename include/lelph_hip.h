@@ -1,0 +1,226 @@
+/*
+ * lelph_hip.h -- C-ABI of the MI355X (gfx950) electron-phonon matrix-element
+ * path  g^nu_mn(k,q) = < psi_m,k+q | dV_q,nu | psi_n,k >.
+ *
+ * Plain C, no MPI types, no torch types.  Every entry point returns an int
+ * status (LELPH_OK == 0); lelph_strerror() turns it into text.  Complex
+ * buffers are interleaved (re, im) doubles, i.e. the "re_im" trailing axis of
+ * the reference's netCDF variables.
+ *
+ * The structs mirror the reference's struct Lattice / struct Pseudo / struct
+ * WFC (src/common/dtypes.h:60-266) field by field, keeping the reference's
+ * names, units and array layouts:
+ *   - k points of the irreducible BZ in cartesian units without 2*pi,
+ *   - G vectors (npw,3) cartesian without 2*pi,
+ *   - blat_vec includes 2*pi (computed with the reference's ELPH_PI),
+ *   - wavefunctions (nspin, nbnds, nspinor, npw),
+ *   - dV (nmodes, nmag, Nx, Ny, Nz), x slowest, z fastest,
+ *   - g (nmodes, nspin, nb_k, nb_kq): row = initial band (k), column = final
+ *     band (k+q)  (src/dvloc/compute_Vbare_loc.c:323-332).
+ *
+ * One context per host thread.  Calls on one context are serialised on the
+ * context's own HIP stream.
+ */
+#ifndef LELPH_HIP_H
+#define LELPH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- status codes ---------------- */
+enum lelph_status
+{
+    LELPH_OK = 0,
+    LELPH_ERR_ARG = 1,         /* bad argument / inconsistent sizes          */
+    LELPH_ERR_ALLOC = 2,       /* device or host allocation failed           */
+    LELPH_ERR_HIP = 3,         /* a HIP runtime call failed                  */
+    LELPH_ERR_RANGE = 4,       /* G vector outside the FFT box, table range  */
+    LELPH_ERR_UNSUPPORTED = 5, /* combination not supported by this build    */
+    LELPH_ERR_STATE = 6,       /* call order violated (e.g. no dV set)       */
+    LELPH_ERR_NOKQ = 7,        /* k+q not on the k grid (incommensurate)     */
+    LELPH_ERR_SYM = 8          /* symmetry matrix is not orthogonal          */
+};
+
+/* ---------------- precision of the contractions ---------------- */
+enum lelph_precision
+{
+    LELPH_FP64 = 0,   /* FP64 FFT, FP64 MFMA contraction                      */
+    LELPH_FP32 = 1,   /* FP32 FFT (FP64 twiddles), FP32 MFMA, FP64 finalise   */
+    LELPH_BF16X3 = 2, /* FP32 FFT, BF16 2-piece split, 3 MFMA products        */
+    LELPH_BF16X6 = 3  /* FP32 FFT, BF16 3-piece split, 6 MFMA products        */
+};
+
+/* ---------------- kernel flags (src/elph/set_kernel.c:18-52) ------------ */
+enum lelph_kernel
+{
+    LELPH_KERNEL_DFPT = 0,       /* dvscf + bare local + non local          */
+    LELPH_KERNEL_DFPT_LOCAL = 1, /* dvscf + bare local                      */
+    LELPH_KERNEL_BARE = 2,       /* bare local + non local (no screening)   */
+    LELPH_KERNEL_BARE_LOCAL = 3  /* bare local only                         */
+};
+
+/* struct symmetry, src/common/dtypes.h:60-68 */
+typedef struct lelph_symmetry
+{
+    double Rmat[9]; /* cartesian rotation, row major                          */
+    double tau[3];  /* fractional translation, cartesian                      */
+    int time_rev;   /* 1 if the operation includes time reversal              */
+} lelph_symmetry;
+
+/* struct Lattice, src/common/dtypes.h:80-140 */
+typedef struct lelph_lattice
+{
+    int natom;
+    int nsym;
+    int timerev;
+    int nspin;
+    int nspinor;
+    int nbnds;
+    int nmodes;
+    int nmag;
+    char dimension; /* '3', '2' or '1' (Coulomb cut-off in dVlocq)            */
+    int is_soc_present;
+    int64_t fft_dims[3];
+    double alat_vec[9]; /* a[i] = alat_vec[:,i]                               */
+    double blat_vec[9]; /* b[i] = blat_vec[:,i], includes 2*pi                */
+    double volume;
+    const double* atomic_pos; /* (natom,3) cartesian                          */
+    const int* atom_type;     /* (natom)                                      */
+    int64_t nkpts_iBZ;
+    const double* kpt_iredBZ; /* (nkpts_iBZ,3) cartesian, no 2*pi            */
+    int64_t nkpts_BZ;
+    const double* kpt_fullBZ_crys; /* (nkpts_BZ,3) crystal                   */
+    const int* kmap;               /* (nkpts_BZ,2): (ik_iBZ, isym)           */
+    const lelph_symmetry* syms;    /* (nsym)                                  */
+} lelph_lattice;
+
+/* struct local_pseudo, src/common/dtypes.h:212-224 */
+typedef struct lelph_local_pseudo
+{
+    const double* Vloc_atomic; /* (ngrid) local pseudopotential V(r), Ry    */
+    const double* r_grid;      /* (ngrid)                                    */
+    const double* rab_grid;    /* (ngrid) dr/di                              */
+    int64_t ngrid;
+    double Zval;
+} lelph_local_pseudo;
+
+/* struct Pseudo, src/common/dtypes.h:226-250 */
+typedef struct lelph_pseudo
+{
+    int ntype;
+    int lmax;
+    int64_t nltimesj;
+    const double* PP_table; /* (nltimesj, ntype, 3): (l+1, 2j, pp_spin)     */
+    const double* Fsign;    /* (nltimesj, ntype)                            */
+    /* SOC f-coefficients: NULL, or (nltimesj*ntype) pointers, entry
+     * [lidx*ntype+itype] -> complex (2l+1, 2l+1, nspinor, nspinor)         */
+    const double* const* fCoeff;
+    const lelph_local_pseudo* loc_pseudo; /* (ntype)                        */
+    double qmax_abs; /* max |q_i| (crystal) the V_loc(G) table must cover    */
+} lelph_pseudo;
+
+/* one (k, k+q) work item of a batch: the arguments of elphLocal */
+typedef struct lelph_pair
+{
+    int ikq, ik, kqsym, ksym;
+} lelph_pair;
+
+typedef struct lelph_ctx lelph_ctx;
+
+/* ------------------------------------------------------------------------
+ * Context.  Owns the device copies of the iBZ wavefunctions, G lists, KB
+ * projectors, the V_loc(G) table and the current dV_q.
+ * Replaces the allocation done by read_and_alloc_save_data
+ * (src/io/get_save_data.c:51-638) for the data the hot path reads.
+ * ---------------------------------------------------------------------- */
+int lelph_create(const lelph_lattice* lattice, const lelph_pseudo* pseudo,
+                 int precision, lelph_ctx** out);
+void lelph_destroy(lelph_ctx* ctx);
+const char* lelph_strerror(int status);
+int lelph_set_precision(lelph_ctx* ctx, int precision);
+int lelph_get_device_count(int* count);
+
+/* iBZ wavefunction ik_ibz: wfc (nspin,nbnds,nspinor,npw) complex, gvec
+ * (npw,3) cartesian without 2*pi, Fk (nltimesj,ntype,npw) or NULL.
+ * Replaces the struct WFC fill of src/io/get_save_data.c:516-609. */
+int lelph_upload_wfc(lelph_ctx* ctx, int64_t ik_ibz, const double* wfc,
+                     const double* gvec, const double* Fk, int64_t npw);
+
+/* V_loc(|G|) spline table: create_vlocg_table (src/dvloc/vlocg_table.c:16-154).
+ * Optional host copies of the result (each may be NULL): g_co (npts),
+ * vlocg (ntype,npts), vploc (ntype,npts); *npts_out receives npts. */
+int lelph_create_vlocg_table(lelph_ctx* ctx, int64_t* npts_out, double* g_co,
+                             double* vlocg, double* vploc);
+
+/* dVlocq (src/dvloc/dVbare_loc.c:16-231): bare local dV of q (crystal) in
+ * the mode basis eigVec (nmodes, 3*natom).  Vlocr (nmodes,Nx,Ny,Nz) is an
+ * optional host copy of the result (may be NULL). */
+int lelph_dvlocq(lelph_ctx* ctx, const double qpt[3], const double* eigVec,
+                 double* Vlocr);
+
+/* Per-q set-up of the potential the local part contracts with; the caller
+ * sequence of src/elph/elph_driver.c:210-245:
+ *   dvscf   : (nmodes,nmag,Nx,Ny,Nz) complex or NULL (no screening),
+ *   patterns: NULL if dvscf is already in the mode basis, else the
+ *             (nmodes,nmodes) displacement patterns; dvscf is then in the
+ *             pattern basis and is rotated as eig @ pats^H @ dvscf
+ *             (src/io/qe/read_dvscf.c:107-158),
+ *   kernel  : lelph_kernel; bare local dV is added (dVlocq + add_dvscf_qe,
+ *             src/dvloc/add_dvscf.c:12-92).
+ * The eigVec is kept for the non-local part of this q. */
+int lelph_set_dvscf(lelph_ctx* ctx, const double qpt[3], const double* eigVec,
+                    const double* dvscf, const double* patterns, int kernel);
+
+/* Optional host copy of the current dV (nmodes,nmag,Nx,Ny,Nz). */
+int lelph_get_dvscf(lelph_ctx* ctx, double* dvscf);
+
+/* elphLocal (src/dvloc/compute_Vbare_loc.c:17-353).  Zeroes and fills
+ * elph_kq (nmodes,nspin,nbnds,nbnds) with the local part. */
+int lelph_elph_local(lelph_ctx* ctx, const double qpt[3], int ikq, int ik,
+                     int kqsym, int ksym, double* elph_kq);
+
+/* add_elphNonLocal (src/nonloc/Vbare_nloc.c:35-577).  ADDS the
+ * Kleinman-Bylander part to elph_kq (the caller runs elph_local first, as
+ * src/elph/compute_elph_matq.c:135-143 requires). */
+int lelph_add_elph_nonlocal(lelph_ctx* ctx, int ikq, int ik, int kqsym,
+                            int ksym, const double* eigVec, double* elph_kq);
+
+/* elphLocal + add_elphNonLocal in one device round trip (non_loc = 0/1). */
+int lelph_gkq(lelph_ctx* ctx, const double qpt[3], int ikq, int ik, int kqsym,
+              int ksym, int non_loc, double* elph_kq);
+
+/* A batch of (k,k+q) pairs for the current q; g_out (npairs, nmodes, nspin,
+ * nbnds, nbnds).  Device-resident between pairs, one copy back at the end. */
+int lelph_gkq_batch(lelph_ctx* ctx, const double qpt[3], int64_t npairs,
+                    const lelph_pair* pairs, int non_loc, double* g_out);
+
+/* The k loop of compute_and_write_elphq (src/elph/compute_elph_matq.c:20-234)
+ * for the iBZ q point qpt (crystal): k+q lookup on the full-BZ grid,
+ * kminusq ("yambo" convention) swap, local + non-local per k.  g_out is the
+ * (nkpts_BZ, nmodes, nspin, nbnds, nbnds) slab elph_mat[qpos, :, ...]. */
+int lelph_compute_elphq(lelph_ctx* ctx, const double qpt[3], int non_loc,
+                        int kminusq, double* g_out);
+
+/* elph_q_rotate (src/symmetries/elphq_expand.c:16-73): g(Sk,Sq) from g(k,q)
+ * and the electronic D matrices (nspin,nb,nb) of the k+q (Dmats_l) and k
+ * (Dmats_r) points.  Host-side FP64. */
+int lelph_elph_q_rotate(const double* Dmats_l, const double* elph_mat_q,
+                        const double* Dmats_r, int nmodes, int nspin,
+                        int nbnds, int tim_rev, double* elph_mat_Sq);
+
+/* get_KplusQ_idxs (src/common/numerical_func.c:471-524). */
+int lelph_get_kplusq_idxs(int64_t nbz, const double* kpoints_crys,
+                          const double qpt_crys[3], int* kplusq_idxs);
+
+/* Timing of the last lelph_gkq / batch call, milliseconds per phase:
+ * [0] scatter+FFT, [1] local contraction, [2] non-local, [3] total. */
+int lelph_last_timings(lelph_ctx* ctx, double* ms4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LELPH_HIP_H */

@@ -1,0 +1,147 @@
+"""ctypes mirror of include/lelph_hip.h (the C-ABI boundary types).
+
+The structs follow the reference's struct Lattice / Pseudo / WFC
+(src/common/dtypes.h:60-266).  `pack_system` builds them from a
+`letzelphc_amd.synth.System` and keeps every numpy buffer alive for as long
+as the returned `Packed` object lives.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+c_double_p = C.POINTER(C.c_double)
+c_int_p = C.POINTER(C.c_int)
+
+LELPH_OK = 0
+STATUS_NAMES = {
+    0: "LELPH_OK", 1: "LELPH_ERR_ARG", 2: "LELPH_ERR_ALLOC", 3: "LELPH_ERR_HIP",
+    4: "LELPH_ERR_RANGE", 5: "LELPH_ERR_UNSUPPORTED", 6: "LELPH_ERR_STATE",
+    7: "LELPH_ERR_NOKQ", 8: "LELPH_ERR_SYM",
+}
+
+PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x6": 3}
+KERNELS = {"dfpt": 0, "dfpt_local": 1, "bare": 2, "bare_local": 3}
+
+
+class Symmetry(C.Structure):
+    _fields_ = [("Rmat", C.c_double * 9), ("tau", C.c_double * 3),
+                ("time_rev", C.c_int)]
+
+
+class Lattice(C.Structure):
+    _fields_ = [
+        ("natom", C.c_int), ("nsym", C.c_int), ("timerev", C.c_int),
+        ("nspin", C.c_int), ("nspinor", C.c_int), ("nbnds", C.c_int),
+        ("nmodes", C.c_int), ("nmag", C.c_int), ("dimension", C.c_char),
+        ("is_soc_present", C.c_int), ("fft_dims", C.c_int64 * 3),
+        ("alat_vec", C.c_double * 9), ("blat_vec", C.c_double * 9),
+        ("volume", C.c_double), ("atomic_pos", c_double_p),
+        ("atom_type", c_int_p), ("nkpts_iBZ", C.c_int64),
+        ("kpt_iredBZ", c_double_p), ("nkpts_BZ", C.c_int64),
+        ("kpt_fullBZ_crys", c_double_p), ("kmap", c_int_p),
+        ("syms", C.POINTER(Symmetry)),
+    ]
+
+
+class LocalPseudo(C.Structure):
+    _fields_ = [("Vloc_atomic", c_double_p), ("r_grid", c_double_p),
+                ("rab_grid", c_double_p), ("ngrid", C.c_int64),
+                ("Zval", C.c_double)]
+
+
+class Pseudo(C.Structure):
+    _fields_ = [("ntype", C.c_int), ("lmax", C.c_int), ("nltimesj", C.c_int64),
+                ("PP_table", c_double_p), ("Fsign", c_double_p),
+                ("fCoeff", C.POINTER(c_double_p)),
+                ("loc_pseudo", C.POINTER(LocalPseudo)), ("qmax_abs", C.c_double)]
+
+
+class Pair(C.Structure):
+    _fields_ = [("ikq", C.c_int), ("ik", C.c_int), ("kqsym", C.c_int),
+                ("ksym", C.c_int)]
+
+
+def dptr(a: np.ndarray):
+    """double* to a C-contiguous float64 / complex128 array."""
+    assert a.flags["C_CONTIGUOUS"], "array must be C contiguous"
+    assert a.dtype in (np.float64, np.complex128), a.dtype
+    return a.ctypes.data_as(c_double_p)
+
+
+def iptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"] and a.dtype == np.int32
+    return a.ctypes.data_as(c_int_p)
+
+
+@dataclass
+class Packed:
+    lattice: Lattice
+    pseudo: Pseudo
+    keep: list = field(default_factory=list)
+
+
+def pack_system(sys) -> Packed:
+    """Builds Lattice/Pseudo structs for a synth.System."""
+    keep = []
+
+    def hold(a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(a)
+        return a
+
+    lat = Lattice()
+    lat.natom = sys.natom
+    lat.nsym = len(sys.syms)
+    lat.timerev = sys.timerev
+    lat.nspin = sys.nspin
+    lat.nspinor = sys.nspinor
+    lat.nbnds = sys.nbnds
+    lat.nmodes = sys.nmodes
+    lat.nmag = sys.nmag
+    lat.dimension = sys.dimension.encode()
+    lat.is_soc_present = 0
+    for i in range(3):
+        lat.fft_dims[i] = int(sys.fft_dims[i])
+    for i in range(9):
+        lat.alat_vec[i] = float(sys.alat.reshape(-1)[i])
+        lat.blat_vec[i] = float(sys.blat.reshape(-1)[i])
+    lat.volume = float(sys.volume)
+    lat.atomic_pos = dptr(hold(sys.atomic_pos, np.float64))
+    lat.atom_type = iptr(hold(sys.atom_type, np.int32))
+    lat.nkpts_iBZ = len(sys.kpt_ibz_cart)
+    lat.kpt_iredBZ = dptr(hold(sys.kpt_ibz_cart, np.float64))
+    lat.nkpts_BZ = len(sys.kpt_bz_crys)
+    lat.kpt_fullBZ_crys = dptr(hold(sys.kpt_bz_crys, np.float64))
+    lat.kmap = iptr(hold(sys.kmap, np.int32))
+    syms = (Symmetry * len(sys.syms))()
+    for i, (R, tau, tr) in enumerate(sys.syms):
+        for j in range(9):
+            syms[i].Rmat[j] = float(np.asarray(R).reshape(-1)[j])
+        for j in range(3):
+            syms[i].tau[j] = float(tau[j])
+        syms[i].time_rev = int(tr)
+    keep.append(syms)
+    lat.syms = C.cast(syms, C.POINTER(Symmetry))
+
+    pp = Pseudo()
+    ps = sys.pseudo
+    pp.ntype = ps.ntype
+    pp.lmax = ps.lmax
+    pp.nltimesj = ps.nltimesj
+    pp.PP_table = dptr(hold(ps.PP_table, np.float64))
+    pp.Fsign = dptr(hold(ps.Fsign, np.float64))
+    pp.fCoeff = None
+    locs = (LocalPseudo * ps.ntype)()
+    for t, (v, r, rab, z) in enumerate(ps.loc):
+        locs[t].Vloc_atomic = dptr(hold(v, np.float64))
+        locs[t].r_grid = dptr(hold(r, np.float64))
+        locs[t].rab_grid = dptr(hold(rab, np.float64))
+        locs[t].ngrid = len(r)
+        locs[t].Zval = float(z)
+    keep.append(locs)
+    pp.loc_pseudo = C.cast(locs, C.POINTER(LocalPseudo))
+    pp.qmax_abs = float(ps.qmax_abs)
+    return Packed(lat, pp, keep)

@@ -1,0 +1,198 @@
+// contract.hpp -- local e-ph contraction over the real-space grid on MFMA.
+//
+// The reference computes, per mode nu and band n, the forward FFT of
+// dV_nu(r) psi_n,k(r), keeps the k+q sphere times 1/N and contracts it with
+// psi_m,k+q in G space (src/fft/convolution.c:13-189,
+// src/dvloc/compute_Vbare_loc.c:286-342).  By Parseval on the same box this
+// equals
+//     g[nu][n][m] = (1/N) sum_{sigma,r} conj(psi_kq,m,sigma(r))
+//                                    (dV_nu psi_k,n)_sigma(r)
+// where psi_kq(r) is the backward transform of the (shifted, rotated) k+q
+// sphere -- exactly, since both sides use the same box.  That is one complex
+// GEMM per (nu, spin): rows (nu, n), columns m, reduction over (sigma, r).
+// The A operand dV.psi_k is formed while staging tiles into LDS, so it never
+// touches HBM.  The reduction is split over K; partials are reduced in a
+// fixed order in FP64 (no float atomics: deterministic).
+//
+// MFMA lane layouts used (16x16x4, wave64): A: lane l holds A[l%16][l/16];
+// B: lane l holds B[l/16][l%16]; D: lane l, register v holds D[row][l%16]
+// with row = Mfma<T>::row(l, v) (FP64: l/16 + 4v, measured on gfx950;
+// FP32: 4(l/16) + v).
+#pragma once
+
+#include "common.hpp"
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double>
+{
+    typedef dbl4 acc;
+    static __device__ inline acc op(double a, double b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <>
+struct Mfma<float>
+{
+    typedef flt4 acc;
+    static __device__ inline acc op(float a, float b, acc c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+#define CT_KC 16
+#define CT_LD (CT_KC + 1)
+#define CT_VB 2  // modes per block
+#define CT_NB 32 // k bands (rows) per block
+#define CT_MB 64 // k+q bands (columns) per block
+
+// Loads A[n][r] = (dV_nu psi_k,n)_sigma(r) for the block's modes.
+template <typename T>
+__device__ inline cpx<T> load_dvpsi(const cpx<T>* __restrict__ psik,
+                                    const cpx<T>* __restrict__ dv, int v, int is, int n,
+                                    int sp, int64_t r, int nb, int nspinor, int nmag,
+                                    int64_t N)
+{
+    const cpx<T>* pk = psik + ((int64_t)(is * nb + n) * nspinor) * N;
+    if (nmag == 4 && nspinor == 2)
+    {
+        // 2x2 spinor potential (src/fft/convolution.c:322-352)
+        const cpx<T>* V = dv + (int64_t)v * 4 * N;
+        return cadd(cmul(V[(int64_t)(2 * sp) * N + r], pk[r]),
+                    cmul(V[(int64_t)(2 * sp + 1) * N + r], pk[N + r]));
+    }
+    const cpx<T>* V = dv + ((int64_t)v * nmag + (nmag == 2 ? is : 0)) * N;
+    return cmul(V[r], pk[(int64_t)sp * N + r]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_contract(const cpx<T>* __restrict__ psik,
+                                                  const cpx<T>* __restrict__ psikq,
+                                                  const cpx<T>* __restrict__ dv,
+                                                  z64* __restrict__ part, int nm, int nb,
+                                                  int nspin, int nspinor, int nmag,
+                                                  int64_t N, int64_t chunks_per_split)
+{
+    typedef typename Mfma<T>::acc acc_t;
+    __shared__ cpx<T> sA[CT_VB][CT_NB][CT_LD];
+    __shared__ cpx<T> sB[CT_MB][CT_LD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ntn = (nb + CT_NB - 1) / CT_NB, ntm = (nb + CT_MB - 1) / CT_MB;
+    int bx = blockIdx.x;
+    const int tm = bx % ntm;
+    bx /= ntm;
+    const int tn = bx % ntn;
+    bx /= ntn;
+    const int v0 = bx * CT_VB, n0 = tn * CT_NB, m0 = tm * CT_MB;
+    const int split = blockIdx.y, is = blockIdx.z;
+    const int64_t nchunks = (N + CT_KC - 1) / CT_KC;
+    const int64_t c0 = (int64_t)split * chunks_per_split;
+    const int64_t c1 = (c0 + chunks_per_split < nchunks) ? c0 + chunks_per_split : nchunks;
+    const int wv = wave >> 1, wm = wave & 1;
+
+    acc_t accr[2][2], acci[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int v = 0; v < 4; ++v)
+            {
+                accr[a][b][v] = 0;
+                acci[a][b][v] = 0;
+            }
+
+    for (int sp = 0; sp < nspinor; ++sp)
+    {
+        for (int64_t c = c0; c < c1; ++c)
+        {
+            const int64_t r0 = c * CT_KC;
+            for (int f = tid; f < CT_VB * CT_NB * CT_KC; f += 256)
+            {
+                const int vv = f / (CT_NB * CT_KC);
+                const int rem = f - vv * (CT_NB * CT_KC);
+                const int nn = rem / CT_KC, rr = rem - nn * CT_KC;
+                const int v = v0 + vv, n = n0 + nn;
+                const int64_t r = r0 + rr;
+                cpx<T> a = {0, 0};
+                if (v < nm && n < nb && r < N)
+                    a = load_dvpsi<T>(psik, dv, v, is, n, sp, r, nb, nspinor, nmag, N);
+                sA[vv][nn][rr] = a;
+            }
+            for (int f = tid; f < CT_MB * CT_KC; f += 256)
+            {
+                const int mm = f / CT_KC, rr = f - mm * CT_KC;
+                const int m = m0 + mm;
+                const int64_t r = r0 + rr;
+                cpx<T> b = {0, 0};
+                if (m < nb && r < N) b = psikq[((int64_t)(is * nb + m) * nspinor + sp) * N + r];
+                sB[mm][rr] = b;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < CT_KC / 4; ++kk)
+            {
+                const int i = lane & 15, kq = kk * 4 + (lane >> 4);
+                cpx<T> a[2], b[2];
+                a[0] = sA[wv][i][kq];
+                a[1] = sA[wv][16 + i][kq];
+                b[0] = sB[wm * 32 + i][kq];
+                b[1] = sB[wm * 32 + 16 + i][kq];
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                    {
+                        // C += a conj(b): re = ar br + ai bi, im = ai br - ar bi
+                        accr[rb][cb] = Mfma<T>::op(a[rb].x, b[cb].x, accr[rb][cb]);
+                        accr[rb][cb] = Mfma<T>::op(a[rb].y, b[cb].y, accr[rb][cb]);
+                        acci[rb][cb] = Mfma<T>::op(a[rb].y, b[cb].x, acci[rb][cb]);
+                        acci[rb][cb] = Mfma<T>::op(-a[rb].x, b[cb].y, acci[rb][cb]);
+                    }
+            }
+            __syncthreads();
+        }
+    }
+    const int v = v0 + wv;
+    if (v >= nm) return;
+    for (int rb = 0; rb < 2; ++rb)
+        for (int cb = 0; cb < 2; ++cb)
+            for (int e = 0; e < 4; ++e)
+            {
+                const int row = n0 + rb * 16 + Mfma<T>::row(lane, e);
+                const int col = m0 + wm * 32 + cb * 16 + (lane & 15);
+                if (row < nb && col < nb)
+                {
+                    const int64_t o = ((((int64_t)split * nm + v) * nspin + is) * nb + row) * nb + col;
+                    part[o] = {(double)accr[rb][cb][e], (double)acci[rb][cb][e]};
+                }
+            }
+}
+
+// g[i] = scale * sum_s part[s][i], fixed order.
+__global__ void k_reduce_parts(const z64* __restrict__ part, int nsplit, int64_t count,
+                               double scale, z64* __restrict__ g)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    double re = 0, im = 0;
+    for (int s = 0; s < nsplit; ++s)
+    {
+        z64 p = part[(int64_t)s * count + i];
+        re += p.x;
+        im += p.y;
+    }
+    g[i] = {re * scale, im * scale};
+}
+
+// dst[i] = (float) src[i]
+__global__ void k_z64_to_c32(const z64* __restrict__ s, c32* __restrict__ d, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = {(float)s[i].x, (float)s[i].y};
+}

@@ -1,0 +1,336 @@
+// fft.hpp -- batched mixed-radix 3-D FFT for gfx950.
+//
+// The box layout is the reference's (Nx, Ny, Nz) row major with z fastest
+// (src/fft/bwdfft.c:136-219).  A 3-D transform is three passes, one per
+// axis.  Each pass is one kernel: a workgroup stages `tl` lines of one axis
+// through LDS with coalesced loads (lines adjacent in memory are taken
+// together so every row of the tile is one contiguous read), runs the
+// Stockham autosort stages in LDS (radix 2/3/4/5 butterflies, any other
+// prime by direct DFT) and writes the lines back.  Twiddles are a table of
+// e^{+2 pi i t/n} computed on the host in FP64 (rounded to FP32 for the
+// FP32 path).  Sign and normalisation follow FFTW: BACKWARD (+1) is e^{+i},
+// FORWARD (-1) is e^{-i}, both unnormalised.
+#pragma once
+
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+
+#define FFT_MAX_STAGES 16
+#define FFT_MAX_RADIX 16
+
+struct FftStages
+{
+    int nst;
+    int radix[FFT_MAX_STAGES];
+};
+
+struct FftAxis
+{
+    int n = 0;
+    FftStages st{};
+    z64* w64 = nullptr; // device (n) e^{+2 pi i t / n}
+    c32* w32 = nullptr;
+};
+
+static inline int fft_factor(int n, FftStages* st)
+{
+    st->nst = 0;
+    int m = n;
+    while (m % 4 == 0 && st->nst < FFT_MAX_STAGES)
+    {
+        st->radix[st->nst++] = 4;
+        m /= 4;
+    }
+    for (int p = 2; m > 1 && st->nst < FFT_MAX_STAGES;)
+    {
+        if (m % p == 0)
+        {
+            if (p > FFT_MAX_RADIX) return LELPH_ERR_UNSUPPORTED;
+            st->radix[st->nst++] = p;
+            m /= p;
+        }
+        else
+            ++p;
+    }
+    return m == 1 ? LELPH_OK : LELPH_ERR_UNSUPPORTED;
+}
+
+static inline int fft_axis_init(FftAxis* ax, int n)
+{
+    ax->n = n;
+    LELPH_TRY(fft_factor(n, &ax->st));
+    std::vector<z64> w(n);
+    std::vector<c32> wf(n);
+    for (int t = 0; t < n; ++t)
+    {
+        double a = 2.0 * M_PI * (double)t / (double)n;
+        w[t] = {cos(a), sin(a)};
+        wf[t] = {(float)w[t].x, (float)w[t].y};
+    }
+    HIP_TRY(hipMalloc(&ax->w64, sizeof(z64) * n));
+    HIP_TRY(hipMalloc(&ax->w32, sizeof(c32) * n));
+    HIP_TRY(hipMemcpy(ax->w64, w.data(), sizeof(z64) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ax->w32, wf.data(), sizeof(c32) * n, hipMemcpyHostToDevice));
+    return LELPH_OK;
+}
+
+static inline void fft_axis_free(FftAxis* ax)
+{
+    if (ax->w64) (void)hipFree(ax->w64);
+    if (ax->w32) (void)hipFree(ax->w32);
+    ax->w64 = nullptr;
+    ax->w32 = nullptr;
+}
+
+// twiddle w^(SIGN): the table holds e^{+i}; forward uses the conjugate.
+template <typename T, int SIGN>
+__device__ inline cpx<T> tw(const cpx<T>* W, int t)
+{
+    cpx<T> w = W[t];
+    if (SIGN < 0) w.y = -w.y;
+    return w;
+}
+
+// multiply by SIGN * i
+template <typename T, int SIGN>
+__device__ inline cpx<T> mul_si(cpx<T> a)
+{
+    return SIGN > 0 ? cpx<T>{-a.y, a.x} : cpx<T>{a.y, -a.x};
+}
+
+template <typename T, int SIGN>
+__device__ inline void dft_small(cpx<T>* v, int R, const cpx<T>* W, int n)
+{
+    if (R == 2)
+    {
+        cpx<T> a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    }
+    else if (R == 3)
+    {
+        const T c = (T)-0.5, s = (T)0.86602540378443864676;
+        cpx<T> t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+        cpx<T> m = {v[0].x + c * t1.x, v[0].y + c * t1.y};
+        cpx<T> is = mul_si<T, SIGN>(cpx<T>{s * t2.x, s * t2.y});
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(m, is);
+        v[2] = csub(m, is);
+    }
+    else if (R == 4)
+    {
+        cpx<T> a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+        cpx<T> c = cadd(v[1], v[3]), d = mul_si<T, SIGN>(csub(v[1], v[3]));
+        v[0] = cadd(a, c);
+        v[2] = csub(a, c);
+        v[1] = cadd(b, d);
+        v[3] = csub(b, d);
+    }
+    else if (R == 5)
+    {
+        const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+        const T s1 = (T)0.95105651629515357212, s2 = (T)0.58778525229247312917;
+        cpx<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+        cpx<T> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+        cpx<T> m1 = {v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
+        cpx<T> m2 = {v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
+        cpx<T> s_1 = mul_si<T, SIGN>(cpx<T>{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y});
+        cpx<T> s_2 = mul_si<T, SIGN>(cpx<T>{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y});
+        v[0] = cadd(v[0], cadd(t1, t2));
+        v[1] = cadd(m1, s_1);
+        v[4] = csub(m1, s_1);
+        v[2] = cadd(m2, s_2);
+        v[3] = csub(m2, s_2);
+    }
+}
+
+// One Stockham stage of radix R on nl lines of n points held in LDS
+// (Govindaraju et al., SC'08, "FftIteration"): butterfly j reads
+// in[j + r n/R], twiddles by w^(r (j mod Ns) n/(Ns R)), and writes
+// out[(j/Ns) Ns R + (j mod Ns) + r Ns].
+template <typename T, int SIGN, int R>
+__device__ inline void fft_stage(const cpx<T>* in, cpx<T>* out, const cpx<T>* W, int n,
+                                 int nl, int Ns, int tid, int bd)
+{
+    const int nbf = n / R;
+    const int tws = n / (Ns * R);
+    for (int w = tid; w < nl * nbf; w += bd)
+    {
+        const int t = w / nbf, j = w - t * nbf;
+        const int k = j % Ns;
+        cpx<T> v[R];
+        const cpx<T>* src = in + t * n + j;
+        v[0] = src[0];
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = cmul(src[r * nbf], tw<T, SIGN>(W, r * k * tws));
+        dft_small<T, SIGN>(v, R, W, n);
+        cpx<T>* dst = out + t * n + (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
+    }
+}
+
+template <typename T, int SIGN>
+__device__ inline void fft_stage_any(const cpx<T>* in, cpx<T>* out, const cpx<T>* W, int n,
+                                     int nl, int Ns, int R, int tid, int bd)
+{
+    const int nbf = n / R;
+    const int tws = n / (Ns * R);
+    const int step = n / R;
+    for (int w = tid; w < nl * nbf; w += bd)
+    {
+        const int t = w / nbf, j = w - t * nbf;
+        const int k = j % Ns;
+        const cpx<T>* src = in + t * n + j;
+        cpx<T>* dst = out + t * n + (j / Ns) * Ns * R + k;
+        for (int s = 0; s < R; ++s)
+        {
+            cpx<T> acc = {0, 0};
+            for (int r = 0; r < R; ++r)
+            {
+                cpx<T> x = src[r * nbf];
+                if (r) x = cmul(x, tw<T, SIGN>(W, r * k * tws));
+                acc = cadd(acc, cmul(x, tw<T, SIGN>(W, ((r * s) % R) * step)));
+            }
+            dst[s * Ns] = acc;
+        }
+    }
+}
+
+// One axis pass over a batch of lines.  Line (o, i) element j lives at
+//   data[o * ostride + i * lstride + j * estride],  i in [0, n_inner).
+// A block takes tl consecutive i of one o.
+template <typename T, int SIGN>
+__global__ __launch_bounds__(256) void k_fft_lines(cpx<T>* __restrict__ data, int n,
+                                                   int64_t n_inner, int64_t ostride,
+                                                   int64_t lstride, int64_t estride,
+                                                   int tl, FftStages st,
+                                                   const cpx<T>* __restrict__ Wg)
+{
+    extern __shared__ char fft_smem[];
+    cpx<T>* W = reinterpret_cast<cpx<T>*>(fft_smem);
+    cpx<T>* b0 = W + n;
+    cpx<T>* b1 = b0 + (int64_t)tl * n;
+    const int tid = threadIdx.x, bd = blockDim.x;
+    const int64_t tiles = (n_inner + tl - 1) / tl;
+    const int64_t o = blockIdx.x / tiles;
+    const int64_t i0 = (blockIdx.x % tiles) * tl;
+    const int nl = (int)((n_inner - i0) < tl ? (n_inner - i0) : tl);
+    cpx<T>* base = data + o * ostride + i0 * lstride;
+
+    for (int t = tid; t < n; t += bd) W[t] = Wg[t];
+    const int tot = nl * n;
+    if (lstride == 1)
+    {
+        for (int f = tid; f < tot; f += bd)
+        {
+            int j = f / nl, t = f - j * nl;
+            b0[t * n + j] = base[(int64_t)j * estride + t];
+        }
+    }
+    else
+    {
+        for (int f = tid; f < tot; f += bd)
+        {
+            int t = f / n, j = f - t * n;
+            b0[t * n + j] = base[(int64_t)t * lstride + (int64_t)j * estride];
+        }
+    }
+    __syncthreads();
+
+    cpx<T>* in = b0;
+    cpx<T>* out = b1;
+    int Ns = 1;
+    for (int s = 0; s < st.nst; ++s)
+    {
+        const int R = st.radix[s];
+        switch (R)
+        {
+            case 2: fft_stage<T, SIGN, 2>(in, out, W, n, nl, Ns, tid, bd); break;
+            case 3: fft_stage<T, SIGN, 3>(in, out, W, n, nl, Ns, tid, bd); break;
+            case 4: fft_stage<T, SIGN, 4>(in, out, W, n, nl, Ns, tid, bd); break;
+            case 5: fft_stage<T, SIGN, 5>(in, out, W, n, nl, Ns, tid, bd); break;
+            default: fft_stage_any<T, SIGN>(in, out, W, n, nl, Ns, R, tid, bd); break;
+        }
+        __syncthreads();
+        cpx<T>* tmp = in;
+        in = out;
+        out = tmp;
+        Ns *= R;
+    }
+
+    if (lstride == 1)
+    {
+        for (int f = tid; f < tot; f += bd)
+        {
+            int j = f / nl, t = f - j * nl;
+            base[(int64_t)j * estride + t] = in[t * n + j];
+        }
+    }
+    else
+    {
+        for (int f = tid; f < tot; f += bd)
+        {
+            int t = f / n, j = f - t * n;
+            base[(int64_t)t * lstride + (int64_t)j * estride] = in[t * n + j];
+        }
+    }
+}
+
+template <typename T>
+static inline const cpx<T>* axis_tw(const FftAxis& a);
+template <>
+inline const z64* axis_tw<double>(const FftAxis& a)
+{
+    return a.w64;
+}
+template <>
+inline const c32* axis_tw<float>(const FftAxis& a)
+{
+    return a.w32;
+}
+
+template <typename T>
+static int fft_pass(const FftAxis& ax, cpx<T>* data, int64_t n_outer, int64_t n_inner,
+                    int64_t ostride, int64_t lstride, int64_t estride, int sign,
+                    hipStream_t stream)
+{
+    const int n = ax.n;
+    const int64_t lds_budget = 60 * 1024;
+    int64_t tl = (lds_budget / (int64_t)sizeof(cpx<T>) - n) / (2 * n);
+    if (tl > 64) tl = 64;
+    if (tl > n_inner) tl = n_inner;
+    if (tl < 1) return LELPH_ERR_UNSUPPORTED;
+    const int64_t tiles = cdiv(n_inner, tl);
+    const int64_t nblk = tiles * n_outer;
+    if (nblk > 0x7fffffffLL) return LELPH_ERR_UNSUPPORTED;
+    const size_t lds = sizeof(cpx<T>) * (size_t)(n + 2 * tl * n);
+    if (sign > 0)
+        hipLaunchKernelGGL((k_fft_lines<T, 1>), dim3((unsigned)nblk), dim3(256), lds, stream,
+                           data, n, n_inner, ostride, lstride, estride, (int)tl, ax.st,
+                           axis_tw<T>(ax));
+    else
+        hipLaunchKernelGGL((k_fft_lines<T, -1>), dim3((unsigned)nblk), dim3(256), lds, stream,
+                           data, n, n_inner, ostride, lstride, estride, (int)tl, ax.st,
+                           axis_tw<T>(ax));
+    HIP_TRY(hipGetLastError());
+    return LELPH_OK;
+}
+
+// 3-D transform of nsets boxes (Nx,Ny,Nz), in place.
+template <typename T>
+static int fft3d_batch(const FftAxis* ax, cpx<T>* data, int64_t nsets, int sign,
+                       hipStream_t stream)
+{
+    const int64_t Nx = ax[0].n, Ny = ax[1].n, Nz = ax[2].n;
+    const int64_t nf = Nx * Ny * Nz;
+    // z: contiguous lines
+    LELPH_TRY(fft_pass<T>(ax[2], data, 1, nsets * Nx * Ny, 0, Nz, 1, sign, stream));
+    // y: outer (set, x), inner z
+    LELPH_TRY(fft_pass<T>(ax[1], data, nsets * Nx, Nz, Ny * Nz, 1, Nz, sign, stream));
+    // x: outer set, inner (y, z)
+    LELPH_TRY(fft_pass<T>(ax[0], data, nsets, Ny * Nz, nf, 1, Ny * Nz, sign, stream));
+    return LELPH_OK;
+}

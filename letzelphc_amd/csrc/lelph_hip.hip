@@ -1,0 +1,1128 @@
+// lelph_hip.hip -- C-ABI implementation of the MI355X e-ph matrix-element path.
+//
+// Host orchestration of the gfx950 kernels in fft.hpp, wfc.hpp,
+// contract.hpp, nonloc.hpp and dvloc.hpp.  One context owns one HIP stream
+// and all device buffers; the iBZ wavefunctions and the current dV_q stay
+// resident in HBM across (k, q) pairs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "contract.hpp"
+#include "dvloc.hpp"
+#include "fft.hpp"
+#include "nonloc.hpp"
+#include "wfc.hpp"
+
+namespace
+{
+
+struct DevWfc
+{
+    int64_t npw = 0;
+    z64* wfc = nullptr;
+    double* gvec = nullptr;
+    double* Fk = nullptr;
+};
+
+// a growable device buffer
+struct DBuf
+{
+    void* p = nullptr;
+    size_t n = 0;
+    int ensure(size_t bytes)
+    {
+        if (bytes <= n) return LELPH_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc(&p, bytes) != hipSuccess)
+        {
+            p = nullptr;
+            return LELPH_ERR_ALLOC;
+        }
+        n = bytes;
+        return LELPH_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <typename X>
+    X* as()
+    {
+        return reinterpret_cast<X*>(p);
+    }
+};
+
+} // namespace
+
+struct lelph_ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int precision = LELPH_FP64;
+
+    // lattice (host copy)
+    lelph_lattice lat{};
+    std::vector<double> atomic_pos, kpt_ibz, kpt_bz;
+    std::vector<int> atom_type, kmap;
+    std::vector<lelph_symmetry> syms;
+
+    // pseudo (host copy)
+    int ntype = 0, lmax = 0;
+    int64_t nlj = 0;
+    std::vector<double> PP, Fsign, Zval, socfac;
+    std::vector<std::vector<double>> locV, locR, locRab;
+    double qmax_abs = 0;
+    std::vector<z64> fco;
+    std::vector<int64_t> fco_off;
+    bool has_fco = false;
+
+    // projectors: plist (natom, pmax, 3) = (lidx, l, l*l + m index) or -1
+    int pmax = 0;
+    std::vector<int> plist;
+
+    // device constant tables
+    int* d_atype = nullptr;
+    double* d_apos = nullptr;
+    double* d_Fsign = nullptr;
+    double* d_socfac = nullptr;
+    double* d_Zval = nullptr;
+    int* d_plist = nullptr;
+    z64* d_fco = nullptr;
+    int64_t* d_fco_off = nullptr;
+    int* d_err = nullptr;
+
+    FftAxis ax[3];
+    int64_t nf = 0;
+
+    std::vector<DevWfc> wf;
+    int64_t npw_max = 0;
+
+    // V_loc table
+    int64_t npts = 0;
+    double dg = 0;
+    std::vector<double> h_gco, h_vlocg, h_vploc;
+    double* d_gco = nullptr;
+    double* d_vlocg = nullptr;
+    double* d_vploc = nullptr;
+
+    // current potential
+    bool dv_set = false;
+    double q[3] = {0, 0, 0};
+    std::vector<z64> eig;
+    DBuf dv64, dv32, eig_d, vloc_tmp, rot_tmp;
+    bool dv32_valid = false;
+
+    // workspaces
+    DBuf idx_k, idx_kq, ph_k, ph_kq, box_k, box_kq, part, g, gbatch;
+    DBuf nlK, nlKp, Y_K, Y_Kp, wr_K, wr_Kp, beta_K, beta_Kp, bb_K, bb_Kp, bbpart, ebuf;
+
+    double timings[4] = {0, 0, 0, 0};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+/* ------------------------------------------------------------------------ */
+/* helpers                                                                   */
+/* ------------------------------------------------------------------------ */
+
+static void matvec3(const double* M, const double* v, bool trans, double* out)
+{
+    if (trans)
+    {
+        out[0] = M[0] * v[0] + M[3] * v[1] + M[6] * v[2];
+        out[1] = M[1] * v[0] + M[4] * v[1] + M[7] * v[2];
+        out[2] = M[2] * v[0] + M[5] * v[1] + M[8] * v[2];
+    }
+    else
+    {
+        out[0] = M[0] * v[0] + M[1] * v[1] + M[2] * v[2];
+        out[1] = M[3] * v[0] + M[4] * v[1] + M[5] * v[2];
+        out[2] = M[6] * v[0] + M[7] * v[1] + M[8] * v[2];
+    }
+}
+
+static double det3(const double* m)
+{
+    return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+           m[2] * (m[7] * m[3] - m[4] * m[6]);
+}
+
+static z64 zc(double re, double im) { return z64{re, im}; }
+
+// SU2mat (src/wfc/su2mat.c:16-117), invert_sym = false
+static int su2mat(const double* sym_in, int nspinor, int time_rev, z64* su)
+{
+    su[0] = zc(1, 0);
+    su[1] = zc(0, 0);
+    su[2] = zc(0, 0);
+    su[3] = zc(1, 0);
+    if (nspinor == 1) return LELPH_OK;
+    const double det = det3(sym_in);
+    if (fabs(fabs(det) - 1) > 1e-4) return LELPH_ERR_SYM;
+    double s[9];
+    for (int i = 0; i < 9; ++i) s[i] = sym_in[i] / det;
+    double al, be, de;
+    if (fabs(s[6] + 1) < LELPH_EPS)
+    {
+        al = 0;
+        be = LELPH_RPI / 2.0;
+        de = atan2(s[1], s[2]);
+    }
+    else if (fabs(s[6] - 1) < LELPH_EPS)
+    {
+        al = 0;
+        be = -LELPH_RPI / 2.0;
+        de = atan2(-s[1], -s[2]);
+    }
+    else
+    {
+        be = -asin(s[6]);
+        al = atan2(s[3] / cos(be), s[0] / cos(be));
+        de = atan2(s[7] / cos(be), s[8] / cos(be));
+    }
+    al /= 2;
+    be /= 2;
+    de /= 2;
+    // exp(-i a sz) exp(-i b sy) exp(-i d sx)
+    const z64 sd[4] = {zc(cos(de), 0), zc(0, -sin(de)), zc(0, -sin(de)), zc(cos(de), 0)};
+    const z64 sb[4] = {zc(cos(be), 0), zc(-sin(be), 0), zc(sin(be), 0), zc(cos(be), 0)};
+    const z64 sa[4] = {zc(cos(al), -sin(al)), zc(0, 0), zc(0, 0), zc(cos(al), sin(al))};
+    z64 t[4];
+    t[0] = cadd(cmul(sb[0], sd[0]), cmul(sb[1], sd[2]));
+    t[1] = cadd(cmul(sb[0], sd[1]), cmul(sb[1], sd[3]));
+    t[2] = cadd(cmul(sb[2], sd[0]), cmul(sb[3], sd[2]));
+    t[3] = cadd(cmul(sb[2], sd[1]), cmul(sb[3], sd[3]));
+    su[0] = cadd(cmul(sa[0], t[0]), cmul(sa[1], t[2]));
+    su[1] = cadd(cmul(sa[0], t[1]), cmul(sa[1], t[3]));
+    su[2] = cadd(cmul(sa[2], t[0]), cmul(sa[3], t[2]));
+    su[3] = cadd(cmul(sa[2], t[1]), cmul(sa[3], t[3]));
+    if (time_rev)
+    {
+        const z64 a1 = su[0], b1 = su[1], c1 = su[2], d1 = su[3];
+        su[0] = zc(-c1.x, -c1.y);
+        su[1] = zc(-d1.x, -d1.y);
+        su[2] = a1;
+        su[3] = b1;
+    }
+    return LELPH_OK;
+}
+
+// geometry of elphLocal (src/dvloc/compute_Vbare_loc.c:93-154)
+static int local_geometry(const lelph_ctx* c, const double* qpt, int ikq, int ik, int kqsym,
+                          int ksym, SideGeom* gkq, SideGeom* gk)
+{
+    const double* lat = c->lat.alat_vec;
+    const double* blat = c->lat.blat_vec;
+    const double* kqv = c->kpt_ibz.data() + 3 * ikq;
+    const double* kv = c->kpt_ibz.data() + 3 * ik;
+    const lelph_symmetry& Skq = c->syms[kqsym];
+    const lelph_symmetry& Sk = c->syms[ksym];
+    double tSkq[3], tSk[3], kSk[3], kSkq[3], qc[3], ulm[3];
+    matvec3(Skq.Rmat, kqv, false, tSkq);
+    matvec3(Sk.Rmat, kv, false, tSk);
+    matvec3(lat, tSk, true, kSk);
+    for (int i = 0; i < 3; ++i) kSkq[i] = kSk[i] + qpt[i];
+    for (int i = 0; i < 3; ++i) tSkq[i] -= tSk[i];
+    matvec3(blat, qpt, false, qc);
+    for (int i = 0; i < 3; ++i) qc[i] /= (2 * LELPH_RPI);
+    for (int i = 0; i < 3; ++i) ulm[i] = tSkq[i] - qc[i];
+    matvec3(blat, Skq.tau, true, gkq->tauc);
+    matvec3(blat, Sk.tau, true, gk->tauc);
+    for (int i = 0; i < 3; ++i)
+    {
+        gkq->tauc[i] /= (2 * LELPH_RPI);
+        gk->tauc[i] /= (2 * LELPH_RPI);
+    }
+    // rotateGvecs: M = lat^T S, G0 -> lat^T G0 (src/wfc/rotate_Gvecs.c:94-115)
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            gkq->M[3 * i + j] = lat[i] * Skq.Rmat[j] + lat[3 + i] * Skq.Rmat[3 + j] +
+                                lat[6 + i] * Skq.Rmat[6 + j];
+            gk->M[3 * i + j] =
+                lat[i] * Sk.Rmat[j] + lat[3 + i] * Sk.Rmat[3 + j] + lat[6 + i] * Sk.Rmat[6 + j];
+        }
+    matvec3(lat, ulm, true, gkq->G0c);
+    gk->G0c[0] = gk->G0c[1] = gk->G0c[2] = 0;
+    for (int i = 0; i < 3; ++i)
+    {
+        gkq->kc[i] = kSkq[i];
+        gk->kc[i] = kSk[i];
+    }
+    gkq->conj = Skq.time_rev ? 1 : 0;
+    gk->conj = Sk.time_rev ? 1 : 0;
+    LELPH_TRY(su2mat(Skq.Rmat, c->lat.nspinor, Skq.time_rev, gkq->su2));
+    LELPH_TRY(su2mat(Sk.Rmat, c->lat.nspinor, Sk.time_rev, gk->su2));
+    return LELPH_OK;
+}
+
+template <typename T>
+static cpx<T>* dv_ptr(lelph_ctx* c);
+template <>
+z64* dv_ptr<double>(lelph_ctx* c)
+{
+    return c->dv64.as<z64>();
+}
+template <>
+c32* dv_ptr<float>(lelph_ctx* c)
+{
+    return c->dv32.as<c32>();
+}
+
+static int ensure_dv32(lelph_ctx* c)
+{
+    if (c->dv32_valid) return LELPH_OK;
+    const int64_t n = (int64_t)c->lat.nmodes * c->lat.nmag * c->nf;
+    LELPH_TRY(c->dv32.ensure(sizeof(c32) * n));
+    hipLaunchKernelGGL(k_z64_to_c32, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, c->stream,
+                       c->dv64.as<z64>(), c->dv32.as<c32>(), n);
+    HIP_TRY(hipGetLastError());
+    c->dv32_valid = true;
+    return LELPH_OK;
+}
+
+// psi_k(r), psi_kq(r) boxes: scatter + backward FFT
+template <typename T>
+static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
+{
+    const lelph_lattice& L = c->lat;
+    if (pr.ik < 0 || pr.ikq < 0 || pr.ik >= (int)c->wf.size() || pr.ikq >= (int)c->wf.size())
+        return LELPH_ERR_ARG;
+    if (pr.ksym < 0 || pr.kqsym < 0 || pr.ksym >= L.nsym || pr.kqsym >= L.nsym) return LELPH_ERR_ARG;
+    const DevWfc& wk = c->wf[pr.ik];
+    const DevWfc& wkq = c->wf[pr.ikq];
+    if (!wk.wfc || !wkq.wfc) return LELPH_ERR_STATE;
+    SideGeom gkq, gk;
+    LELPH_TRY(local_geometry(c, qpt, pr.ikq, pr.ik, pr.kqsym, pr.ksym, &gkq, &gk));
+    const int64_t nsb = (int64_t)L.nspin * L.nbnds;
+    const int64_t nbox = nsb * L.nspinor;
+    const size_t bb = sizeof(cpx<T>) * (size_t)(nbox * c->nf);
+    LELPH_TRY(c->box_k.ensure(bb));
+    LELPH_TRY(c->box_kq.ensure(bb));
+    LELPH_TRY(c->idx_k.ensure(sizeof(int) * c->npw_max));
+    LELPH_TRY(c->idx_kq.ensure(sizeof(int) * c->npw_max));
+    LELPH_TRY(c->ph_k.ensure(sizeof(z64) * c->npw_max));
+    LELPH_TRY(c->ph_kq.ensure(sizeof(z64) * c->npw_max));
+    HIP_TRY(hipMemsetAsync(c->box_k.p, 0, bb, c->stream));
+    HIP_TRY(hipMemsetAsync(c->box_kq.p, 0, bb, c->stream));
+    const int Nx = (int)L.fft_dims[0], Ny = (int)L.fft_dims[1], Nz = (int)L.fft_dims[2];
+    hipLaunchKernelGGL(k_gindex, dim3((unsigned)cdiv(wk.npw, 256)), dim3(256), 0, c->stream,
+                       wk.gvec, wk.npw, gk, Nx, Ny, Nz, c->idx_k.as<int>(), c->ph_k.as<z64>(),
+                       c->d_err);
+    hipLaunchKernelGGL(k_gindex, dim3((unsigned)cdiv(wkq.npw, 256)), dim3(256), 0, c->stream,
+                       wkq.gvec, wkq.npw, gkq, Nx, Ny, Nz, c->idx_kq.as<int>(),
+                       c->ph_kq.as<z64>(), c->d_err);
+    hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)cdiv(nsb * wk.npw, 256)), dim3(256), 0,
+                       c->stream, wk.wfc, nsb, L.nspinor, wk.npw, c->idx_k.as<int>(),
+                       c->ph_k.as<z64>(), gk, c->nf, c->box_k.as<cpx<T>>());
+    hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)cdiv(nsb * wkq.npw, 256)), dim3(256), 0,
+                       c->stream, wkq.wfc, nsb, L.nspinor, wkq.npw, c->idx_kq.as<int>(),
+                       c->ph_kq.as<z64>(), gkq, c->nf, c->box_kq.as<cpx<T>>());
+    HIP_TRY(hipGetLastError());
+    LELPH_TRY(fft3d_batch<T>(c->ax, c->box_k.as<cpx<T>>(), nbox, +1, c->stream));
+    LELPH_TRY(fft3d_batch<T>(c->ax, c->box_kq.as<cpx<T>>(), nbox, +1, c->stream));
+    return LELPH_OK;
+}
+
+static int pick_splits(int64_t base_blocks, int64_t nchunks, int64_t out_elems, int64_t* cps,
+                       int* nsplit)
+{
+    int64_t ns = cdiv(2048, base_blocks > 0 ? base_blocks : 1);
+    if (ns < 1) ns = 1;
+    // keep the partial buffer below ~512 MB
+    const int64_t cap = std::max<int64_t>(1, (512ll << 20) / (16 * std::max<int64_t>(out_elems, 1)));
+    ns = std::min(ns, cap);
+    ns = std::min(ns, nchunks);
+    if (ns < 1) ns = 1;
+    *cps = cdiv(nchunks, ns);
+    *nsplit = (int)cdiv(nchunks, *cps);
+    return LELPH_OK;
+}
+
+// local part into g (device): elphLocal
+template <typename T>
+static int run_local(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g)
+{
+    const lelph_lattice& L = c->lat;
+    if (!c->dv_set) return LELPH_ERR_STATE;
+    if (L.nmag == 2 && (L.nspin != 2 || L.nspinor != 1)) return LELPH_ERR_ARG;
+    LELPH_TRY(build_boxes<T>(c, qpt, pr));
+    const int nm = L.nmodes, nb = L.nbnds, nspin = L.nspin;
+    const int nmag_eff = (L.nspinor == 1 && L.nmag == 4) ? 1 : L.nmag;
+    const int64_t nchunks = cdiv(c->nf, CT_KC);
+    const int64_t tiles = (int64_t)cdiv(nm, CT_VB) * cdiv(nb, CT_NB) * cdiv(nb, CT_MB);
+    const int64_t gsz = (int64_t)nm * nspin * nb * nb;
+    int64_t cps;
+    int nsplit;
+    pick_splits(tiles * nspin, nchunks, gsz, &cps, &nsplit);
+    LELPH_TRY(c->part.ensure(sizeof(z64) * (size_t)(nsplit * gsz)));
+    hipLaunchKernelGGL(k_contract<T>, dim3((unsigned)tiles, (unsigned)nsplit, (unsigned)nspin),
+                       dim3(256), 0, c->stream, c->box_k.as<cpx<T>>(), c->box_kq.as<cpx<T>>(),
+                       dv_ptr<T>(c), c->part.as<z64>(), nm, nb, nspin, L.nspinor, nmag_eff, c->nf,
+                       cps);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)cdiv(gsz, 256)), dim3(256), 0, c->stream,
+                       c->part.as<z64>(), nsplit, gsz, 1.0 / (double)c->nf, g);
+    HIP_TRY(hipGetLastError());
+    return LELPH_OK;
+}
+
+// non-local part added onto g (device): add_elphNonLocal
+static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g)
+{
+    const lelph_lattice& L = c->lat;
+    if (!c->dv_set) return LELPH_ERR_STATE;
+    const DevWfc& wK = c->wf[pr.ikq];
+    const DevWfc& wKp = c->wf[pr.ik];
+    if (!wK.Fk || !wKp.Fk) return LELPH_ERR_STATE;
+    const int nlm = (c->lmax + 1) * (c->lmax + 1);
+    const int64_t nsb = (int64_t)L.nspin * L.nbnds;
+    const int64_t ncol = nsb * L.nspinor;
+    const int natom = L.natom;
+    const int rows = natom * c->pmax * 4;
+    const int64_t npm = c->npw_max;
+    LELPH_TRY(c->nlK.ensure(sizeof(double) * 3 * npm));
+    LELPH_TRY(c->nlKp.ensure(sizeof(double) * 3 * npm));
+    LELPH_TRY(c->Y_K.ensure(sizeof(double) * nlm * npm));
+    LELPH_TRY(c->Y_Kp.ensure(sizeof(double) * nlm * npm));
+    LELPH_TRY(c->wr_K.ensure(sizeof(z64) * ncol * npm));
+    LELPH_TRY(c->wr_Kp.ensure(sizeof(z64) * ncol * npm));
+    LELPH_TRY(c->beta_K.ensure(sizeof(z64) * (size_t)rows * npm));
+    LELPH_TRY(c->beta_Kp.ensure(sizeof(z64) * (size_t)rows * npm));
+    LELPH_TRY(c->bb_K.ensure(sizeof(z64) * (size_t)rows * ncol));
+    LELPH_TRY(c->bb_Kp.ensure(sizeof(z64) * (size_t)rows * ncol));
+    const int64_t es = (int64_t)L.nspin * L.nbnds * L.nbnds;
+    LELPH_TRY(c->ebuf.ensure(sizeof(z64) * (size_t)(3 * natom * es)));
+
+    const lelph_symmetry& SK = c->syms[pr.kqsym];
+    const lelph_symmetry& SKp = c->syms[pr.ksym];
+    NlSide sK{}, sKp{};
+    std::memcpy(sK.S, SK.Rmat, sizeof(sK.S));
+    std::memcpy(sKp.S, SKp.Rmat, sizeof(sKp.S));
+    std::memcpy(sK.k, c->kpt_ibz.data() + 3 * pr.ikq, sizeof(sK.k));
+    std::memcpy(sKp.k, c->kpt_ibz.data() + 3 * pr.ik, sizeof(sKp.k));
+    std::memcpy(sK.tau, SK.tau, sizeof(sK.tau));
+    std::memcpy(sKp.tau, SKp.tau, sizeof(sKp.tau));
+    sK.conj_w = SK.time_rev ? 0 : 1;   // 'C' unless time reversed (Vbare_nloc.c:397-407)
+    sKp.conj_w = SKp.time_rev ? 1 : 0; // 'T' unless time reversed
+    LELPH_TRY(su2mat(SK.Rmat, L.nspinor, SK.time_rev, sK.su2));
+    LELPH_TRY(su2mat(SKp.Rmat, L.nspinor, SKp.time_rev, sKp.su2));
+
+    hipLaunchKernelGGL(k_nl_geom, dim3((unsigned)cdiv(wK.npw, 128)), dim3(128), 0, c->stream,
+                       wK.gvec, wK.npw, sK, c->lmax, wK.wfc, nsb, L.nspinor, c->nlK.as<double>(),
+                       c->Y_K.as<double>(), c->wr_K.as<z64>());
+    hipLaunchKernelGGL(k_nl_geom, dim3((unsigned)cdiv(wKp.npw, 128)), dim3(128), 0, c->stream,
+                       wKp.gvec, wKp.npw, sKp, c->lmax, wKp.wfc, nsb, L.nspinor,
+                       c->nlKp.as<double>(), c->Y_Kp.as<double>(), c->wr_Kp.as<z64>());
+    NlProj pj{natom, c->pmax, c->ntype};
+    hipLaunchKernelGGL(k_nl_beta<double>, dim3((unsigned)cdiv(wK.npw, 256), natom * c->pmax),
+                       dim3(256), 0, c->stream, c->nlK.as<double>(), c->Y_K.as<double>(), wK.Fk,
+                       wK.npw, pj, c->d_plist, c->d_atype, c->d_apos, c->d_Fsign, -1,
+                       c->beta_K.as<z64>());
+    hipLaunchKernelGGL(k_nl_beta<double>, dim3((unsigned)cdiv(wKp.npw, 256), natom * c->pmax),
+                       dim3(256), 0, c->stream, c->nlKp.as<double>(), c->Y_Kp.as<double>(),
+                       wKp.Fk, wKp.npw, pj, c->d_plist, c->d_atype, c->d_apos, c->d_Fsign, +1,
+                       c->beta_Kp.as<z64>());
+    HIP_TRY(hipGetLastError());
+    // projector GEMMs
+    for (int side = 0; side < 2; ++side)
+    {
+        const DevWfc& w = side == 0 ? wK : wKp;
+        const z64* A = side == 0 ? c->beta_K.as<z64>() : c->beta_Kp.as<z64>();
+        const z64* B = side == 0 ? c->wr_K.as<z64>() : c->wr_Kp.as<z64>();
+        z64* out = side == 0 ? c->bb_K.as<z64>() : c->bb_Kp.as<z64>();
+        const int conjB = side == 0 ? sK.conj_w : sKp.conj_w;
+        const int64_t nchunks = cdiv(w.npw, CT_KC);
+        const int64_t tiles = cdiv(rows, GZ_MB) * cdiv(ncol, GZ_NB);
+        int64_t cps;
+        int nsplit;
+        pick_splits(tiles, nchunks, (int64_t)rows * ncol, &cps, &nsplit);
+        LELPH_TRY(c->bbpart.ensure(sizeof(z64) * (size_t)nsplit * rows * ncol));
+        hipLaunchKernelGGL(k_zgemm_nt<double>, dim3((unsigned)tiles, (unsigned)nsplit), dim3(256),
+                           0, c->stream, A, B, rows, (int)ncol, w.npw, conjB, cps,
+                           c->bbpart.as<z64>());
+        hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)cdiv((int64_t)rows * ncol, 256)),
+                           dim3(256), 0, c->stream, c->bbpart.as<z64>(), nsplit,
+                           (int64_t)rows * ncol, 1.0, out);
+        HIP_TRY(hipGetLastError());
+    }
+    const int64_t ne = (int64_t)natom * 3 * es;
+    hipLaunchKernelGGL(k_nl_kkp, dim3((unsigned)cdiv(ne, 256)), dim3(256), 0, c->stream,
+                       c->bb_K.as<z64>(), c->bb_Kp.as<z64>(), pj, c->d_plist, c->d_atype,
+                       c->d_socfac, c->d_fco, c->has_fco ? c->d_fco_off : nullptr, L.nspin,
+                       L.nbnds, L.nspinor, c->ebuf.as<z64>());
+    const z64 pre = {0.0, -2 * LELPH_RPI * LELPH_E2};
+    hipLaunchKernelGGL(k_nl_mode, dim3((unsigned)cdiv((int64_t)L.nmodes * es, 256)), dim3(256), 0,
+                       c->stream, c->ebuf.as<z64>(), c->eig_d.as<z64>(), L.nmodes, 3 * natom, es,
+                       pre, g);
+    HIP_TRY(hipGetLastError());
+    return LELPH_OK;
+}
+
+static int run_local_split(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g)
+{
+    (void)c;
+    (void)qpt;
+    (void)pr;
+    (void)g;
+    return LELPH_ERR_UNSUPPORTED;
+}
+
+static int run_local_any(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g)
+{
+    switch (c->precision)
+    {
+        case LELPH_FP64: return run_local<double>(c, qpt, pr, g);
+        case LELPH_FP32:
+            LELPH_TRY(ensure_dv32(c));
+            return run_local<float>(c, qpt, pr, g);
+        case LELPH_BF16X3:
+        case LELPH_BF16X6:
+            LELPH_TRY(ensure_dv32(c));
+            return run_local_split(c, qpt, pr, g);
+        default: return LELPH_ERR_ARG;
+    }
+}
+
+static int check_err(lelph_ctx* c)
+{
+    int e = 0;
+    HIP_TRY(hipMemcpyAsync(&e, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (e)
+    {
+        HIP_TRY(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+        return LELPH_ERR_RANGE;
+    }
+    return LELPH_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* C-ABI                                                                     */
+/* ------------------------------------------------------------------------ */
+
+extern "C" {
+
+const char* lelph_strerror(int s)
+{
+    switch (s)
+    {
+        case LELPH_OK: return "ok";
+        case LELPH_ERR_ARG: return "bad argument";
+        case LELPH_ERR_ALLOC: return "allocation failed";
+        case LELPH_ERR_HIP: return "HIP runtime error";
+        case LELPH_ERR_RANGE: return "G vector outside the FFT box or table range";
+        case LELPH_ERR_UNSUPPORTED: return "unsupported configuration";
+        case LELPH_ERR_STATE: return "call order violated (missing wfc or dV)";
+        case LELPH_ERR_NOKQ:
+            return "K+Q point cannot be found in kgrid. The k-grid and q-grid are not "
+                   "commensurate";
+        case LELPH_ERR_SYM: return "Wrong symmetric matrices";
+        default: return "unknown status";
+    }
+}
+
+int lelph_get_device_count(int* count)
+{
+    if (!count) return LELPH_ERR_ARG;
+    HIP_TRY(hipGetDeviceCount(count));
+    return LELPH_OK;
+}
+
+static void free_ctx(lelph_ctx* c)
+{
+    if (!c) return;
+    for (auto& w : c->wf)
+    {
+        if (w.wfc) (void)hipFree(w.wfc);
+        if (w.gvec) (void)hipFree(w.gvec);
+        if (w.Fk) (void)hipFree(w.Fk);
+    }
+    for (int i = 0; i < 3; ++i) fft_axis_free(&c->ax[i]);
+    void* ptrs[] = {c->d_atype, c->d_apos, c->d_Fsign, c->d_socfac, c->d_Zval, c->d_plist,
+                    c->d_fco,   c->d_fco_off, c->d_err, c->d_gco,  c->d_vlocg, c->d_vploc};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    DBuf* bufs[] = {&c->dv64, &c->dv32, &c->eig_d, &c->vloc_tmp, &c->rot_tmp, &c->idx_k,
+                    &c->idx_kq, &c->ph_k, &c->ph_kq, &c->box_k, &c->box_kq, &c->part, &c->g,
+                    &c->gbatch, &c->nlK, &c->nlKp, &c->Y_K, &c->Y_Kp, &c->wr_K, &c->wr_Kp,
+                    &c->beta_K, &c->beta_Kp, &c->bb_K, &c->bb_Kp, &c->bbpart, &c->ebuf};
+    for (DBuf* b : bufs) b->release();
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+void lelph_destroy(lelph_ctx* c) { free_ctx(c); }
+
+extern "C++" {
+template <typename X>
+static int upload(X** d, const X* h, size_t n)
+{
+    HIP_TRY(hipMalloc(d, sizeof(X) * (n > 0 ? n : 1)));
+    if (n) HIP_TRY(hipMemcpy(*d, h, sizeof(X) * n, hipMemcpyHostToDevice));
+    return LELPH_OK;
+}
+}
+
+static int create_impl(const lelph_lattice* L, const lelph_pseudo* P, int precision,
+                       lelph_ctx* c)
+{
+    if (L->natom < 1 || L->nbnds < 1 || L->nmodes < 1 || L->nsym < 1) return LELPH_ERR_ARG;
+    if (L->nspin < 1 || L->nspin > 2 || L->nspinor < 1 || L->nspinor > 2) return LELPH_ERR_ARG;
+    if (L->nmag != 1 && L->nmag != 2 && L->nmag != 4) return LELPH_ERR_ARG;
+    if (L->natom > DVLOC_MAXATOM || P->ntype > DVLOC_MAXTYPE) return LELPH_ERR_UNSUPPORTED;
+    if (precision < LELPH_FP64 || precision > LELPH_BF16X6) return LELPH_ERR_ARG;
+    c->precision = precision;
+    HIP_TRY(hipGetDevice(&c->device));
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
+    c->lat = *L;
+    c->atomic_pos.assign(L->atomic_pos, L->atomic_pos + 3 * L->natom);
+    c->atom_type.assign(L->atom_type, L->atom_type + L->natom);
+    c->kpt_ibz.assign(L->kpt_iredBZ, L->kpt_iredBZ + 3 * L->nkpts_iBZ);
+    if (L->nkpts_BZ > 0 && L->kpt_fullBZ_crys && L->kmap)
+    {
+        c->kpt_bz.assign(L->kpt_fullBZ_crys, L->kpt_fullBZ_crys + 3 * L->nkpts_BZ);
+        c->kmap.assign(L->kmap, L->kmap + 2 * L->nkpts_BZ);
+    }
+    c->syms.assign(L->syms, L->syms + L->nsym);
+    c->lat.atomic_pos = c->atomic_pos.data();
+    c->lat.atom_type = c->atom_type.data();
+    c->lat.kpt_iredBZ = c->kpt_ibz.data();
+    c->lat.kpt_fullBZ_crys = c->kpt_bz.empty() ? nullptr : c->kpt_bz.data();
+    c->lat.kmap = c->kmap.empty() ? nullptr : c->kmap.data();
+    c->lat.syms = c->syms.data();
+    for (int i = 0; i < L->natom; ++i)
+        if (L->atom_type[i] < 0 || L->atom_type[i] >= P->ntype) return LELPH_ERR_ARG;
+
+    c->ntype = P->ntype;
+    c->lmax = P->lmax;
+    c->nlj = P->nltimesj;
+    c->PP.assign(P->PP_table, P->PP_table + 3 * P->nltimesj * P->ntype);
+    c->Fsign.assign(P->Fsign, P->Fsign + P->nltimesj * P->ntype);
+    c->qmax_abs = P->qmax_abs;
+    for (int t = 0; t < P->ntype; ++t)
+    {
+        const lelph_local_pseudo& lp = P->loc_pseudo[t];
+        c->locV.emplace_back(lp.Vloc_atomic, lp.Vloc_atomic + lp.ngrid);
+        c->locR.emplace_back(lp.r_grid, lp.r_grid + lp.ngrid);
+        c->locRab.emplace_back(lp.rab_grid, lp.rab_grid + lp.ngrid);
+        c->Zval.push_back(lp.Zval);
+    }
+    // projector lists and j-average factors (Vbare_nloc.c:318-395, 472-491)
+    c->socfac.assign(P->nltimesj * P->ntype, 1.0);
+    c->fco_off.assign(P->nltimesj * P->ntype, -1);
+    int pmax = 0;
+    for (int a = 0; a < L->natom; ++a)
+    {
+        const int t = L->atom_type[a];
+        int np = 0;
+        for (int64_t li = 0; li < P->nltimesj; ++li)
+        {
+            const int l = (int)rint(P->PP_table[P->ntype * 3 * li + t * 3] - 1);
+            if (l < 0) continue;
+            if (l > P->lmax) return LELPH_ERR_ARG;
+            np += 2 * l + 1;
+        }
+        pmax = std::max(pmax, np);
+    }
+    c->pmax = std::max(pmax, 1);
+    c->plist.assign((size_t)L->natom * c->pmax * 3, -1);
+    for (int a = 0; a < L->natom; ++a)
+    {
+        const int t = L->atom_type[a];
+        int p = 0;
+        for (int64_t li = 0; li < P->nltimesj; ++li)
+        {
+            const int l = (int)rint(P->PP_table[P->ntype * 3 * li + t * 3] - 1);
+            if (l < 0) continue;
+            for (int m = 0; m <= 2 * l; ++m, ++p)
+            {
+                int* e = &c->plist[3 * ((size_t)a * c->pmax + p)];
+                e[0] = (int)li;
+                e[1] = l;
+                e[2] = l * l + m;
+            }
+        }
+    }
+    for (int64_t li = 0; li < P->nltimesj; ++li)
+        for (int t = 0; t < P->ntype; ++t)
+        {
+            const int l = (int)rint(P->PP_table[P->ntype * 3 * li + t * 3] - 1);
+            const int j = (int)rint(P->PP_table[P->ntype * 3 * li + t * 3 + 1]);
+            if (l >= 0 && j != 0 && !L->is_soc_present)
+                c->socfac[li * P->ntype + t] = (j + 1.0) / (4.0 * l + 2.0);
+            if (P->fCoeff && l >= 0 && P->fCoeff[P->ntype * li + t])
+            {
+                const int n = (2 * l + 1) * (2 * l + 1) * L->nspinor * L->nspinor;
+                c->fco_off[li * P->ntype + t] = (int64_t)c->fco.size();
+                const double* f = P->fCoeff[P->ntype * li + t];
+                for (int i = 0; i < n; ++i) c->fco.push_back(z64{f[2 * i], f[2 * i + 1]});
+                c->has_fco = true;
+            }
+        }
+    LELPH_TRY(upload(&c->d_atype, c->atom_type.data(), c->atom_type.size()));
+    LELPH_TRY(upload(&c->d_apos, c->atomic_pos.data(), c->atomic_pos.size()));
+    LELPH_TRY(upload(&c->d_Fsign, c->Fsign.data(), c->Fsign.size()));
+    LELPH_TRY(upload(&c->d_socfac, c->socfac.data(), c->socfac.size()));
+    LELPH_TRY(upload(&c->d_Zval, c->Zval.data(), c->Zval.size()));
+    LELPH_TRY(upload(&c->d_plist, c->plist.data(), c->plist.size()));
+    LELPH_TRY(upload(&c->d_fco, c->fco.data(), c->fco.size()));
+    LELPH_TRY(upload(&c->d_fco_off, c->fco_off.data(), c->fco_off.size()));
+    HIP_TRY(hipMalloc(&c->d_err, sizeof(int)));
+    HIP_TRY(hipMemset(c->d_err, 0, sizeof(int)));
+
+    for (int i = 0; i < 3; ++i)
+    {
+        if (L->fft_dims[i] < 1 || L->fft_dims[i] > 4096) return LELPH_ERR_ARG;
+        LELPH_TRY(fft_axis_init(&c->ax[i], (int)L->fft_dims[i]));
+    }
+    c->nf = L->fft_dims[0] * L->fft_dims[1] * L->fft_dims[2];
+    c->wf.resize(L->nkpts_iBZ);
+    return LELPH_OK;
+}
+
+int lelph_create(const lelph_lattice* L, const lelph_pseudo* P, int precision, lelph_ctx** out)
+{
+    if (!L || !P || !out) return LELPH_ERR_ARG;
+    *out = nullptr;
+    lelph_ctx* c = new (std::nothrow) lelph_ctx();
+    if (!c) return LELPH_ERR_ALLOC;
+    int s = create_impl(L, P, precision, c);
+    if (s != LELPH_OK)
+    {
+        free_ctx(c);
+        return s;
+    }
+    *out = c;
+    return LELPH_OK;
+}
+
+int lelph_set_precision(lelph_ctx* c, int p)
+{
+    if (!c || p < LELPH_FP64 || p > LELPH_BF16X6) return LELPH_ERR_ARG;
+    c->precision = p;
+    return LELPH_OK;
+}
+
+int lelph_upload_wfc(lelph_ctx* c, int64_t ik, const double* wfc, const double* gvec,
+                     const double* Fk, int64_t npw)
+{
+    if (!c || ik < 0 || ik >= (int64_t)c->wf.size() || !wfc || !gvec || npw < 1)
+        return LELPH_ERR_ARG;
+    DevWfc& w = c->wf[ik];
+    if (w.wfc) (void)hipFree(w.wfc);
+    if (w.gvec) (void)hipFree(w.gvec);
+    if (w.Fk) (void)hipFree(w.Fk);
+    w = DevWfc{};
+    const int64_t nset = (int64_t)c->lat.nspin * c->lat.nbnds * c->lat.nspinor;
+    w.npw = npw;
+    HIP_TRY(hipMalloc(&w.wfc, sizeof(z64) * nset * npw));
+    HIP_TRY(hipMemcpy(w.wfc, wfc, sizeof(z64) * nset * npw, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&w.gvec, sizeof(double) * 3 * npw));
+    HIP_TRY(hipMemcpy(w.gvec, gvec, sizeof(double) * 3 * npw, hipMemcpyHostToDevice));
+    if (Fk)
+    {
+        const size_t n = (size_t)c->nlj * c->ntype * npw;
+        HIP_TRY(hipMalloc(&w.Fk, sizeof(double) * n));
+        HIP_TRY(hipMemcpy(w.Fk, Fk, sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    c->npw_max = std::max(c->npw_max, npw);
+    return LELPH_OK;
+}
+
+// prepare_spline (src/common/spline.c:26-139): not-a-knot derivatives
+static void prepare_spline(int64_t n, const double* x, const double* y, double* dy)
+{
+    std::vector<double> A(n), B(n), Cc(n), D(n);
+    double h1 = x[1] - x[0], s1 = (y[1] - y[0]) / h1;
+    double h2 = x[2] - x[1], s2 = (y[2] - y[1]) / h2;
+    A[0] = h2 * h2;
+    B[0] = h2 * h2 - h1 * h1;
+    Cc[0] = -h1 * h1;
+    D[0] = 2 * (s1 * h2 * h2 - s2 * h1 * h1);
+    h1 = x[n - 2] - x[n - 3];
+    s1 = (y[n - 2] - y[n - 3]) / h1;
+    h2 = x[n - 1] - x[n - 2];
+    s2 = (y[n - 1] - y[n - 2]) / h2;
+    A[n - 1] = h2 * h2;
+    B[n - 1] = h2 * h2 - h1 * h1;
+    Cc[n - 1] = -h1 * h1;
+    D[n - 1] = 2 * (s1 * h2 * h2 - s2 * h1 * h1);
+    for (int64_t i = 1; i < n - 1; ++i)
+    {
+        const double hi = x[i + 1] - x[i], hm = x[i] - x[i - 1];
+        const double si = (y[i + 1] - y[i]) / hi, sm = (y[i] - y[i - 1]) / hm;
+        A[i] = hi;
+        B[i] = 2.0 * (hi + hm);
+        Cc[i] = hm;
+        D[i] = 3 * (sm * hi + si * hm);
+    }
+    double al = -Cc[0] / Cc[1];
+    A[0] += al * A[1];
+    B[0] += al * B[1];
+    Cc[0] += al * Cc[1];
+    D[0] += al * D[1];
+    Cc[0] = B[0];
+    B[0] = A[0];
+    A[0] = 0;
+    al = -A[n - 1] / A[n - 2];
+    A[n - 1] += al * A[n - 2];
+    B[n - 1] += al * B[n - 2];
+    Cc[n - 1] += al * Cc[n - 2];
+    D[n - 1] += al * D[n - 2];
+    A[n - 1] = B[n - 1];
+    B[n - 1] = Cc[n - 1];
+    Cc[n - 1] = 0;
+    for (int64_t i = 1; i < n; ++i)
+    {
+        const double w = A[i] / B[i - 1];
+        B[i] -= w * Cc[i - 1];
+        D[i] -= w * D[i - 1];
+    }
+    dy[n - 1] = D[n - 1] / B[n - 1];
+    for (int64_t i = n - 2; i >= 0; --i) dy[i] = (D[i] - Cc[i] * dy[i + 1]) / B[i];
+}
+
+// Simpson weights (src/common/numerical_func.c:155-192), 1/3 included
+static std::vector<double> simpson_weights(int64_t n)
+{
+    std::vector<double> w(n, 0.0);
+    if (n < 3) return w;
+    w[0] = 1;
+    for (int64_t i = 1; i < n - 1; i += 2) w[i] += 4.0;
+    for (int64_t i = 2; i < n - 1; i += 2) w[i] += 2.0;
+    if (n % 2 != 0)
+        w[n - 1] += 1.0;
+    else
+    {
+        w[n - 3] -= 0.25;
+        w[n - 2] += 1.0;
+        w[n - 1] += 1.25;
+    }
+    for (auto& x : w) x /= 3.0;
+    return w;
+}
+
+int lelph_create_vlocg_table(lelph_ctx* c, int64_t* npts_out, double* g_co, double* vlocg,
+                             double* vploc)
+{
+    if (!c) return LELPH_ERR_ARG;
+    const double* b = c->lat.blat_vec;
+    double bi[3];
+    bi[0] = sqrt(b[0] * b[0] + b[3] * b[3] + b[6] * b[6]);
+    bi[1] = sqrt(b[1] * b[1] + b[4] * b[4] + b[7] * b[7]);
+    bi[2] = sqrt(b[2] * b[2] + b[5] * b[5] + b[8] * b[8]);
+    double bmin = bi[0], bmax = bi[0], Nmax = (double)c->lat.fft_dims[0];
+    for (int i = 0; i < 3; ++i)
+    {
+        bmin = std::min(bmin, bi[i]);
+        bmax = std::max(bmax, bi[i]);
+        Nmax = std::max(Nmax, (double)c->lat.fft_dims[i]);
+    }
+    if (bmin > 0.01) bmin = 0.01;
+    const double gmax = (1.7 * (Nmax + 4.0) + 3 * fabs(c->qmax_abs)) * bmax;
+    const int64_t npts = (int64_t)ceil(gmax / bmin);
+    if (npts < 4) return LELPH_ERR_ARG;
+    const double diff = gmax / (npts - 1.0);
+    c->h_gco.resize(npts);
+    for (int64_t i = 0; i < npts; ++i) c->h_gco[i] = 0.0 + i * diff;
+    c->npts = npts;
+    c->dg = diff;
+    c->h_vlocg.assign((size_t)npts * c->ntype, 0.0);
+    c->h_vploc.assign((size_t)npts * c->ntype, 0.0);
+    if (c->d_gco) (void)hipFree(c->d_gco);
+    if (c->d_vlocg) (void)hipFree(c->d_vlocg);
+    if (c->d_vploc) (void)hipFree(c->d_vploc);
+    c->d_gco = c->d_vlocg = c->d_vploc = nullptr;
+    LELPH_TRY(upload(&c->d_gco, c->h_gco.data(), (size_t)npts));
+    HIP_TRY(hipMalloc(&c->d_vlocg, sizeof(double) * npts * c->ntype));
+    for (int t = 0; t < c->ntype; ++t)
+    {
+        const int64_t ng = (int64_t)c->locR[t].size();
+        std::vector<double> w = simpson_weights(ng);
+        double *dV, *dr, *drab, *dw;
+        LELPH_TRY(upload(&dV, c->locV[t].data(), ng));
+        LELPH_TRY(upload(&dr, c->locR[t].data(), ng));
+        LELPH_TRY(upload(&drab, c->locRab[t].data(), ng));
+        LELPH_TRY(upload(&dw, w.data(), ng));
+        hipLaunchKernelGGL(k_vloc_table, dim3((unsigned)cdiv(npts, 4)), dim3(256), 0, c->stream,
+                           c->d_gco, npts, dV, dr, drab, dw, ng, c->Zval[t], c->lat.dimension,
+                           c->lat.volume, c->d_vlocg + (int64_t)t * npts);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        (void)hipFree(dV);
+        (void)hipFree(dr);
+        (void)hipFree(drab);
+        (void)hipFree(dw);
+    }
+    HIP_TRY(hipMemcpy(c->h_vlocg.data(), c->d_vlocg, sizeof(double) * npts * c->ntype,
+                      hipMemcpyDeviceToHost));
+    for (int t = 0; t < c->ntype; ++t)
+        prepare_spline(npts, c->h_gco.data(), c->h_vlocg.data() + (size_t)t * npts,
+                       c->h_vploc.data() + (size_t)t * npts);
+    LELPH_TRY(upload(&c->d_vploc, c->h_vploc.data(), (size_t)npts * c->ntype));
+    if (npts_out) *npts_out = npts;
+    if (g_co) std::memcpy(g_co, c->h_gco.data(), sizeof(double) * npts);
+    if (vlocg) std::memcpy(vlocg, c->h_vlocg.data(), sizeof(double) * npts * c->ntype);
+    if (vploc) std::memcpy(vploc, c->h_vploc.data(), sizeof(double) * npts * c->ntype);
+    return LELPH_OK;
+}
+
+static int set_eig(lelph_ctx* c, const double* eigVec)
+{
+    const int nm = c->lat.nmodes;
+    c->eig.resize((size_t)nm * nm);
+    std::memcpy(c->eig.data(), eigVec, sizeof(z64) * nm * nm);
+    LELPH_TRY(c->eig_d.ensure(sizeof(z64) * nm * nm));
+    HIP_TRY(hipMemcpyAsync(c->eig_d.p, c->eig.data(), sizeof(z64) * nm * nm,
+                           hipMemcpyHostToDevice, c->stream));
+    return LELPH_OK;
+}
+
+// dVlocq on the device; result in c->vloc_tmp (nm, nf)
+static int dvlocq_dev(lelph_ctx* c, const double* qpt)
+{
+    if (c->npts == 0) LELPH_TRY(lelph_create_vlocg_table(c, nullptr, nullptr, nullptr, nullptr));
+    const lelph_lattice& L = c->lat;
+    const int nm = L.nmodes;
+    LELPH_TRY(c->vloc_tmp.ensure(sizeof(z64) * nm * c->nf));
+    DvlocArgs a{};
+    std::memcpy(a.q, qpt, sizeof(a.q));
+    std::memcpy(a.blat, L.blat_vec, sizeof(a.blat));
+    a.latz = L.alat_vec[8];
+    a.volume = L.volume;
+    a.Nx = (int)L.fft_dims[0];
+    a.Ny = (int)L.fft_dims[1];
+    a.Nz = (int)L.fft_dims[2];
+    a.ntype = c->ntype;
+    a.natom = L.natom;
+    a.nm = nm;
+    a.cutoff = L.dimension;
+    a.npts = c->npts;
+    a.dg = c->dg;
+    hipLaunchKernelGGL(k_dvloc_G, dim3((unsigned)cdiv(c->nf, 128)), dim3(128), 0, c->stream, a,
+                       c->d_gco, c->d_vlocg, c->d_vploc, c->d_Zval, c->d_atype, c->d_apos,
+                       c->eig_d.as<z64>(), c->vloc_tmp.as<z64>(), c->d_err);
+    HIP_TRY(hipGetLastError());
+    LELPH_TRY(fft3d_batch<double>(c->ax, c->vloc_tmp.as<z64>(), nm, +1, c->stream));
+    return LELPH_OK;
+}
+
+int lelph_dvlocq(lelph_ctx* c, const double qpt[3], const double* eigVec, double* Vlocr)
+{
+    if (!c || !qpt || !eigVec) return LELPH_ERR_ARG;
+    LELPH_TRY(set_eig(c, eigVec));
+    LELPH_TRY(dvlocq_dev(c, qpt));
+    LELPH_TRY(check_err(c));
+    if (Vlocr)
+        HIP_TRY(hipMemcpy(Vlocr, c->vloc_tmp.p, sizeof(z64) * c->lat.nmodes * c->nf,
+                          hipMemcpyDeviceToHost));
+    return LELPH_OK;
+}
+
+int lelph_set_dvscf(lelph_ctx* c, const double qpt[3], const double* eigVec, const double* dvscf,
+                    const double* patterns, int kernel)
+{
+    if (!c || !qpt || !eigVec) return LELPH_ERR_ARG;
+    if (kernel < LELPH_KERNEL_DFPT || kernel > LELPH_KERNEL_BARE_LOCAL) return LELPH_ERR_ARG;
+    const lelph_lattice& L = c->lat;
+    const int nm = L.nmodes;
+    const int64_t ld = (int64_t)L.nmag * c->nf;
+    const size_t bytes = sizeof(z64) * (size_t)nm * ld;
+    c->dv_set = false;
+    c->dv32_valid = false;
+    LELPH_TRY(set_eig(c, eigVec));
+    LELPH_TRY(c->dv64.ensure(bytes));
+    const bool screened = (kernel == LELPH_KERNEL_DFPT || kernel == LELPH_KERNEL_DFPT_LOCAL);
+    if (screened && dvscf)
+    {
+        HIP_TRY(hipMemcpyAsync(c->dv64.p, dvscf, bytes, hipMemcpyHostToDevice, c->stream));
+        if (patterns)
+        {
+            // eig @ pats^H (read_dvscf.c:121-127), FP64 on the host (nm x nm)
+            std::vector<z64> ep((size_t)nm * nm);
+            const z64* pat = reinterpret_cast<const z64*>(patterns);
+            for (int i = 0; i < nm; ++i)
+                for (int j = 0; j < nm; ++j)
+                {
+                    z64 acc = {0, 0};
+                    for (int k = 0; k < nm; ++k)
+                        acc = cadd(acc, cmul(c->eig[(size_t)i * nm + k],
+                                             cconj(pat[(size_t)j * nm + k])));
+                    ep[(size_t)i * nm + j] = acc;
+                }
+            z64* dep;
+            LELPH_TRY(upload(&dep, ep.data(), ep.size()));
+            LELPH_TRY(c->rot_tmp.ensure(bytes));
+            hipLaunchKernelGGL(k_mode_rotate, dim3((unsigned)cdiv((int64_t)nm * ld, 256)),
+                               dim3(256), 0, c->stream, c->dv64.as<z64>(), dep, nm, ld,
+                               c->rot_tmp.as<z64>());
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(c->dv64.p, c->rot_tmp.p, bytes, hipMemcpyDeviceToDevice,
+                                   c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            (void)hipFree(dep);
+        }
+    }
+    else
+        HIP_TRY(hipMemsetAsync(c->dv64.p, 0, bytes, c->stream));
+    // every kernel adds the bare local part (set_kernel.c: bare_loc = true)
+    LELPH_TRY(dvlocq_dev(c, qpt));
+    hipLaunchKernelGGL(k_add_dvscf, dim3((unsigned)cdiv((int64_t)nm * c->nf, 256)), dim3(256), 0,
+                       c->stream, c->dv64.as<z64>(), c->vloc_tmp.as<z64>(), nm, L.nmag, c->nf);
+    HIP_TRY(hipGetLastError());
+    LELPH_TRY(check_err(c));
+    std::memcpy(c->q, qpt, sizeof(c->q));
+    c->dv_set = true;
+    return LELPH_OK;
+}
+
+int lelph_get_dvscf(lelph_ctx* c, double* dvscf)
+{
+    if (!c || !dvscf) return LELPH_ERR_ARG;
+    if (!c->dv_set) return LELPH_ERR_STATE;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(dvscf, c->dv64.p, sizeof(z64) * c->lat.nmodes * c->lat.nmag * c->nf,
+                      hipMemcpyDeviceToHost));
+    return LELPH_OK;
+}
+
+static int gkq_device(lelph_ctx* c, const double* qpt, const lelph_pair& pr, int local,
+                      int nonloc, z64* g)
+{
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    if (local)
+        LELPH_TRY(run_local_any(c, qpt, pr, g));
+    else
+        HIP_TRY(hipMemsetAsync(g, 0, sizeof(z64) * c->lat.nmodes * c->lat.nspin * c->lat.nbnds *
+                                         c->lat.nbnds,
+                               c->stream));
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    if (nonloc) LELPH_TRY(run_nonlocal(c, pr, g));
+    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    return LELPH_OK;
+}
+
+static int finish_timings(lelph_ctx* c)
+{
+    HIP_TRY(hipEventSynchronize(c->ev[2]));
+    float a = 0, b = 0;
+    HIP_TRY(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    c->timings[0] = 0;
+    c->timings[1] = a;
+    c->timings[2] = b;
+    c->timings[3] = a + b;
+    return LELPH_OK;
+}
+
+static int one_pair(lelph_ctx* c, const double* qpt, int ikq, int ik, int kqsym, int ksym,
+                    int local, int nonloc, double* host_g, int accumulate)
+{
+    if (!c || !qpt || !host_g) return LELPH_ERR_ARG;
+    const lelph_pair pr{ikq, ik, kqsym, ksym};
+    if (ik < 0 || ikq < 0 || ik >= (int)c->wf.size() || ikq >= (int)c->wf.size())
+        return LELPH_ERR_ARG;
+    if (ksym < 0 || kqsym < 0 || ksym >= c->lat.nsym || kqsym >= c->lat.nsym) return LELPH_ERR_ARG;
+    const int64_t gsz = (int64_t)c->lat.nmodes * c->lat.nspin * c->lat.nbnds * c->lat.nbnds;
+    LELPH_TRY(c->g.ensure(sizeof(z64) * gsz));
+    if (accumulate)
+        HIP_TRY(hipMemcpyAsync(c->g.p, host_g, sizeof(z64) * gsz, hipMemcpyHostToDevice,
+                               c->stream));
+    if (accumulate && !local)
+    {
+        HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+        HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+        LELPH_TRY(run_nonlocal(c, pr, c->g.as<z64>()));
+        HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    }
+    else
+        LELPH_TRY(gkq_device(c, qpt, pr, local, nonloc, c->g.as<z64>()));
+    HIP_TRY(hipMemcpyAsync(host_g, c->g.p, sizeof(z64) * gsz, hipMemcpyDeviceToHost, c->stream));
+    LELPH_TRY(check_err(c));
+    return finish_timings(c);
+}
+
+int lelph_elph_local(lelph_ctx* c, const double qpt[3], int ikq, int ik, int kqsym, int ksym,
+                     double* elph_kq)
+{
+    return one_pair(c, qpt, ikq, ik, kqsym, ksym, 1, 0, elph_kq, 0);
+}
+
+int lelph_add_elph_nonlocal(lelph_ctx* c, int ikq, int ik, int kqsym, int ksym,
+                            const double* eigVec, double* elph_kq)
+{
+    if (!c || !eigVec) return LELPH_ERR_ARG;
+    if (!c->dv_set) return LELPH_ERR_STATE;
+    LELPH_TRY(set_eig(c, eigVec));
+    return one_pair(c, c->q, ikq, ik, kqsym, ksym, 0, 1, elph_kq, 1);
+}
+
+int lelph_gkq(lelph_ctx* c, const double qpt[3], int ikq, int ik, int kqsym, int ksym, int non_loc,
+              double* elph_kq)
+{
+    return one_pair(c, qpt, ikq, ik, kqsym, ksym, 1, non_loc ? 1 : 0, elph_kq, 0);
+}
+
+int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lelph_pair* pairs,
+                    int non_loc, double* g_out)
+{
+    if (!c || !qpt || npairs < 0 || (npairs > 0 && (!pairs || !g_out))) return LELPH_ERR_ARG;
+    const int64_t gsz = (int64_t)c->lat.nmodes * c->lat.nspin * c->lat.nbnds * c->lat.nbnds;
+    LELPH_TRY(c->gbatch.ensure(sizeof(z64) * std::max<int64_t>(1, npairs) * gsz));
+    double tl = 0, tn = 0;
+    for (int64_t i = 0; i < npairs; ++i)
+    {
+        const lelph_pair& p = pairs[i];
+        if (p.ik < 0 || p.ikq < 0 || p.ik >= (int)c->wf.size() || p.ikq >= (int)c->wf.size())
+            return LELPH_ERR_ARG;
+        if (p.ksym < 0 || p.kqsym < 0 || p.ksym >= c->lat.nsym || p.kqsym >= c->lat.nsym)
+            return LELPH_ERR_ARG;
+        LELPH_TRY(gkq_device(c, qpt, p, 1, non_loc, c->gbatch.as<z64>() + i * gsz));
+        if (i + 1 == npairs)
+        {
+            LELPH_TRY(finish_timings(c));
+            tl += c->timings[1];
+            tn += c->timings[2];
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(g_out, c->gbatch.p, sizeof(z64) * npairs * gsz, hipMemcpyDeviceToHost,
+                           c->stream));
+    LELPH_TRY(check_err(c));
+    c->timings[1] = tl;
+    c->timings[2] = tn;
+    c->timings[3] = tl + tn;
+    return LELPH_OK;
+}
+
+int lelph_internal_grid(const lelph_ctx* c, int64_t* nbz, const double** kcrys, const int** kmap)
+{
+    if (!c || !nbz || !kcrys || !kmap) return LELPH_ERR_ARG;
+    if (c->kpt_bz.empty()) return LELPH_ERR_STATE;
+    *nbz = c->lat.nkpts_BZ;
+    *kcrys = c->kpt_bz.data();
+    *kmap = c->kmap.data();
+    return LELPH_OK;
+}
+
+int lelph_last_timings(lelph_ctx* c, double* ms4)
+{
+    if (!c || !ms4) return LELPH_ERR_ARG;
+    for (int i = 0; i < 4; ++i) ms4[i] = c->timings[i];
+    return LELPH_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,103 @@
+/*
+ * lelph_oracle.h -- FP64 CPU restatement of LetzElPhC's electron-phonon hot
+ * path.  TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the
+ * bench's cpu_baseline leg as the checker; never by the product path.
+ *
+ * Uses the structs of include/lelph_hip.h (the boundary types).  Every
+ * function names the reference file:line it restates.
+ */
+#ifndef LELPH_ORACLE_H
+#define LELPH_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/lelph_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* one iBZ wavefunction, struct WFC (src/common/dtypes.h:252-266) */
+typedef struct orc_wfc
+{
+    const double* wfc;  /* (nspin,nbnds,nspinor,npw) complex interleaved */
+    const double* gvec; /* (npw,3) cart, no 2pi                        */
+    const double* Fk;   /* (nltimesj,ntype,npw) or NULL                 */
+    int64_t npw;
+} orc_wfc;
+
+/* spline table of V_loc(|G|), struct Vloc_table (dtypes.h:190-210) */
+typedef struct orc_vloc_table
+{
+    int64_t npts;
+    double dg;
+    double* g_co;  /* (npts)        */
+    double* vlocg; /* (ntype,npts)  */
+    double* vploc; /* (ntype,npts)  */
+} orc_vloc_table;
+
+/* ---- numerical helpers (src/common/numerical_func.c, spline.c) ---- */
+double orc_legendre(int l, int m, double x);
+double orc_Ylm(int l, int m, const double* vec);
+double orc_simpson(const double* f, const double* dx, int64_t n);
+void orc_prepare_spline(int64_t n, const double* x, const double* y,
+                        double* dy);
+double orc_spline_interpolate(double x, int64_t inear, const double* xi,
+                              const double* yi, const double* dy);
+void orc_reciprocal_vecs(const double* lat, double* blat);
+int orc_SU2mat(const double* sym, int nspinor, int invert, int time_rev,
+               double* su2 /* 4 complex */);
+
+/* ---- FFT (own mixed radix; FFTW sign conventions) ---- */
+/* in-place 3-D DFT of a (n0,n1,n2) row-major box; sign -1 forward, +1
+ * backward, unnormalised (FFTW_FORWARD / FFTW_BACKWARD). */
+int orc_fft3d(double* box, const int64_t* dims, int sign);
+/* 1-D transform of n points (for tests). */
+int orc_fft1d(double* data, int64_t n, int sign);
+
+/* ---- V_loc table (src/dvloc/vlocg_table.c, VlocG_atom.c) ---- */
+double orc_Vloc_Gspace(double* work, char cutoff, double Gnorm, int64_t ngrid,
+                       const double* Vloc, const double* r, const double* rab,
+                       double Zval, double eta, double volume);
+int orc_create_vlocg_table(const lelph_lattice* lat, const lelph_pseudo* pp,
+                           orc_vloc_table* tab);
+void orc_free_vloc_table(orc_vloc_table* tab);
+
+/* ---- dVlocq / add_dvscf (src/dvloc/dVbare_loc.c, add_dvscf.c) ---- */
+int orc_dVlocq(const double* qpt, const lelph_lattice* lat,
+               const lelph_pseudo* pp, const orc_vloc_table* tab,
+               const double* eigVec, double* Vlocr);
+void orc_add_dvscf_qe(double* dVscf, const double* dVloc,
+                      const lelph_lattice* lat);
+/* pattern -> mode rotation of src/io/qe/read_dvscf.c:107-158 */
+void orc_dvscf_pattern_to_mode(double* dvscf, const double* eig,
+                               const double* pats, const lelph_lattice* lat);
+
+/* ---- the hot path ---- */
+int orc_elphLocal(const double* qpt, const orc_wfc* wfcs,
+                  const lelph_lattice* lat, int ikq, int ik, int kqsym,
+                  int ksym, const double* dVlocr, double* elph_kq);
+int orc_add_elphNonLocal(const orc_wfc* wfcs, const lelph_lattice* lat,
+                         const lelph_pseudo* pp, int ikq, int ik, int kqsym,
+                         int ksym, const double* eigVec, double* elph_kq);
+void orc_elph_q_rotate(const double* Dl, const double* g, const double* Dr,
+                       int nmodes, int nspin, int nb, int tim_rev,
+                       double* gSq);
+int orc_get_KplusQ_idxs(int64_t nbz, const double* kpts, const double* q,
+                        int* idx);
+/* the k loop of compute_and_write_elphq; OpenMP over k; g_out
+ * (nkpts_BZ,nmodes,nspin,nb,nb).  k_first/k_count restrict the k range. */
+int orc_compute_elphq(const double* qpt, const orc_wfc* wfcs,
+                      const lelph_lattice* lat, const lelph_pseudo* pp,
+                      const double* eigVec, const double* dVscf, int non_loc,
+                      int kminusq, int64_t k_first, int64_t k_count,
+                      double* g_out);
+/* sets the number of OpenMP threads used by orc_compute_elphq */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

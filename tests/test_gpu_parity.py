@@ -1,0 +1,176 @@
+"""GPU parity of the HIP path against the FP64 CPU oracle, through the C-ABI.
+
+Bars: FP64 path |dg|max/|g|max <= 1e-10; FP32 path <= 1e-6.
+"""
+import numpy as np
+import pytest
+
+from letzelphc_amd import lelph, synth
+from oracle import oracle as O
+
+from .util import rel_err, say
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+TOL32 = 1e-6
+
+
+@pytest.fixture(scope="module")
+def c0():
+    s = synth.config("c0")
+    o = O.OracleSystem(s)
+    O.set_threads(8)
+    dv = o.dv_total(0)
+    return s, o, dv
+
+
+@pytest.fixture(scope="module")
+def c0ctx(c0):
+    s, o, dv = c0
+    ctx = lelph.Context(s, "fp64")
+    ctx.set_dvscf(s.qpts[0], s.eigvec(0), s.dvscf(0), kernel="dfpt")
+    yield ctx
+    ctx.close()
+
+
+def test_vloc_table(c0, c0ctx):
+    s, o, _ = c0
+    say("case: V_loc(G) table c0")
+    g1, v1, d1 = c0ctx.vloc_table()
+    g0, v0, d0 = o.vloc_table()
+    assert g1.shape == g0.shape
+    assert rel_err(g1, g0) == 0.0
+    assert rel_err(v1, v0) < 1e-12
+    assert rel_err(d1, d0) < 1e-9
+
+
+def test_dvlocq(c0, c0ctx):
+    s, o, _ = c0
+    say("case: dVlocq c0 q0")
+    vg = c0ctx.dvlocq(s.qpts[0], s.eigvec(0))
+    vc = o.dvlocq(0)
+    assert rel_err(vg, vc) < TOL64
+
+
+def test_dvscf_total(c0, c0ctx):
+    s, o, dv = c0
+    say("case: dvscf + bare local c0")
+    c0ctx.set_dvscf(s.qpts[0], s.eigvec(0), s.dvscf(0), kernel="dfpt")
+    assert rel_err(c0ctx.get_dvscf(), dv) < TOL64
+
+
+@pytest.mark.parametrize("ip", [0, 1, 3, 17])
+def test_local_fp64(c0, c0ctx, ip):
+    s, o, dv = c0
+    pr = synth.kq_pairs(s, 0)[ip]
+    say(f"case: elph_local fp64 c0 pair {pr}")
+    g = c0ctx.elph_local(*pr)
+    ref = o.elph_local(0, *pr, dv)
+    assert rel_err(g, ref) < TOL64
+
+
+@pytest.mark.parametrize("ip", [0, 3, 17])
+def test_nonlocal_fp64(c0, c0ctx, ip):
+    s, o, dv = c0
+    pr = synth.kq_pairs(s, 0)[ip]
+    say(f"case: add_elph_nonlocal fp64 c0 pair {pr}")
+    base = o.elph_local(0, *pr, dv)
+    g = c0ctx.add_nonlocal(*pr, base)
+    ref = o.add_nonlocal(0, *pr, base)
+    assert rel_err(g - base, ref - base) < TOL64
+
+
+def test_compute_elphq_fp64(c0, c0ctx):
+    s, o, dv = c0
+    say("case: compute_elphq fp64 c0 all k")
+    g = c0ctx.compute_elphq(s.qpts[0], non_loc=True, kminusq=False)
+    ref = o.compute_elphq(0, dv, non_loc=True, kminusq=False)
+    assert rel_err(g, ref) < TOL64
+
+
+def test_compute_elphq_kminusq(c0, c0ctx):
+    s, o, dv = c0
+    say("case: compute_elphq fp64 c0 yambo convention")
+    g = c0ctx.compute_elphq(s.qpts[0], non_loc=True, kminusq=True)
+    ref = o.compute_elphq(0, dv, non_loc=True, kminusq=True, k_first=0, k_count=16)
+    assert rel_err(g[:16], ref) < TOL64
+
+
+def test_compute_elphq_fp32(c0, c0ctx):
+    s, o, dv = c0
+    say("case: compute_elphq fp32 c0 all k")
+    c0ctx.set_precision("fp32")
+    try:
+        g = c0ctx.compute_elphq(s.qpts[0], non_loc=True)
+    finally:
+        c0ctx.set_precision("fp64")
+    ref = o.compute_elphq(0, dv, non_loc=True)
+    assert rel_err(g, ref) < TOL32
+
+
+@pytest.mark.parametrize("kind", ["spin2", "spinor_nmag1", "spinor_nmag4", "cubic_syms",
+                                  "bare", "dfpt_local"])
+def test_tiny_variants(kind):
+    kw = dict(nbnds=4)
+    kernel = "dfpt"
+    if kind == "spin2":
+        kw.update(nspin=2, nmag=2)
+    elif kind == "spinor_nmag1":
+        kw.update(nspinor=2, nmag=1)
+    elif kind == "spinor_nmag4":
+        kw.update(nspinor=2, nmag=4)
+    elif kind == "cubic_syms":
+        kw.update(syms="cubic", q=(0.0, 0.5, 0.0))
+    elif kind in ("bare", "dfpt_local"):
+        kernel = kind
+    s = synth.tiny(**kw)
+    o = O.OracleSystem(s)
+    say(f"case: tiny {kind}")
+    ctx = lelph.Context(s, "fp64")
+    ctx.set_dvscf(s.qpts[0], s.eigvec(0), s.dvscf(0), kernel=kernel)
+    dv = o.dv_total(0, kernel=kernel)
+    assert rel_err(ctx.get_dvscf(), dv) < TOL64
+    non_loc = kernel in ("dfpt", "bare")
+    g = ctx.compute_elphq(s.qpts[0], non_loc=non_loc)
+    ref = o.compute_elphq(0, dv, non_loc=non_loc)
+    ctx.close()
+    assert rel_err(g, ref) < TOL64
+
+
+def test_patterns_rotation():
+    s = synth.tiny(nbnds=4)
+    o = O.OracleSystem(s)
+    say("case: tiny dvscf in pattern basis")
+    rng = np.random.default_rng(5)
+    m = rng.standard_normal((s.nmodes, s.nmodes)) + 1j * rng.standard_normal((s.nmodes, s.nmodes))
+    pats, _ = np.linalg.qr(m)
+    pats = np.ascontiguousarray(pats)
+    dvp = s.dvscf(0).copy()
+    ctx = lelph.Context(s, "fp64")
+    ctx.set_dvscf(s.qpts[0], s.eigvec(0), dvp, patterns=pats, kernel="dfpt")
+    ref = dvp.copy()
+    import ctypes
+    from letzelphc_amd.abi import dptr
+    O.lib().orc_dvscf_pattern_to_mode(dptr(ref), dptr(s.eigvec(0)), dptr(pats), o.lat)
+    O.lib().orc_add_dvscf_qe(dptr(ref), dptr(o.dvlocq(0)), o.lat)
+    got = ctx.get_dvscf()
+    ctx.close()
+    assert rel_err(got, ref) < TOL64
+
+
+def test_error_paths():
+    s = synth.tiny(nbnds=2)
+    say("case: error paths")
+    ctx = lelph.Context(s, "fp64")
+    with pytest.raises(lelph.LelphError) as e:
+        ctx.elph_local(0, 0, 0, 0, qpt=np.zeros(3))
+    assert e.value.status == 6  # no dV set yet
+    ctx.set_dvscf(s.qpts[0], s.eigvec(0), s.dvscf(0))
+    with pytest.raises(lelph.LelphError) as e:
+        ctx.elph_local(0, 999, 0, 0)
+    assert e.value.status == 1
+    with pytest.raises(lelph.LelphError) as e:
+        ctx.compute_elphq(np.array([0.1234, 0.0, 0.0]))
+    assert e.value.status == 7
+    ctx.close()
