@@ -215,9 +215,16 @@ int lelph_elph_q_rotate(const double* Dmats_l, const double* elph_mat_q,
 int lelph_get_kplusq_idxs(int64_t nbz, const double* kpoints_crys,
                           const double qpt_crys[3], int* kplusq_idxs);
 
-/* Timing of the last lelph_gkq / batch call, milliseconds per phase:
- * [0] scatter+FFT, [1] local contraction, [2] non-local, [3] total. */
-int lelph_last_timings(lelph_ctx* ctx, double* ms4);
+/* Device time of the last elph_local / nonlocal / gkq / batch call, summed
+ * over its pairs (HIP events on the context stream), milliseconds:
+ *   [0] sphere scatter + backward FFTs of psi_k and psi_k+q
+ *   [1] local contraction kernel (the MFMA GEMM)
+ *   [2] split-K reduction and 1/N
+ *   [3] non-local part
+ *   [4] total device time
+ *   [5] number of (k,k+q) pairs
+ * n: number of entries wanted (<= 8). */
+int lelph_last_timings(lelph_ctx* ctx, double* ms, int n);
 
 #ifdef __cplusplus
 }

@@ -127,7 +127,11 @@ struct lelph_ctx
     DBuf idx_k, idx_kq, ph_k, ph_kq, box_k, box_kq, part, g, gbatch;
     DBuf nlK, nlKp, Y_K, Y_Kp, wr_K, wr_Kp, beta_K, beta_Kp, bb_K, bb_Kp, bbpart, ebuf;
 
-    double timings[4] = {0, 0, 0, 0};
+    // per-pair events: [0] start, [1] boxes built, [2] contraction done,
+    // [3] local done, [4] non-local done
+    std::vector<hipEvent_t> evs;
+    int64_t ev_pairs = 0;
+    double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
@@ -351,12 +355,14 @@ static int pick_splits(int64_t base_blocks, int64_t nchunks, int64_t out_elems, 
 
 // local part into g (device): elphLocal
 template <typename T>
-static int run_local(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g)
+static int run_local(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g,
+                     hipEvent_t* ev)
 {
     const lelph_lattice& L = c->lat;
     if (!c->dv_set) return LELPH_ERR_STATE;
     if (L.nmag == 2 && (L.nspin != 2 || L.nspinor != 1)) return LELPH_ERR_ARG;
     LELPH_TRY(build_boxes<T>(c, qpt, pr));
+    HIP_TRY(hipEventRecord(ev[1], c->stream));
     const int nm = L.nmodes, nb = L.nbnds, nspin = L.nspin;
     const int nmag_eff = (L.nspinor == 1 && L.nmag == 4) ? 1 : L.nmag;
     const int64_t nchunks = cdiv(c->nf, CT_KC);
@@ -371,6 +377,7 @@ static int run_local(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64*
                        dv_ptr<T>(c), c->part.as<z64>(), nm, nb, nspin, L.nspinor, nmag_eff, c->nf,
                        cps);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev[2], c->stream));
     hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)cdiv(gsz, 256)), dim3(256), 0, c->stream,
                        c->part.as<z64>(), nsplit, gsz, 1.0 / (double)c->nf, g);
     HIP_TRY(hipGetLastError());
@@ -469,27 +476,30 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g)
     return LELPH_OK;
 }
 
-static int run_local_split(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g)
+static int run_local_split(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g,
+                           hipEvent_t* ev)
 {
     (void)c;
     (void)qpt;
     (void)pr;
     (void)g;
+    (void)ev;
     return LELPH_ERR_UNSUPPORTED;
 }
 
-static int run_local_any(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g)
+static int run_local_any(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g,
+                         hipEvent_t* ev)
 {
     switch (c->precision)
     {
-        case LELPH_FP64: return run_local<double>(c, qpt, pr, g);
+        case LELPH_FP64: return run_local<double>(c, qpt, pr, g, ev);
         case LELPH_FP32:
             LELPH_TRY(ensure_dv32(c));
-            return run_local<float>(c, qpt, pr, g);
+            return run_local<float>(c, qpt, pr, g, ev);
         case LELPH_BF16X3:
         case LELPH_BF16X6:
             LELPH_TRY(ensure_dv32(c));
-            return run_local_split(c, qpt, pr, g);
+            return run_local_split(c, qpt, pr, g, ev);
         default: return LELPH_ERR_ARG;
     }
 }
@@ -559,6 +569,8 @@ static void free_ctx(lelph_ctx* c)
                     &c->beta_K, &c->beta_Kp, &c->bb_K, &c->bb_Kp, &c->bbpart, &c->ebuf};
     for (DBuf* b : bufs) b->release();
     for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->evs)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1000,32 +1012,54 @@ int lelph_get_dvscf(lelph_ctx* c, double* dvscf)
     return LELPH_OK;
 }
 
-static int gkq_device(lelph_ctx* c, const double* qpt, const lelph_pair& pr, int local,
-                      int nonloc, z64* g)
+static int ensure_events(lelph_ctx* c, int64_t npairs)
 {
-    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    if (local)
-        LELPH_TRY(run_local_any(c, qpt, pr, g));
-    else
-        HIP_TRY(hipMemsetAsync(g, 0, sizeof(z64) * c->lat.nmodes * c->lat.nspin * c->lat.nbnds *
-                                         c->lat.nbnds,
-                               c->stream));
-    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-    if (nonloc) LELPH_TRY(run_nonlocal(c, pr, g));
-    HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    const size_t need = (size_t)std::max<int64_t>(npairs, 1) * 5;
+    while (c->evs.size() < need)
+    {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        c->evs.push_back(e);
+    }
     return LELPH_OK;
 }
 
-static int finish_timings(lelph_ctx* c)
+// local (+ non-local) of one pair into device g; events ev[0..4]
+static int gkq_device(lelph_ctx* c, const double* qpt, const lelph_pair& pr, int local,
+                      int nonloc, z64* g, hipEvent_t* ev)
 {
-    HIP_TRY(hipEventSynchronize(c->ev[2]));
-    float a = 0, b = 0;
-    HIP_TRY(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
-    HIP_TRY(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
-    c->timings[0] = 0;
-    c->timings[1] = a;
-    c->timings[2] = b;
-    c->timings[3] = a + b;
+    HIP_TRY(hipEventRecord(ev[0], c->stream));
+    if (local)
+        LELPH_TRY(run_local_any(c, qpt, pr, g, ev));
+    else
+    {
+        HIP_TRY(hipEventRecord(ev[1], c->stream));
+        HIP_TRY(hipEventRecord(ev[2], c->stream));
+    }
+    HIP_TRY(hipEventRecord(ev[3], c->stream));
+    if (nonloc) LELPH_TRY(run_nonlocal(c, pr, g));
+    HIP_TRY(hipEventRecord(ev[4], c->stream));
+    return LELPH_OK;
+}
+
+// sums the per-pair phase times of the last call (after the stream drained)
+static int finish_timings(lelph_ctx* c, int64_t npairs)
+{
+    double t[5] = {0, 0, 0, 0, 0};
+    for (int64_t i = 0; i < npairs; ++i)
+    {
+        hipEvent_t* e = c->evs.data() + 5 * i;
+        float a = 0;
+        for (int k = 0; k < 4; ++k)
+        {
+            HIP_TRY(hipEventElapsedTime(&a, e[k], e[k + 1]));
+            t[k] += a;
+        }
+        HIP_TRY(hipEventElapsedTime(&a, e[0], e[4]));
+        t[4] += a;
+    }
+    for (int k = 0; k < 5; ++k) c->timings[k] = t[k];
+    c->timings[5] = (double)npairs;
     return LELPH_OK;
 }
 
@@ -1039,21 +1073,14 @@ static int one_pair(lelph_ctx* c, const double* qpt, int ikq, int ik, int kqsym,
     if (ksym < 0 || kqsym < 0 || ksym >= c->lat.nsym || kqsym >= c->lat.nsym) return LELPH_ERR_ARG;
     const int64_t gsz = (int64_t)c->lat.nmodes * c->lat.nspin * c->lat.nbnds * c->lat.nbnds;
     LELPH_TRY(c->g.ensure(sizeof(z64) * gsz));
+    LELPH_TRY(ensure_events(c, 1));
     if (accumulate)
         HIP_TRY(hipMemcpyAsync(c->g.p, host_g, sizeof(z64) * gsz, hipMemcpyHostToDevice,
                                c->stream));
-    if (accumulate && !local)
-    {
-        HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-        HIP_TRY(hipEventRecord(c->ev[1], c->stream));
-        LELPH_TRY(run_nonlocal(c, pr, c->g.as<z64>()));
-        HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-    }
-    else
-        LELPH_TRY(gkq_device(c, qpt, pr, local, nonloc, c->g.as<z64>()));
+    LELPH_TRY(gkq_device(c, qpt, pr, local, nonloc, c->g.as<z64>(), c->evs.data()));
     HIP_TRY(hipMemcpyAsync(host_g, c->g.p, sizeof(z64) * gsz, hipMemcpyDeviceToHost, c->stream));
     LELPH_TRY(check_err(c));
-    return finish_timings(c);
+    return finish_timings(c, 1);
 }
 
 int lelph_elph_local(lelph_ctx* c, const double qpt[3], int ikq, int ik, int kqsym, int ksym,
@@ -1083,7 +1110,7 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     if (!c || !qpt || npairs < 0 || (npairs > 0 && (!pairs || !g_out))) return LELPH_ERR_ARG;
     const int64_t gsz = (int64_t)c->lat.nmodes * c->lat.nspin * c->lat.nbnds * c->lat.nbnds;
     LELPH_TRY(c->gbatch.ensure(sizeof(z64) * std::max<int64_t>(1, npairs) * gsz));
-    double tl = 0, tn = 0;
+    LELPH_TRY(ensure_events(c, npairs));
     for (int64_t i = 0; i < npairs; ++i)
     {
         const lelph_pair& p = pairs[i];
@@ -1091,21 +1118,13 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
             return LELPH_ERR_ARG;
         if (p.ksym < 0 || p.kqsym < 0 || p.ksym >= c->lat.nsym || p.kqsym >= c->lat.nsym)
             return LELPH_ERR_ARG;
-        LELPH_TRY(gkq_device(c, qpt, p, 1, non_loc, c->gbatch.as<z64>() + i * gsz));
-        if (i + 1 == npairs)
-        {
-            LELPH_TRY(finish_timings(c));
-            tl += c->timings[1];
-            tn += c->timings[2];
-        }
+        LELPH_TRY(gkq_device(c, qpt, p, 1, non_loc, c->gbatch.as<z64>() + i * gsz,
+                             c->evs.data() + 5 * i));
     }
     HIP_TRY(hipMemcpyAsync(g_out, c->gbatch.p, sizeof(z64) * npairs * gsz, hipMemcpyDeviceToHost,
                            c->stream));
     LELPH_TRY(check_err(c));
-    c->timings[1] = tl;
-    c->timings[2] = tn;
-    c->timings[3] = tl + tn;
-    return LELPH_OK;
+    return finish_timings(c, npairs);
 }
 
 int lelph_internal_grid(const lelph_ctx* c, int64_t* nbz, const double** kcrys, const int** kmap)
@@ -1118,10 +1137,10 @@ int lelph_internal_grid(const lelph_ctx* c, int64_t* nbz, const double** kcrys, 
     return LELPH_OK;
 }
 
-int lelph_last_timings(lelph_ctx* c, double* ms4)
+int lelph_last_timings(lelph_ctx* c, double* ms, int n)
 {
-    if (!c || !ms4) return LELPH_ERR_ARG;
-    for (int i = 0; i < 4; ++i) ms4[i] = c->timings[i];
+    if (!c || !ms || n < 0) return LELPH_ERR_ARG;
+    for (int i = 0; i < n && i < 8; ++i) ms[i] = c->timings[i];
     return LELPH_OK;
 }
 

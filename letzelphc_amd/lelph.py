@@ -79,7 +79,7 @@ def lib():
                                           C.c_int, C.c_int, C.c_int, c_double_p]
         L.lelph_get_kplusq_idxs.argtypes = [C.c_int64, c_double_p, c_double_p,
                                             C.POINTER(C.c_int)]
-        L.lelph_last_timings.argtypes = [C.c_void_p, c_double_p]
+        L.lelph_last_timings.argtypes = [C.c_void_p, c_double_p, C.c_int]
         _lib = L
     return _lib
 
@@ -218,10 +218,12 @@ class Context:
                "lelph_compute_elphq")
         return out
 
-    def last_timings(self):
-        t = np.zeros(4)
-        _check(lib().lelph_last_timings(self.h, dptr(t)), "lelph_last_timings")
-        return t
+    def last_timings(self) -> dict:
+        """Device milliseconds of the last call, summed over its pairs."""
+        t = np.zeros(6)
+        _check(lib().lelph_last_timings(self.h, dptr(t), 6), "lelph_last_timings")
+        return {"boxes_ms": t[0], "contract_ms": t[1], "reduce_ms": t[2],
+                "nonlocal_ms": t[3], "total_ms": t[4], "pairs": int(t[5])}
 
 
 def elph_q_rotate(Dl, g, Dr, tim_rev: bool) -> np.ndarray:

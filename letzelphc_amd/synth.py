@@ -24,6 +24,7 @@ The five presets follow the workload descriptions: "c0" 2-atom fcc 24^3,
 from __future__ import annotations
 
 import math
+import os
 from dataclasses import dataclass, field
 from functools import lru_cache
 
@@ -59,7 +60,11 @@ def rng_for(seed: int, *key: int) -> np.random.Generator:
 
 
 def cgauss(rng, shape):
-    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / math.sqrt(2)
+    """complex normal, unit variance: (re, im) pairs from one draw."""
+    n = int(np.prod(shape))
+    z = rng.standard_normal(2 * n).view(np.complex128)
+    z *= 1.0 / math.sqrt(2)
+    return z.reshape(shape)
 
 
 @dataclass
@@ -247,38 +252,54 @@ def _gsphere(kcrys, blat, ecut):
 
 
 def _gram_schmidt_rows(M):
-    """Orthonormalise the rows of M (nb, n) in order (classical GS result,
-    via QR with the diagonal made real positive)."""
-    Q, R = np.linalg.qr(M.T)
-    d = np.diag(R)
-    ph = d / np.abs(d)
-    return np.ascontiguousarray((Q * ph[None, :]).T)
+    """Orthonormalise the rows of M (nb, n) in order: the Gram-Schmidt result
+    M = L U with L lower triangular, positive diagonal, computed as Cholesky
+    QR applied twice (S = M M^H = L L^H, U = L^-1 M)."""
+    U = M
+    for _ in range(2):
+        S = U @ U.conj().T
+        L = np.linalg.cholesky(S)
+        U = np.linalg.inv(L) @ U
+    return np.ascontiguousarray(U)
 
 
-def make_wfcs(sys: System):
-    wf = []
+def _make_wfc(sys: System, ik: int) -> Wfc:
     ps = sys.pseudo
     bscale = sys.blat / (2 * RPI)
-    for ik, kc in enumerate(sys.kpt_ibz_crys):
-        g = _gsphere(kc, sys.blat, sys.ecut)
-        npw = len(g)
-        gcart = np.ascontiguousarray(g.astype(np.float64) @ bscale.T)
-        kg = (g + kc[None, :]) @ sys.blat.T
-        kg2 = np.sum(kg * kg, -1)
-        rng = rng_for(sys.seed, 1, ik)
-        c = cgauss(rng, (sys.nspin, sys.nbnds, sys.nspinor, npw))
-        c *= np.exp(-kg2 / 8.0)[None, None, None, :]
-        for s in range(sys.nspin):
-            M = c[s].reshape(sys.nbnds, sys.nspinor * npw)
-            c[s] = _gram_schmidt_rows(M).reshape(sys.nbnds, sys.nspinor, npw)
-        qn = np.sqrt(kg2)
-        Fk = np.zeros((ps.nltimesj, ps.ntype, npw))
-        pref = 4 * RPI / math.sqrt(sys.volume)
-        for li in range(ps.nltimesj):
-            l = int(round(ps.PP_table[li, 0, 0] - 1))
-            for t in range(ps.ntype):
-                Fk[li, t] = pref * math.sqrt(abs(ps.Dl[li, t])) * kb_radial_ft(l, qn, ps.kb_alpha)
-        wf.append(Wfc(np.ascontiguousarray(c), gcart, np.ascontiguousarray(Fk), g))
+    kc = sys.kpt_ibz_crys[ik]
+    g = _gsphere(kc, sys.blat, sys.ecut)
+    npw = len(g)
+    gcart = np.ascontiguousarray(g.astype(np.float64) @ bscale.T)
+    kg = (g + kc[None, :]) @ sys.blat.T
+    kg2 = np.sum(kg * kg, -1)
+    rng = rng_for(sys.seed, 1, ik)
+    c = cgauss(rng, (sys.nspin, sys.nbnds, sys.nspinor, npw))
+    c *= np.exp(-kg2 / 8.0)[None, None, None, :]
+    for s in range(sys.nspin):
+        M = c[s].reshape(sys.nbnds, sys.nspinor * npw)
+        c[s] = _gram_schmidt_rows(M).reshape(sys.nbnds, sys.nspinor, npw)
+    qn = np.sqrt(kg2)
+    Fk = np.zeros((ps.nltimesj, ps.ntype, npw))
+    pref = 4 * RPI / math.sqrt(sys.volume)
+    for li in range(ps.nltimesj):
+        l = int(round(ps.PP_table[li, 0, 0] - 1))
+        for t in range(ps.ntype):
+            Fk[li, t] = pref * math.sqrt(abs(ps.Dl[li, t])) * kb_radial_ft(l, qn, ps.kb_alpha)
+    return Wfc(np.ascontiguousarray(c), gcart, np.ascontiguousarray(Fk), g)
+
+
+def make_wfcs(sys: System, workers: int = 0):
+    """All iBZ wavefunctions; independent per k, generated on a thread pool
+    (every k has its own Philox stream, so the result does not depend on the
+    number of workers)."""
+    from concurrent.futures import ThreadPoolExecutor
+    nk = len(sys.kpt_ibz_crys)
+    workers = workers or 1
+    if workers <= 1 or nk <= 1:
+        wf = [_make_wfc(sys, ik) for ik in range(nk)]
+    else:
+        with ThreadPoolExecutor(workers) as ex:
+            wf = list(ex.map(lambda ik: _make_wfc(sys, ik), range(nk)))
     sys.wfcs = wf
     return wf
 

@@ -40,9 +40,11 @@ typedef double complex cplx;
 #define REPS6 1e-6      /* ELPH_EPS6   src/common/constants.h:10 */
 #define RE2 2.0         /* ELPH_e2     src/common/constants.h:13 */
 
-static int g_threads = 1;
+static int g_threads = 1;       /* over k points (orc_compute_elphq)        */
+static int g_inner_threads = 1; /* inside one elphLocal (bands, GEMM rows) */
 void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 int orc_get_threads(void) { return g_threads; }
+void orc_set_inner_threads(int n) { g_inner_threads = n > 0 ? n : 1; }
 
 /* ======================================================================
  * 3x3 helpers: MatVec3f / Gemm3x3f / det3x3 / reciprocal_vecs
@@ -606,6 +608,7 @@ static void zgemm_rm(char ta, char tb, const cplx* A, const cplx* B, cplx* Cm,
                      cplx alpha, cplx beta, int64_t lda, int64_t ldb,
                      int64_t ldc, int64_t m, int64_t n, int64_t k)
 {
+#pragma omp parallel for num_threads(g_inner_threads) schedule(static) if (g_inner_threads > 1)
     for (int64_t i = 0; i < m; ++i)
         for (int64_t j = 0; j < n; ++j)
         {
@@ -703,18 +706,19 @@ int orc_elphLocal(const double* qpt, const orc_wfc* wfcs,
     /* psi_k(r), :207-241 */
     cplx* wSk_r = malloc(sizeof(cplx) * nspin * nb * nspinor * nf);
     for (int64_t i = 0; i < 3 * npwk; ++i) gf[i] = gvk[i];
+    int ierr = 0;
+#pragma omp parallel for num_threads(g_inner_threads) schedule(dynamic, 1) reduction(| : ierr) if (g_inner_threads > 1)
     for (int64_t s = 0; s < nspin * nb; ++s)
     {
         cplx* w = wSk + s * nspinor * npwk;
         su2rotate(nspinor, npwk, su2k, w);
         apply_trans_wfc(tauk_c, kvecSk, nspinor, npwk, gf, w, false);
-        if ((err = invfft3D(gvk, npwk, N, nspinor, w, wSk_r + s * nspinor * nf,
-                            trk)))
-            goto out1;
+        ierr |= invfft3D(gvk, npwk, N, nspinor, w, wSk_r + s * nspinor * nf, trk);
     }
+    if ((err = ierr)) goto out1;
 
     cplx* dVpsiG = malloc(sizeof(cplx) * nspin * nb * nspinor * npwkq);
-    cplx* work = malloc(sizeof(cplx) * nf);
+    cplx* work = malloc(sizeof(cplx) * nf * g_inner_threads);
     cplx* mn = calloc(nb * nb, sizeof(cplx));
     for (int64_t i = 0; i < nmodes * nb * nb * nspin; ++i) elph_kq[i] = 0;
     if ((nmag == 2 && nspin != 2) || (nmag == 2 && nspinor != 1))
@@ -730,14 +734,20 @@ int orc_elphLocal(const double* qpt, const orc_wfc* wfcs,
         {
             const cplx* psi = wSk_r + is * nb * nspinor * nf;
             const cplx* dV = dv_nu + nf * is;
+            int cerr = 0;
+#pragma omp parallel for num_threads(g_inner_threads) schedule(dynamic, 1) reduction(| : cerr) if (g_inner_threads > 1)
             for (int64_t ib = 0; ib < nb; ++ib)
             {
+                int tn = 0;
+#ifdef _OPENMP
+                tn = omp_get_thread_num();
+#endif
                 cplx* o = dVpsiG + is * nb * nspinor * npwkq + ib * nspinor * npwkq;
-                if ((err = fft_convolution3D(gvkq, npwkq, N, (int)nspinor,
-                                             (int)nmag, dV, psi + ib * nspinor * nf,
-                                             o, false, work)))
-                    goto out2;
+                cerr |= fft_convolution3D(gvkq, npwkq, N, (int)nspinor, (int)nmag, dV,
+                                          psi + ib * nspinor * nf, o, false,
+                                          work + (int64_t)tn * nf);
             }
+            if ((err = cerr)) goto out2;
             char bc = trkq ? 'T' : 'C';
             zgemm_rm('N', bc, dVpsiG + is * nb * nspinor * npwkq,
                      wSkq + is * nb * nspinor * npwkq, mn, 1.0, 0.0,

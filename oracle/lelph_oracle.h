@@ -86,7 +86,7 @@ void orc_elph_q_rotate(const double* Dl, const double* g, const double* Dr,
 int orc_get_KplusQ_idxs(int64_t nbz, const double* kpts, const double* q,
                         int* idx);
 /* the k loop of compute_and_write_elphq; OpenMP over k; g_out
- * (nkpts_BZ,nmodes,nspin,nb,nb).  k_first/k_count restrict the k range. */
+ * (k_count,nmodes,nspin,nb,nb) for full-BZ k in [k_first, k_first+k_count). */
 int orc_compute_elphq(const double* qpt, const orc_wfc* wfcs,
                       const lelph_lattice* lat, const lelph_pseudo* pp,
                       const double* eigVec, const double* dVscf, int non_loc,
@@ -94,6 +94,8 @@ int orc_compute_elphq(const double* qpt, const orc_wfc* wfcs,
                       double* g_out);
 /* sets the number of OpenMP threads used by orc_compute_elphq */
 void orc_set_threads(int n);
+/* threads used inside one orc_elphLocal (bands, GEMM rows) */
+void orc_set_inner_threads(int n);
 int orc_get_threads(void);
 
 #ifdef __cplusplus

@@ -80,6 +80,7 @@ def lib():
                                         C.POINTER(Pseudo), c_double_p, c_double_p, C.c_int,
                                         C.c_int, C.c_int64, C.c_int64, c_double_p]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_inner_threads.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -98,6 +99,10 @@ def fft3d(box: np.ndarray, sign: int) -> np.ndarray:
 
 def set_threads(n: int):
     lib().orc_set_threads(int(n))
+
+
+def set_inner_threads(n: int):
+    lib().orc_set_inner_threads(int(n))
 
 
 class OracleSystem:
