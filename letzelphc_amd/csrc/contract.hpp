@@ -53,6 +53,7 @@ struct Mfma<float>
 #define CT_VB 2  // modes per block
 #define CT_NB 32 // k bands (rows) per block
 #define CT_MB 64 // k+q bands (columns) per block
+#define CT_FLUSH 4
 
 // Loads A[n][r] = (dV_nu psi_k,n)_sigma(r) for the block's modes.
 template <typename T>
@@ -98,19 +99,42 @@ __global__ __launch_bounds__(256) void k_contract(const cpx<T>* __restrict__ psi
     const int64_t c1 = (c0 + chunks_per_split < nchunks) ? c0 + chunks_per_split : nchunks;
     const int wv = wave >> 1, wm = wave & 1;
 
+    // FP32: the MFMA accumulators are added into FP64 every CT_FLUSH chunks,
+    // so no FP32 summation chain is longer than CT_FLUSH * CT_KC points.
+    constexpr bool flush = sizeof(T) == 4;
     acc_t accr[2][2], acci[2][2];
+    double dr[2][2][4], di[2][2][4];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b)
             for (int v = 0; v < 4; ++v)
             {
                 accr[a][b][v] = 0;
                 acci[a][b][v] = 0;
+                dr[a][b][v] = 0;
+                di[a][b][v] = 0;
             }
 
+    int nfl = 0;
     for (int sp = 0; sp < nspinor; ++sp)
     {
         for (int64_t c = c0; c < c1; ++c)
         {
+            if (flush && ++nfl > CT_FLUSH)
+            {
+                nfl = 1;
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                        {
+                            dr[a][b][e] += (double)accr[a][b][e];
+                            di[a][b][e] += (double)acci[a][b][e];
+                            accr[a][b][e] = 0;
+                            acci[a][b][e] = 0;
+                        }
+            }
             const int64_t r0 = c * CT_KC;
             for (int f = tid; f < CT_VB * CT_NB * CT_KC; f += 256)
             {
@@ -169,7 +193,8 @@ __global__ __launch_bounds__(256) void k_contract(const cpx<T>* __restrict__ psi
                 if (row < nb && col < nb)
                 {
                     const int64_t o = ((((int64_t)split * nm + v) * nspin + is) * nb + row) * nb + col;
-                    part[o] = {(double)accr[rb][cb][e], (double)acci[rb][cb][e]};
+                    part[o] = {dr[rb][cb][e] + (double)accr[rb][cb][e],
+                               di[rb][cb][e] + (double)acci[rb][cb][e]};
                 }
             }
 }

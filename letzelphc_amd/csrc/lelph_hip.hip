@@ -18,6 +18,7 @@
 #include "dvloc.hpp"
 #include "fft.hpp"
 #include "nonloc.hpp"
+#include "split.hpp"
 #include "wfc.hpp"
 
 namespace
@@ -124,7 +125,7 @@ struct lelph_ctx
     bool dv32_valid = false;
 
     // workspaces
-    DBuf idx_k, idx_kq, ph_k, ph_kq, box_k, box_kq, part, g, gbatch;
+    DBuf idx_k, idx_kq, ph_k, ph_kq, box_k, box_kq, part, g, gbatch, presplit;
     DBuf nlK, nlKp, Y_K, Y_Kp, wr_K, wr_Kp, beta_K, beta_Kp, bb_K, bb_Kp, bbpart, ebuf;
 
     // per-pair events: [0] start, [1] boxes built, [2] contraction done,
@@ -476,15 +477,69 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g)
     return LELPH_OK;
 }
 
+// local part with split-product bf16 MFMA (split.hpp)
+template <int NP, int NPROD>
+static int run_local_split_t(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g,
+                             hipEvent_t* ev)
+{
+    const lelph_lattice& L = c->lat;
+    if (!c->dv_set) return LELPH_ERR_STATE;
+    if (L.nmag == 2 && (L.nspin != 2 || L.nspinor != 1)) return LELPH_ERR_ARG;
+    LELPH_TRY(build_boxes<float>(c, qpt, pr));
+    const int nm = L.nmodes, nb = L.nbnds, nspin = L.nspin;
+    const int nmag_eff = (L.nspinor == 1 && L.nmag == 4) ? 1 : L.nmag;
+    const int64_t nchunks = cdiv(c->nf, SP_KC);
+    const int nbp = (int)(cdiv(nb, SP_PAD) * SP_PAD);
+    // psi_k+q -> bf16 pieces, chunk-major
+    const int64_t nBp = presplit_elems(nspin, L.nspinor, nchunks, NP, nbp);
+    LELPH_TRY(c->presplit.ensure(sizeof(__bf16) * (size_t)nBp));
+    const int64_t nthr = (int64_t)nspin * L.nspinor * nbp * nchunks * SP_KC;
+    hipLaunchKernelGGL((k_presplit_B<NP>), dim3((unsigned)cdiv(nthr, 256)), dim3(256), 0,
+                       c->stream, c->box_kq.as<c32>(), nb, nbp, L.nspinor, c->nf, nchunks, nthr,
+                       c->presplit.as<__bf16>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev[1], c->stream));
+    SplitArgs sa{};
+    sa.nm = nm;
+    sa.nb = nb;
+    sa.nbp = nbp;
+    sa.nspin = nspin;
+    sa.nspinor = L.nspinor;
+    sa.nmag = nmag_eff;
+    sa.ntn = nbp / SP_TR;
+    sa.ntm = (int)cdiv(nb, SP_TC);
+    sa.N = c->nf;
+    sa.nchunks = nchunks;
+    // nspin * nsplit must be a multiple of 8 (XCD grouping); ~768+ blocks
+    const int per_group = nm * sa.ntn * sa.ntm;
+    int64_t ns = cdiv(768, (int64_t)per_group * nspin);
+    ns = std::max<int64_t>(8, cdiv(ns, 8) * 8);
+    ns = std::min<int64_t>(ns, std::max<int64_t>(8, (nchunks / 8) * 8));
+    sa.nsplit = (int)ns;
+    sa.chunks_per_split = cdiv(nchunks, ns);
+    const int64_t gsz = (int64_t)nm * nspin * nb * nb;
+    LELPH_TRY(c->part.ensure(sizeof(z64) * (size_t)(ns * gsz)));
+    const size_t lds = split_lds_bytes<NP>();
+    HIP_TRY(hipFuncSetAttribute((const void*)k_contract_split<NP, NPROD>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int64_t nblk = (int64_t)nspin * ns * per_group;
+    hipLaunchKernelGGL((k_contract_split<NP, NPROD>), dim3((unsigned)nblk), dim3(SP_THREADS), lds,
+                       c->stream, c->box_k.as<c32>(), c->presplit.as<__bf16>(),
+                       c->dv32.as<c32>(), c->part.as<z64>(), sa);
+    HIP_TRY(hipGetLastError());
+    const int nsplit = (int)ns;
+    HIP_TRY(hipEventRecord(ev[2], c->stream));
+    hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)cdiv(gsz, 256)), dim3(256), 0, c->stream,
+                       c->part.as<z64>(), nsplit, gsz, 1.0 / (double)c->nf, g);
+    HIP_TRY(hipGetLastError());
+    return LELPH_OK;
+}
+
 static int run_local_split(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g,
                            hipEvent_t* ev)
 {
-    (void)c;
-    (void)qpt;
-    (void)pr;
-    (void)g;
-    (void)ev;
-    return LELPH_ERR_UNSUPPORTED;
+    if (c->precision == LELPH_BF16X3) return run_local_split_t<2, 3>(c, qpt, pr, g, ev);
+    return run_local_split_t<3, 6>(c, qpt, pr, g, ev);
 }
 
 static int run_local_any(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g,
@@ -565,7 +620,7 @@ static void free_ctx(lelph_ctx* c)
         if (p) (void)hipFree(p);
     DBuf* bufs[] = {&c->dv64, &c->dv32, &c->eig_d, &c->vloc_tmp, &c->rot_tmp, &c->idx_k,
                     &c->idx_kq, &c->ph_k, &c->ph_kq, &c->box_k, &c->box_kq, &c->part, &c->g,
-                    &c->gbatch, &c->nlK, &c->nlKp, &c->Y_K, &c->Y_Kp, &c->wr_K, &c->wr_Kp,
+                    &c->gbatch, &c->presplit, &c->nlK, &c->nlKp, &c->Y_K, &c->Y_Kp, &c->wr_K, &c->wr_Kp,
                     &c->beta_K, &c->beta_Kp, &c->bb_K, &c->bb_Kp, &c->bbpart, &c->ebuf};
     for (DBuf* b : bufs) b->release();
     for (auto& e : c->ev)
