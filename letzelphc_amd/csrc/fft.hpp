@@ -279,8 +279,201 @@ __global__ __launch_bounds__(256) void k_fft_lines(cpx<T>* __restrict__ data, in
     }
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time-size variant: line length N, lines per block TL and every
+// stage's radix are constants, so all index arithmetic reduces to shifts and
+// multiplications.  Same factorisation rule as fft_factor (4s first, then
+// primes in increasing order).
+constexpr int fft_radix_c(int n, int s)
+{
+    int m = n, k = 0;
+    while (m % 4 == 0)
+    {
+        if (k == s) return 4;
+        m /= 4;
+        ++k;
+    }
+    for (int p = 2; m > 1;)
+    {
+        if (m % p == 0)
+        {
+            if (k == s) return p;
+            m /= p;
+            ++k;
+        }
+        else
+            ++p;
+    }
+    return 0;
+}
+constexpr int fft_ns_c(int n, int s) // product of the radices before stage s
+{
+    int r = 1;
+    for (int i = 0; i < s; ++i) r *= fft_radix_c(n, i);
+    return r;
+}
+
+template <typename T, int SIGN, int N, int TL, int S>
+struct FftStagesC
+{
+    static __device__ inline void run(cpx<T>*& in, cpx<T>*& out, const cpx<T>* W, int tid)
+    {
+        constexpr int R = fft_radix_c(N, S);
+        if constexpr (R != 0)
+        {
+            constexpr int Ns = fft_ns_c(N, S);
+            constexpr int NBF = N / R;
+            constexpr int TWS = N / (Ns * R);
+            for (int w = tid; w < TL * NBF; w += 256)
+            {
+                const int t = w / NBF, j = w - t * NBF;
+                const int k = j % Ns;
+                cpx<T> v[R];
+                const cpx<T>* src = in + t * N + j;
+                v[0] = src[0];
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[r] = cmul(src[r * NBF], tw<T, SIGN>(W, r * k * TWS));
+                if constexpr (R <= 5)
+                    dft_small<T, SIGN>(v, R, W, N);
+                else
+                {
+                    cpx<T> o[R];
+#pragma unroll
+                    for (int s = 0; s < R; ++s)
+                    {
+                        cpx<T> acc = {0, 0};
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            acc = cadd(acc, cmul(v[r], tw<T, SIGN>(W, ((r * s) % R) * (N / R))));
+                        o[s] = acc;
+                    }
+#pragma unroll
+                    for (int s = 0; s < R; ++s) v[s] = o[s];
+                }
+                cpx<T>* dst = out + t * N + (j / Ns) * Ns * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
+            }
+            __syncthreads();
+            cpx<T>* tmp = in;
+            in = out;
+            out = tmp;
+            FftStagesC<T, SIGN, N, TL, S + 1>::run(in, out, W, tid);
+        }
+    }
+};
+
+// One axis pass, compile-time N and TL (see k_fft_lines for the addressing).
+// Lines i >= n_inner of the last tile are computed on zeros and not stored.
+template <typename T, int SIGN, int N, int TL>
+__global__ __launch_bounds__(256) void k_fft_lines_c(cpx<T>* __restrict__ data, int64_t n_inner,
+                                                     int64_t ostride, int64_t lstride,
+                                                     int64_t estride,
+                                                     const cpx<T>* __restrict__ Wg)
+{
+    __shared__ cpx<T> W[N];
+    __shared__ cpx<T> b0[TL * N];
+    __shared__ cpx<T> b1[TL * N];
+    const int tid = threadIdx.x;
+    const int64_t tiles = (n_inner + TL - 1) / TL;
+    const int64_t o = blockIdx.x / tiles;
+    const int64_t i0 = (blockIdx.x - o * tiles) * TL;
+    const int nl = (int)((n_inner - i0) < TL ? (n_inner - i0) : TL);
+    cpx<T>* base = data + o * ostride + i0 * lstride;
+
+    for (int t = tid; t < N; t += 256) W[t] = Wg[t];
+    if (lstride == 1)
+    {
+        for (int f = tid; f < TL * N; f += 256)
+        {
+            const int j = f / TL, t = f - j * TL;
+            b0[t * N + j] = (t < nl) ? base[(int64_t)j * estride + t] : cpx<T>{0, 0};
+        }
+    }
+    else
+    {
+        for (int f = tid; f < TL * N; f += 256)
+        {
+            const int t = f / N, j = f - t * N;
+            b0[t * N + j] = (t < nl) ? base[(int64_t)t * lstride + (int64_t)j * estride]
+                                     : cpx<T>{0, 0};
+        }
+    }
+    __syncthreads();
+    cpx<T>* in = b0;
+    cpx<T>* out = b1;
+    FftStagesC<T, SIGN, N, TL, 0>::run(in, out, W, tid);
+    if (lstride == 1)
+    {
+        for (int f = tid; f < TL * N; f += 256)
+        {
+            const int j = f / TL, t = f - j * TL;
+            if (t < nl) base[(int64_t)j * estride + t] = in[t * N + j];
+        }
+    }
+    else
+    {
+        for (int f = tid; f < TL * N; f += 256)
+        {
+            const int t = f / N, j = f - t * N;
+            if (t < nl) base[(int64_t)t * lstride + (int64_t)j * estride] = in[t * N + j];
+        }
+    }
+}
+
+// lines per block: two buffers of TL*N plus the twiddles within ~60 KB
+template <typename T, int N>
+constexpr int fft_tl_c()
+{
+    return (N * (int)sizeof(cpx<T>) * 2 * 32 <= 56 * 1024) ? 32
+           : (N * (int)sizeof(cpx<T>) * 2 * 16 <= 56 * 1024) ? 16
+           : 8;
+}
+
+template <typename T, int N>
+static bool fft_launch_c(cpx<T>* data, int64_t n_outer, int64_t n_inner, int64_t ostride,
+                         int64_t lstride, int64_t estride, int sign, const cpx<T>* W,
+                         hipStream_t stream)
+{
+    constexpr int TL = fft_tl_c<T, N>();
+    const int64_t nblk = cdiv(n_inner, TL) * n_outer;
+    if (nblk > 0x7fffffffLL) return false;
+    if (sign > 0)
+        hipLaunchKernelGGL((k_fft_lines_c<T, 1, N, TL>), dim3((unsigned)nblk), dim3(256), 0,
+                           stream, data, n_inner, ostride, lstride, estride, W);
+    else
+        hipLaunchKernelGGL((k_fft_lines_c<T, -1, N, TL>), dim3((unsigned)nblk), dim3(256), 0,
+                           stream, data, n_inner, ostride, lstride, estride, W);
+    return true;
+}
+
 template <typename T>
 static inline const cpx<T>* axis_tw(const FftAxis& a);
+
+// compiled line lengths (the grids of the benchmark systems and their
+// neighbours); other lengths use the generic kernel
+template <typename T>
+static bool fft_try_c(int n, cpx<T>* data, int64_t no, int64_t ni, int64_t os, int64_t ls,
+                      int64_t es, int sign, const cpx<T>* W, hipStream_t st)
+{
+    switch (n)
+    {
+        case 12: return fft_launch_c<T, 12>(data, no, ni, os, ls, es, sign, W, st);
+        case 24: return fft_launch_c<T, 24>(data, no, ni, os, ls, es, sign, W, st);
+        case 32: return fft_launch_c<T, 32>(data, no, ni, os, ls, es, sign, W, st);
+        case 36: return fft_launch_c<T, 36>(data, no, ni, os, ls, es, sign, W, st);
+        case 45: return fft_launch_c<T, 45>(data, no, ni, os, ls, es, sign, W, st);
+        case 48: return fft_launch_c<T, 48>(data, no, ni, os, ls, es, sign, W, st);
+        case 60: return fft_launch_c<T, 60>(data, no, ni, os, ls, es, sign, W, st);
+        case 64: return fft_launch_c<T, 64>(data, no, ni, os, ls, es, sign, W, st);
+        case 72: return fft_launch_c<T, 72>(data, no, ni, os, ls, es, sign, W, st);
+        case 96: return fft_launch_c<T, 96>(data, no, ni, os, ls, es, sign, W, st);
+        case 108: return fft_launch_c<T, 108>(data, no, ni, os, ls, es, sign, W, st);
+        case 216: return fft_launch_c<T, 216>(data, no, ni, os, ls, es, sign, W, st);
+        default: return false;
+    }
+}
+
 template <>
 inline const z64* axis_tw<double>(const FftAxis& a)
 {
@@ -298,6 +491,12 @@ static int fft_pass(const FftAxis& ax, cpx<T>* data, int64_t n_outer, int64_t n_
                     hipStream_t stream)
 {
     const int n = ax.n;
+    if (fft_try_c<T>(n, data, n_outer, n_inner, ostride, lstride, estride, sign, axis_tw<T>(ax),
+                     stream))
+    {
+        HIP_TRY(hipGetLastError());
+        return LELPH_OK;
+    }
     const int64_t lds_budget = 60 * 1024;
     int64_t tl = (lds_budget / (int64_t)sizeof(cpx<T>) - n) / (2 * n);
     if (tl > 64) tl = 64;

@@ -493,10 +493,10 @@ static int run_local_split_t(lelph_ctx* c, const double* qpt, const lelph_pair& 
     // psi_k+q -> bf16 pieces, chunk-major
     const int64_t nBp = presplit_elems(nspin, L.nspinor, nchunks, NP, nbp);
     LELPH_TRY(c->presplit.ensure(sizeof(__bf16) * (size_t)nBp));
-    const int64_t nthr = (int64_t)nspin * L.nspinor * nbp * nchunks * SP_KC;
-    hipLaunchKernelGGL((k_presplit_B<NP>), dim3((unsigned)cdiv(nthr, 256)), dim3(256), 0,
-                       c->stream, c->box_kq.as<c32>(), nb, nbp, L.nspinor, c->nf, nchunks, nthr,
-                       c->presplit.as<__bf16>());
+    hipLaunchKernelGGL((k_presplit_B<NP>),
+                       dim3((unsigned)cdiv(nchunks * SP_KC, 256), (unsigned)(nspin * L.nspinor * nbp)),
+                       dim3(256), 0, c->stream, c->box_kq.as<c32>(), nb, nbp, L.nspinor, c->nf,
+                       nchunks, c->presplit.as<__bf16>());
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], c->stream));
     SplitArgs sa{};

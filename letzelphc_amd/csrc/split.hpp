@@ -69,24 +69,21 @@ static inline int64_t presplit_elems(int nspin, int nspinor, int64_t nchunks, in
     return (int64_t)nspin * nspinor * nchunks * NP * 2 * nbp * SP_KC;
 }
 
-// psi_k+q box (nspin, nb, nspinor, N) FP32 -> Bp pieces.  One thread per
-// (set, row, chunk, point); rows m >= nb and points >= N are zero.
+// psi_k+q box (nspin, nb, nspinor, N) FP32 -> Bp pieces.  Grid: x over the
+// padded points (nchunks * 32), y over (spin, spinor, row m < nbp); rows
+// m >= nb and points >= N are zero.
 template <int NP>
 __global__ void k_presplit_B(const c32* __restrict__ box, int nb, int nbp, int nspinor,
-                             int64_t N, int64_t nchunks, int64_t total,
-                             __bf16* __restrict__ Bp)
+                             int64_t N, int64_t nchunks, __bf16* __restrict__ Bp)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total) return;
-    const int kk = (int)(t % SP_KC);
-    int64_t rest = t / SP_KC;
-    const int64_t c = rest % nchunks;
-    rest /= nchunks;
-    const int m = (int)(rest % nbp);
-    rest /= nbp;
-    const int sp = (int)(rest % nspinor);
-    const int is = (int)(rest / nspinor);
-    const int64_t r = c * SP_KC + kk;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nchunks * SP_KC) return;
+    const int64_t c = r >> 5;
+    const int kk = (int)(r & 31);
+    const int y = blockIdx.y;
+    const int m = y % nbp;
+    const int sp = (y / nbp) % nspinor;
+    const int is = y / (nbp * nspinor);
     c32 v = {0, 0};
     if (m < nb && r < N) v = box[((int64_t)(is * nb + m) * nspinor + sp) * N + r];
     __bf16 pr[NP], pi[NP];
