@@ -520,12 +520,24 @@ static int run_local_split_t(lelph_ctx* c, const double* qpt, const lelph_pair& 
     const int64_t gsz = (int64_t)nm * nspin * nb * nb;
     LELPH_TRY(c->part.ensure(sizeof(z64) * (size_t)(ns * gsz)));
     const size_t lds = split_lds_bytes<NP>();
-    HIP_TRY(hipFuncSetAttribute((const void*)k_contract_split<NP, NPROD>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int64_t nblk = (int64_t)nspin * ns * per_group;
-    hipLaunchKernelGGL((k_contract_split<NP, NPROD>), dim3((unsigned)nblk), dim3(SP_THREADS), lds,
-                       c->stream, c->box_k.as<c32>(), c->presplit.as<__bf16>(),
-                       c->dv32.as<c32>(), c->part.as<z64>(), sa);
+    const bool fast = L.nspinor == 1 && nmag_eff != 4 && c->nf % SP_KC == 0;
+    if (fast)
+    {
+        HIP_TRY(hipFuncSetAttribute((const void*)k_contract_split<NP, NPROD, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_contract_split<NP, NPROD, true>), dim3((unsigned)nblk),
+                           dim3(SP_THREADS), lds, c->stream, c->box_k.as<c32>(),
+                           c->presplit.as<__bf16>(), c->dv32.as<c32>(), c->part.as<z64>(), sa);
+    }
+    else
+    {
+        HIP_TRY(hipFuncSetAttribute((const void*)k_contract_split<NP, NPROD, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_contract_split<NP, NPROD, false>), dim3((unsigned)nblk),
+                           dim3(SP_THREADS), lds, c->stream, c->box_k.as<c32>(),
+                           c->presplit.as<__bf16>(), c->dv32.as<c32>(), c->part.as<z64>(), sa);
+    }
     HIP_TRY(hipGetLastError());
     const int nsplit = (int)ns;
     HIP_TRY(hipEventRecord(ev[2], c->stream));

@@ -110,16 +110,18 @@ template <int NP>
 struct SplitStage
 {
     static constexpr int B16 = NP * 2 * SP_TC * SP_KC * 2 / 16 / SP_THREADS; // uint4 per thread
-    c32 psi[8]; // psi_k (or, for 2x2 spinor potentials, dV psi_k)
-    c32 dvv;    // dV_nu at point tid of the chunk (tid < SP_KC), else unused
+    uint4 psi[4]; // 8 complex: psi_k (or, for 2x2 spinor potentials, dV psi_k)
+    c32 dvv;      // dV_nu at point tid of the chunk (tid < SP_KC), else unused
     uint4 b[B16 > 0 ? B16 : 1];
 };
 
 // Block stage of chunk c: psi_k rows [n0, n0+128) x 32 points (thread ->
-// row tid/4, 8 points), the 32 dV values, and the B pieces.  With one
-// spinor and a diagonal potential the product dV psi is formed when the
-// wave builds its A fragment; otherwise the product is staged and dV = 1.
-template <int NP>
+// row tid/4, 8 points), the 32 dV values, and the B pieces.
+// FAST (one spinor, diagonal potential, N a multiple of 32): 16-byte loads,
+// no bounds tests; rows n >= nb read row nb-1 (their results are never
+// stored) and the product dV psi is formed when the wave builds its A
+// fragment.  Otherwise the product is staged element by element and dV = 1.
+template <int NP, bool FAST>
 __device__ inline void stage_load(SplitStage<NP>& st, const c32* __restrict__ psik,
                                   const __bf16* __restrict__ Bp, const c32* __restrict__ dv,
                                   const SplitArgs& a, int v, int is, int sp, int n0, int m0,
@@ -129,30 +131,21 @@ __device__ inline void stage_load(SplitStage<NP>& st, const c32* __restrict__ ps
     const int n = n0 + arow;
     const int64_t r0 = c * SP_KC + aoff;
     const int64_t N = a.N;
-    const bool fast = (a.nspinor == 1) && (a.nmag != 4);
-    if (fast)
+    if constexpr (FAST)
     {
-        const c32* pk = psik + (int64_t)(is * a.nb + n) * N + r0;
-        if (n < a.nb && r0 + 8 <= N)
-        {
+        const int nn = n < a.nb ? n : a.nb - 1;
+        const uint4* pk = reinterpret_cast<const uint4*>(psik + (int64_t)(is * a.nb + nn) * N + r0);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) st.psi[k] = pk[k];
-        }
-        else
-        {
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                st.psi[k] = (n < a.nb && r0 + k < N) ? pk[k] : c32{0.f, 0.f};
-        }
+        for (int k = 0; k < 4; ++k) st.psi[k] = pk[k];
         if (tid < SP_KC)
         {
-            const int64_t r = c * SP_KC + tid;
             const c32* V = dv + ((int64_t)v * a.nmag + (a.nmag == 2 ? is : 0)) * N;
-            st.dvv = r < N ? V[r] : c32{0.f, 0.f};
+            st.dvv = V[c * SP_KC + tid];
         }
     }
     else
     {
+        c32 vals[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k)
         {
@@ -160,8 +153,9 @@ __device__ inline void stage_load(SplitStage<NP>& st, const c32* __restrict__ ps
             c32 val = {0, 0};
             if (n < a.nb && r < N)
                 val = load_dvpsi<float>(psik, dv, v, is, n, sp, r, a.nb, a.nspinor, a.nmag, N);
-            st.psi[k] = val;
+            vals[k] = val;
         }
+        __builtin_memcpy(st.psi, vals, sizeof(vals));
         st.dvv = {1.f, 0.f};
     }
     const uint4* src = reinterpret_cast<const uint4*>(
@@ -181,16 +175,65 @@ __device__ inline void stage_store(const SplitStage<NP>& st, c32* sA, __bf16* sB
                                    int tid)
 {
     const int arow = tid >> 2, aoff = (tid & 3) * 8;
-    c32* d = sA + arow * SP_AL + aoff;
+    uint4* d = reinterpret_cast<uint4*>(sA + arow * SP_AL + aoff);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) d[k] = st.psi[k];
+    for (int k = 0; k < 4; ++k) d[k] = st.psi[k];
     if (tid < SP_KC) sD[tid] = st.dvv;
     uint4* db = reinterpret_cast<uint4*>(sB);
 #pragma unroll
     for (int q = 0; q < SplitStage<NP>::B16; ++q) db[tid + q * SP_THREADS] = st.b[q];
 }
 
-template <int NP, int NPROD>
+typedef __attribute__((address_space(3))) char lds_char;
+
+// FAST staging of chunk c straight from global memory into LDS
+// (global_load_lds_dwordx4: each wave instruction fills 1 KB of LDS, lane l
+// its bytes [16 l, 16 l + 16)).  A: the padded tile, 128 rows x 272 bytes,
+// as 34 slices; pad slots reload the row's first 16 bytes.  B: NP*2 planes of
+// 64 rows x 64 bytes.  D: the 32 dV values (wave 0, lanes 0..15).  The
+// caller waits (s_waitcnt) and synchronises before reading the buffer.
+template <int NP>
+__device__ inline void stage_async(const c32* __restrict__ psik, const __bf16* __restrict__ Bp,
+                                   const c32* __restrict__ dv, const SplitArgs& a, int v, int is,
+                                   int n0, int m0, int64_t c, lds_char* LA, lds_char* LB,
+                                   lds_char* LD, int wave, int lane)
+{
+    const int64_t N = a.N;
+    constexpr int RB = SP_AL * 8;             // bytes per padded LDS row of A
+    constexpr int ASL = SP_TR * RB / 1024;    // 1 KB slices of the A tile
+    static_assert(SP_TR * RB % 1024 == 0, "A tile must be whole 1 KB slices");
+#pragma unroll
+    for (int i = wave; i < ASL; i += 8)
+    {
+        const int o = i * 1024 + lane * 16;
+        const int row = o / RB;
+        int col = o - row * RB;
+        if (col >= SP_KC * 8) col = 0;
+        const int n = n0 + row;
+        const int nn = n < a.nb ? n : a.nb - 1;
+        const char* src =
+            reinterpret_cast<const char*>(psik + (int64_t)(is * a.nb + nn) * N + c * SP_KC) + col;
+        __builtin_amdgcn_global_load_lds(src, LA + i * 1024, 16, 0, 0);
+    }
+    const __bf16* bsrc =
+        Bp + (((int64_t)is * a.nchunks + c) * NP * 2 * a.nbp + m0) * SP_KC;
+#pragma unroll
+    for (int i = wave; i < NP * 2 * 4; i += 8)
+    {
+        const int plane = i >> 2, part = i & 3;
+        const char* src = reinterpret_cast<const char*>(bsrc + (int64_t)plane * a.nbp * SP_KC) +
+                          part * 1024 + lane * 16;
+        __builtin_amdgcn_global_load_lds(src, LB + i * 1024, 16, 0, 0);
+    }
+    if (wave == 0 && lane < SP_KC / 2)
+    {
+        const c32* V = dv + ((int64_t)v * a.nmag + (a.nmag == 2 ? is : 0)) * N + c * SP_KC;
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const char*>(V) + lane * 16, LD, 16, 0,
+                                         0);
+    }
+}
+
+template <int NP, int NPROD, bool FAST>
 __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split(
     const c32* __restrict__ psik, const __bf16* __restrict__ Bp, const c32* __restrict__ dv,
     z64* __restrict__ part, SplitArgs a)
@@ -233,18 +276,39 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split(
         }
 
     SplitStage<NP> st;
+    lds_char* L = (lds_char*)sp_smem;
+    constexpr int A_BYTES = SP_TR * SP_AL * 8;
+    constexpr int B_BYTES = BELEMS * 2;
     int nfl = 0;
     for (int sp = 0; sp < a.nspinor; ++sp)
     {
         if (c0 >= c1) break;
-        stage_load<NP>(st, psik, Bp, dv, a, v, is, sp, n0, m0, c0, tid);
-        stage_store<NP>(st, sA0, sB0, sD0, tid);
+        if constexpr (FAST)
+        {
+            stage_async<NP>(psik, Bp, dv, a, v, is, n0, m0, c0, L, L + 2 * A_BYTES,
+                            L + 2 * A_BYTES + 2 * B_BYTES, wave, lane);
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+        else
+        {
+            stage_load<NP, FAST>(st, psik, Bp, dv, a, v, is, sp, n0, m0, c0, tid);
+            stage_store<NP>(st, sA0, sB0, sD0, tid);
+        }
         __syncthreads();
         int cur = 0;
         for (int64_t c = c0; c < c1; ++c)
         {
             const bool more = (c + 1 < c1);
-            if (more) stage_load<NP>(st, psik, Bp, dv, a, v, is, sp, n0, m0, c + 1, tid);
+            if (more)
+            {
+                if constexpr (FAST)
+                    stage_async<NP>(psik, Bp, dv, a, v, is, n0, m0, c + 1,
+                                    L + (cur ? 0 : A_BYTES), L + 2 * A_BYTES + (cur ? 0 : B_BYTES),
+                                    L + 2 * A_BYTES + 2 * B_BYTES + (cur ? 0 : SP_KC * 8), wave,
+                                    lane);
+                else
+                    stage_load<NP, FAST>(st, psik, Bp, dv, a, v, is, sp, n0, m0, c + 1, tid);
+            }
             const c32* sA = cur ? sA1 : sA0;
             const __bf16* sb = cur ? sB1 : sB0;
             const c32* sD = cur ? sD1 : sD0;
@@ -313,7 +377,10 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split(
             if (more)
             {
                 // the other buffer was last read before the previous barrier
-                stage_store<NP>(st, cur ? sA0 : sA1, cur ? sB0 : sB1, cur ? sD0 : sD1, tid);
+                if constexpr (FAST)
+                    __builtin_amdgcn_s_waitcnt(0);
+                else
+                    stage_store<NP>(st, cur ? sA0 : sA1, cur ? sB0 : sB1, cur ? sD0 : sD1, tid);
                 __syncthreads();
                 cur ^= 1;
             }
