@@ -111,8 +111,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the (k,q) pairs "
-                                   "shard over GPUs without collectives (see DESIGN.md)",
+        print(json.dumps({"error": "runs on one GPU only; the (k,q) pairs of a q shard "
+                                   "over GPUs without collectives, one process per GPU",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

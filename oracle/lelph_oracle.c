@@ -1303,3 +1303,23 @@ int orc_compute_elphq(const double* qpt, const orc_wfc* wfcs,
     free(kq);
     return errs;
 }
+
+/* ---- exported for the parity tests: the static helpers above ---- */
+void orc_su2rotate(int nspinor, int64_t npw, const double* su2, double* wfc)
+{
+    su2rotate(nspinor, npw, (const cplx*)su2, (cplx*)wfc);
+}
+
+void orc_apply_trans_wfc(const double* tr, const double* kvec, int64_t nsets,
+                         int64_t npw, const double* gvecs, double* wfc,
+                         int conjugate)
+{
+    apply_trans_wfc(tr, kvec, nsets, npw, gvecs, (cplx*)wfc, conjugate != 0);
+}
+
+void orc_sum_VNL_KKp(const double* K, const double* Kp, const double* fcoeff,
+                     int64_t nspin, int64_t nb, int64_t nspinor, double* out)
+{
+    sum_VNL_KKp((const cplx*)K, (const cplx*)Kp, (const cplx*)fcoeff, nspin,
+                nb, nspinor, (cplx*)out);
+}

@@ -98,6 +98,14 @@ void orc_set_threads(int n);
 void orc_set_inner_threads(int n);
 int orc_get_threads(void);
 
+/* static helpers, exported for the parity tests */
+void orc_su2rotate(int nspinor, int64_t npw, const double* su2, double* wfc);
+void orc_apply_trans_wfc(const double* tr, const double* kvec, int64_t nsets,
+                         int64_t npw, const double* gvecs, double* wfc,
+                         int conjugate);
+void orc_sum_VNL_KKp(const double* K, const double* Kp, const double* fcoeff,
+                     int64_t nspin, int64_t nb, int64_t nspinor, double* out);
+
 #ifdef __cplusplus
 }
 #endif

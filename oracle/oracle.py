@@ -79,6 +79,12 @@ def lib():
         L.orc_compute_elphq.argtypes = [c_double_p, C.POINTER(OrcWfc), C.POINTER(Lattice),
                                         C.POINTER(Pseudo), c_double_p, c_double_p, C.c_int,
                                         C.c_int, C.c_int64, C.c_int64, c_double_p]
+        L.orc_reciprocal_vecs.argtypes = [c_double_p, c_double_p]
+        L.orc_su2rotate.argtypes = [C.c_int, C.c_int64, c_double_p, c_double_p]
+        L.orc_apply_trans_wfc.argtypes = [c_double_p, c_double_p, C.c_int64, C.c_int64,
+                                          c_double_p, c_double_p, C.c_int]
+        L.orc_sum_VNL_KKp.argtypes = [c_double_p, c_double_p, c_double_p, C.c_int64,
+                                      C.c_int64, C.c_int64, c_double_p]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_inner_threads.argtypes = [C.c_int]
         _lib = L

@@ -1,0 +1,166 @@
+"""Independent numpy formulations used to pin the oracle (tests only).
+
+Everything here is written from the physics of the e-ph matrix element, not
+from the reference's code path:
+
+  * Bloch functions of full-BZ points are built from the iBZ coefficients by
+    rotating the state, psi_Sk(r) = psi_k(S^-1 r) (time reversal: the complex
+    conjugate), so u_kbz(r) = sum_G c'(G) exp(i 2pi (S_c G + G0) . x) with
+    G0 = S_c k_ibz - k_bz and x the fractional grid coordinate;
+  * the local matrix element is the real-space grid sum
+    g = (1/N) sum_x conj(u_k+q,m) exp(i 2pi Gu . x) dV u_k,n with the umklapp
+    Gu = k_bz + q - kq_bz;
+  * the non-local operator is assembled as a dense (K, K') matrix with the
+    Legendre addition theorem sum_m Y_lm(a) Y_lm(b) = (2l+1)/(4pi) P_l(a.b),
+    which holds for any real-harmonic convention.
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy.special import eval_legendre
+
+from letzelphc_amd import synth
+
+
+def sym_crys(s, isym):
+    R = np.asarray(s.syms[isym][0])
+    return synth._sym_crys(R, s.alat, s.blat)
+
+
+def frac_grid(dims):
+    ax = [np.arange(n) / n for n in dims]
+    return np.stack(np.meshgrid(*ax, indexing="ij"), -1)  # (Nx,Ny,Nz,3)
+
+
+def u_box(s, ibz_index, isym, kbz):
+    """(nspin, nb, Nx, Ny, Nz) periodic parts u_kbz(x) for nspinor == 1."""
+    assert s.nspinor == 1
+    w = s.wfcs[ibz_index]
+    Sc = sym_crys(s, isym)
+    tr = int(s.syms[isym][2])
+    G0 = Sc @ s.kpt_ibz_crys[ibz_index] - kbz
+    assert np.allclose(G0, np.rint(G0), atol=1e-6)
+    Gr = np.rint(w.gint @ Sc.T + G0[None, :]).astype(np.int64)
+    c = w.wfc[:, :, 0, :]
+    if tr:
+        c = np.conj(c)
+    dims = s.fft_dims
+    box = np.zeros((s.nspin, s.nbnds) + tuple(dims), np.complex128)
+    idx = tuple((Gr[:, i] % dims[i]) for i in range(3))
+    box[:, :, idx[0], idx[1], idx[2]] = c
+    N = int(np.prod(dims))
+    return np.fft.ifftn(box, axes=(2, 3, 4)) * N
+
+
+def elph_local(s, kbz_i, q, dv):
+    """g[nu, spin, n_k, m_kq] for full-BZ point kbz_i and q (crystal), dV
+    (nm, nmag, Nx, Ny, Nz) with nmag 1 or 2 (per spin)."""
+    kb = s.kpt_bz_crys
+    kqv = kb[kbz_i] + q
+    d = kb - kqv[None, :]
+    d = d - np.rint(d)
+    j = int(np.argmin(np.linalg.norm(d, axis=1)))
+    assert np.linalg.norm(d[j]) < 1e-5
+    Gu = kb[kbz_i] + q - kb[j]
+    uk = u_box(s, int(s.kmap[kbz_i][0]), int(s.kmap[kbz_i][1]), kb[kbz_i])
+    ukq = u_box(s, int(s.kmap[j][0]), int(s.kmap[j][1]), kb[j])
+    x = frac_grid(s.fft_dims)
+    ph = np.exp(2j * np.pi * (x @ np.rint(Gu)))
+    N = s.nfft
+    g = np.zeros((s.nmodes, s.nspin, s.nbnds, s.nbnds), np.complex128)
+    for v in range(s.nmodes):
+        for sp in range(s.nspin):
+            V = dv[v, sp if s.nmag == 2 else 0] * ph
+            A = (uk[sp] * V[None]).reshape(s.nbnds, -1)
+            B = ukq[sp].reshape(s.nbnds, -1)
+            g[v, sp] = A @ B.conj().T / N
+    return g
+
+
+def vnl_matrix(s, ikq, ik, qshift_tau=None):
+    """Per atom: K (npwK,3), K' (npwKp,3) cartesian without 2pi, and the
+    dense V_a(K, K') = sum_l sign(D_l) F_l(K) F_l(K') (2l+1)/(4pi)
+    P_l(K^.K'^) exp(-i 2pi (K - K').tau_a), identity symmetry."""
+    ps = s.pseudo
+    wK, wKp = s.wfcs[ikq], s.wfcs[ik]
+    K = s.kpt_ibz_cart[ikq][None, :] + wK.gvec
+    Kp = s.kpt_ibz_cart[ik][None, :] + wKp.gvec
+
+    def unit(v):
+        n = np.linalg.norm(v, axis=1, keepdims=True)
+        return np.where(n > 1e-12, v / np.where(n > 0, n, 1), 0.0)
+
+    cosg = unit(K) @ unit(Kp).T
+    pos = s.atomic_pos if qshift_tau is None else qshift_tau
+    out = []
+    for ia in range(s.natom):
+        t = int(s.atom_type[ia])
+        V = np.zeros((len(K), len(Kp)), np.complex128)
+        for li in range(ps.nltimesj):
+            l = int(round(ps.PP_table[li, t, 0] - 1))
+            sg = ps.Fsign[li, t]
+            V += (sg * (2 * l + 1) / (4 * np.pi)) * np.outer(wK.Fk[li, t], wKp.Fk[li, t]) \
+                * eval_legendre(l, np.clip(cosg, -1, 1))
+        ph = np.exp(-2j * np.pi * (K @ pos[ia]))[:, None] * np.exp(2j * np.pi * (Kp @ pos[ia]))[None, :]
+        out.append(V * ph)
+    return K, Kp, out
+
+
+def elph_nonlocal(s, ikq, ik, eig, pos=None):
+    """g_NL[nu, spin, n_k, m_kq] = e2 * d/dtau <psi_kq,m| V_NL |psi_k,n> along
+    the mode pattern, via the analytic derivative -i 2pi (K - K')."""
+    K, Kp, Vs = vnl_matrix(s, ikq, ik, pos)
+    ck = s.wfcs[ik].wfc[:, :, 0, :]
+    ckq = s.wfcs[ikq].wfc[:, :, 0, :]
+    g = np.zeros((s.nmodes, s.nspin, s.nbnds, s.nbnds), np.complex128)
+    for ia, V in enumerate(Vs):
+        for x in range(3):
+            D = (-2j * np.pi) * (K[:, x][:, None] - Kp[:, x][None, :]) * V
+            for sp in range(s.nspin):
+                M = ckq[sp].conj() @ D @ ck[sp].T  # (m, n)
+                g[:, sp] += synth.E2 * eig[:, 3 * ia + x][:, None, None] * M.T[None]
+    return g
+
+
+def vnl_expect(s, ikq, ik, pos):
+    """<psi_kq,m| V_NL(pos) |psi_k,n> per atom: (natom, nspin, n, m)."""
+    _, _, Vs = vnl_matrix(s, ikq, ik, pos)
+    ck = s.wfcs[ik].wfc[:, :, 0, :]
+    ckq = s.wfcs[ikq].wfc[:, :, 0, :]
+    out = np.zeros((s.natom, s.nspin, s.nbnds, s.nbnds), np.complex128)
+    for ia, V in enumerate(Vs):
+        for sp in range(s.nspin):
+            out[ia, sp] = (ckq[sp].conj() @ V @ ck[sp].T).T
+    return out
+
+
+def vloc_total_G(s, t, G):
+    """Closed form of V_loc(|G|) for the synthetic local potential
+    V(r) = -e2 Z erf(r)/r + A exp(-r^2/b):
+    (1/Omega) [-4pi e2 Z exp(-G^2/4)/G^2 + A (pi b)^(3/2) exp(-b G^2/4)]."""
+    ps = s.pseudo
+    Z = ps.loc[t][3]
+    G = np.asarray(G, np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lr = np.where(G > 1e-12, -4 * np.pi * synth.E2 * Z * np.exp(-G * G / 4) / (G * G), 0.0)
+    sr = ps.loc_A * (np.pi * ps.loc_b) ** 1.5 * np.exp(-ps.loc_b * G * G / 4)
+    return (lr + sr) / s.volume
+
+
+def dvlocq_closed(s, q, eig):
+    """(nm, Nx, Ny, Nz) bare local dV_q in the mode basis from the closed-form
+    V_loc: coefficients sum_{a,x} eig (-i)(q+G)_x exp(-i (q+G).tau_a) V(|q+G|)
+    on the box, unnormalised backward transform."""
+    dims = s.fft_dims
+    mi = [synth.miller_axis(n) for n in dims]
+    gx, gy, gz = np.meshgrid(mi[0] + q[0], mi[1] + q[1], mi[2] + q[2], indexing="ij")
+    qg = np.stack([gx, gy, gz], -1) @ s.blat.T
+    qn = np.linalg.norm(qg, axis=-1)
+    coef = np.zeros((s.nmodes,) + tuple(dims), np.complex128)
+    for ia in range(s.natom):
+        V = vloc_total_G(s, int(s.atom_type[ia]), qn)
+        ph = np.exp(-1j * (qg @ s.atomic_pos[ia]))
+        for x in range(3):
+            coef += eig[:, 3 * ia + x][:, None, None, None] * ((-1j) * qg[..., x] * ph * V)[None]
+    N = int(np.prod(dims))
+    return np.fft.ifftn(coef, axes=(1, 2, 3)) * N

@@ -1,0 +1,209 @@
+"""Writes tests/golden/ref_vectors.npz: inputs and outputs of the reference's
+own helper routines, compiled from its sources into oracle/_ref/liblelph_ref.so
+(`make -C oracle ref`, COMPILE_ELPH_DOUBLE):
+
+  legendre, Ylm, simpson             src/common/numerical_func.c:28-192
+  prepare_spline, spline_interpolate src/common/spline.c:13-139
+  reciprocal_vecs                    src/common/numerical_func.c
+  get_KplusQ_idxs                    src/common/numerical_func.c:471-524
+  SU2mat                             src/wfc/su2mat.c:16-117
+  su2rotate                          src/wfc/su2rotate.c:13-41
+  apply_trans_wfc                    src/wfc/trans_wfc.c:15-52
+  Vloc_Gspace                        src/dvloc/VlocG_atom.c:10-57
+  sum_VNL_KKp                        src/nonloc/sum_VNL_KKp.c:8-58
+
+Run:  python tests/golden/make_ref_vectors.py
+The fixture is data (seeded inputs, the reference's outputs); the tests
+compare the oracle's restatement against it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "liblelph_ref.so")
+OUT = os.path.join(HERE, "ref_vectors.npz")
+
+dp = C.POINTER(C.c_double)
+ll = C.c_longlong
+
+
+def P(a):
+    return a.ctypes.data_as(dp)
+
+
+def load_ref():
+    L = C.CDLL(REF_LIB)
+    L.legendre.restype = C.c_double
+    L.legendre.argtypes = [C.c_int, C.c_int, C.c_double]
+    L.Ylm.restype = C.c_double
+    L.Ylm.argtypes = [C.c_int, C.c_int, dp]
+    L.simpson.restype = C.c_double
+    L.simpson.argtypes = [dp, dp, ll]
+    L.prepare_spline.argtypes = [ll, dp, dp, dp]
+    L.spline_interpolate.restype = C.c_double
+    L.spline_interpolate.argtypes = [C.c_double, ll, dp, dp, dp]
+    L.reciprocal_vecs.argtypes = [dp, dp]
+    L.get_KplusQ_idxs.argtypes = [ll, dp, dp, C.POINTER(C.c_int)]
+    L.SU2mat.argtypes = [dp, ll, C.c_bool, C.c_bool, dp]
+    L.su2rotate.argtypes = [C.c_int, ll, ll, dp, dp]
+    L.apply_trans_wfc.argtypes = [dp, dp, ll, ll, dp, dp, C.c_bool]
+    L.Vloc_Gspace.restype = C.c_double
+    L.Vloc_Gspace.argtypes = [dp, C.c_char, C.c_double, ll, dp, dp, dp, C.c_double,
+                              C.c_double, C.c_double]
+    L.sum_VNL_KKp.argtypes = [dp, dp, dp, ll, ll, ll, dp]
+    return L
+
+
+def rotations(rng, n):
+    """n random proper rotations and their improper partners."""
+    out = []
+    for _ in range(n):
+        q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        if np.linalg.det(q) < 0:
+            q = -q
+        out.append(q)
+        out.append(-q)
+    # special cases of the Euler-angle branches: identity, C2 about z, C4 about z
+    out.append(np.eye(3))
+    out.append(np.diag([-1.0, -1.0, 1.0]))
+    out.append(np.array([[0.0, -1, 0], [1, 0, 0], [0, 0, 1]]))
+    out.append(np.diag([1.0, -1.0, -1.0]))
+    return np.array(out)
+
+
+def make(L, seed=2024):
+    rng = np.random.default_rng(seed)
+    f = {}
+    # legendre / Ylm
+    lm = [(l, m) for l in range(5) for m in range(-l, l + 1)]
+    xs = np.concatenate([rng.uniform(-1, 1, 9), [-1.0, 0.0, 1.0]])
+    f["leg_lm"] = np.array(lm, np.int64)
+    f["leg_x"] = xs
+    f["leg_out"] = np.array([[L.legendre(l, m, x) for x in xs] for l, m in lm])
+    vecs = np.concatenate([rng.standard_normal((10, 3)),
+                           [[0, 0, 0], [0, 0, 1.5], [0, 0, -2.0], [1e-7, 0, 0],
+                            [0.3, 0, 0], [0, -0.4, 0]]])
+    ylm = [(l, m) for l in range(4) for m in range(-l, l + 1)]
+    f["ylm_lm"] = np.array(ylm, np.int64)
+    f["ylm_v"] = vecs
+    out = np.zeros((len(ylm), len(vecs)))
+    for i, (l, m) in enumerate(ylm):
+        for j, v in enumerate(vecs):
+            vv = np.ascontiguousarray(v, np.float64)
+            out[i, j] = L.Ylm(l, m, P(vv))
+    f["ylm_out"] = out
+    # simpson on odd and even point counts
+    for n in (5, 6, 101, 1000):
+        fv = rng.standard_normal(n)
+        dx = rng.uniform(0.01, 0.02, n)
+        f[f"simp{n}_f"] = fv
+        f[f"simp{n}_dx"] = dx
+        f[f"simp{n}_out"] = np.array(L.simpson(P(fv), P(dx), n))
+    # spline on a uniform table (as the V_loc(|G|) table)
+    n = 40
+    x = np.linspace(0.0, 3.9, n)
+    y = np.sin(1.3 * x) * np.exp(-0.2 * x) + 0.1 * rng.standard_normal(n)
+    dy = np.zeros(n)
+    L.prepare_spline(n, P(x), P(y), P(dy))
+    xi = np.sort(rng.uniform(0.0, 3.89, 25))
+    yi = np.array([L.spline_interpolate(t, int(np.floor(t / (x[1] - x[0]))), P(x), P(y), P(dy))
+                   for t in xi])
+    f.update(spl_x=x, spl_y=y, spl_dy=dy, spl_xi=xi, spl_yi=yi)
+    # reciprocal vectors (lattice vectors as columns, as struct Lattice)
+    lat = rng.standard_normal((3, 3)) + 4 * np.eye(3)
+    blat = np.zeros((3, 3))
+    L.reciprocal_vecs(P(np.ascontiguousarray(lat)), P(blat))
+    f.update(rec_lat=lat, rec_blat=blat)
+    # k+q indices on a shuffled 4x4x3 grid
+    g = np.stack(np.meshgrid(np.arange(4) / 4, np.arange(4) / 4, np.arange(3) / 3,
+                             indexing="ij"), -1).reshape(-1, 3)
+    g = g[rng.permutation(len(g))] + rng.integers(-1, 2, g.shape)
+    g = np.ascontiguousarray(g)
+    qs = np.array([[0.25, 0.0, 1 / 3], [-0.5, 0.75, 0.0], [0.0, 0.0, 0.0]])
+    idx = np.zeros((len(qs), len(g)), np.int32)
+    for i, q in enumerate(qs):
+        qq = np.ascontiguousarray(q)
+        L.get_KplusQ_idxs(len(g), P(g), P(qq), idx[i].ctypes.data_as(C.POINTER(C.c_int)))
+    f.update(kq_k=g, kq_q=qs, kq_idx=idx)
+    # SU(2) matrices
+    R = rotations(rng, 4)
+    su = np.zeros((len(R), 2, 2, 2, 2), np.complex128)
+    for i, r in enumerate(R):
+        rr = np.ascontiguousarray(r)
+        for inv in (0, 1):
+            for tr in (0, 1):
+                m = np.zeros(4, np.complex128)
+                L.SU2mat(P(rr), 2, bool(inv), bool(tr), m.ctypes.data_as(dp))
+                su[i, inv, tr] = m.reshape(2, 2)
+    f.update(su2_R=R, su2_out=su)
+    # su2rotate
+    npw, nsets = 17, 3
+    w = (rng.standard_normal((nsets, 2, npw)) + 1j * rng.standard_normal((nsets, 2, npw)))
+    w = np.ascontiguousarray(w)
+    m = np.ascontiguousarray(su[0, 0, 0].reshape(4))
+    wo = w.copy()
+    L.su2rotate(2, npw, nsets, m.ctypes.data_as(dp), wo.ctypes.data_as(dp))
+    f.update(su2rot_su=m, su2rot_in=w, su2rot_out=wo)
+    # apply_trans_wfc
+    tau = rng.standard_normal(3)
+    kv = rng.standard_normal(3)
+    gv = np.ascontiguousarray(rng.integers(-3, 4, (npw, 3)).astype(np.float64) * 0.37)
+    ws = np.ascontiguousarray(rng.standard_normal((nsets, npw)) + 1j * rng.standard_normal((nsets, npw)))
+    outs = []
+    for cj in (0, 1):
+        t = ws.copy()
+        L.apply_trans_wfc(P(tau), P(kv), nsets, npw, P(gv), t.ctypes.data_as(dp), bool(cj))
+        outs.append(t)
+    f.update(tr_tau=tau, tr_k=kv, tr_g=gv, tr_in=ws, tr_out=np.array(outs))
+    # Vloc_Gspace on the synthetic local pseudopotential
+    from letzelphc_amd import synth
+    ps = synth.tiny(with_wfc=False).pseudo
+    v, r, rab, Z = ps.loc[1]
+    vol = 123.4
+    Gs = np.array([0.0, 5e-6, 0.05, 0.7, 2.3, 9.1])
+    vg = np.zeros((2, len(Gs)))
+    work = np.zeros(len(r) + 1)
+    v = np.ascontiguousarray(v)
+    r = np.ascontiguousarray(r)
+    rab = np.ascontiguousarray(rab)
+    for ic, cut in enumerate((b"3", b"2")):
+        for j, G in enumerate(Gs):
+            vg[ic, j] = L.Vloc_Gspace(P(work), cut, G, len(r), P(v), P(r), P(rab), Z, 1.0, vol)
+    f.update(vl_v=v, vl_r=r, vl_rab=rab, vl_Z=np.array(Z), vl_vol=np.array(vol), vl_G=Gs,
+             vl_out=vg)
+    # sum_VNL_KKp
+    for nspinor in (1, 2):
+        nspin, nb = 2, 3
+        sh = (4, nspin, nb, nspinor)
+        K = np.ascontiguousarray(rng.standard_normal(sh) + 1j * rng.standard_normal(sh))
+        Kp = np.ascontiguousarray(rng.standard_normal(sh) + 1j * rng.standard_normal(sh))
+        fc = np.ascontiguousarray(rng.standard_normal(nspinor * nspinor)
+                                  + 1j * rng.standard_normal(nspinor * nspinor))
+        o0 = np.ascontiguousarray(rng.standard_normal((3, nspin, nb, nb))
+                                  + 1j * rng.standard_normal((3, nspin, nb, nb)))
+        o = o0.copy()
+        L.sum_VNL_KKp(K.ctypes.data_as(dp), Kp.ctypes.data_as(dp), fc.ctypes.data_as(dp),
+                      nspin, nb, nspinor, o.ctypes.data_as(dp))
+        f.update({f"vnl{nspinor}_K": K, f"vnl{nspinor}_Kp": Kp, f"vnl{nspinor}_f": fc,
+                  f"vnl{nspinor}_in": o0, f"vnl{nspinor}_out": o})
+    return f
+
+
+def main():
+    if not os.path.exists(REF_LIB):
+        raise SystemExit(f"{REF_LIB} missing: run `make -C oracle ref` first")
+    f = make(load_ref())
+    np.savez_compressed(OUT, **f)
+    print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {len(f)} arrays)")
+
+
+if __name__ == "__main__":
+    main()

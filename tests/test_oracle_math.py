@@ -1,0 +1,271 @@
+"""Pins of the FP64 oracle by mathematics: brute-force DFT, orthonormality,
+SU(2) homomorphism, real-space Parseval form of elphLocal with explicitly
+rotated Bloch functions, closed forms (constant dV, Gaussian local
+potential), translation covariance, the non-local derivative sum rule and
+finite differences.  Each check is chosen so that a dropped term, a wrong
+sign or index, or a transposed operand in the oracle fails it.
+"""
+import ctypes as C
+import dataclasses
+
+import numpy as np
+import pytest
+
+from letzelphc_amd import synth
+from letzelphc_amd.abi import dptr
+from oracle import oracle as O
+
+from . import bruteforce as BF
+from .util import rel_err, say
+
+
+# ---------------------------------------------------------------- FFT
+def _dft_matrix(n, sign):
+    j = np.arange(n)
+    return np.exp(sign * 2j * np.pi * np.outer(j, j) / n)
+
+
+@pytest.mark.parametrize("dims", [(3, 4, 5), (6, 6, 6), (7, 1, 9), (12, 10, 8), (2, 27, 25)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft_vs_bruteforce(dims, sign):
+    say(f"case: oracle fft3d {dims} sign {sign}")
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(dims) + 1j * rng.standard_normal(dims)
+    ref = x
+    for ax, n in enumerate(dims):
+        ref = np.moveaxis(np.tensordot(_dft_matrix(n, sign), np.moveaxis(ref, ax, 0), 1), 0, ax)
+    got = O.fft3d(x, sign)
+    assert rel_err(got, ref) < 1e-13
+
+
+def test_fft_plane_wave_and_conj():
+    """coefficient 1 at G -> exp(+i 2pi G.x) on the box (backward, unnormalised);
+    backward(conj c) = conj of the index-reversed backward(c)."""
+    say("case: oracle fft3d plane wave / conj identity")
+    dims = (8, 9, 10)
+    G = (3, -2, 4)
+    c = np.zeros(dims, np.complex128)
+    c[G[0] % 8, G[1] % 9, G[2] % 10] = 1
+    x = BF.frac_grid(dims)
+    assert rel_err(O.fft3d(c, 1), np.exp(2j * np.pi * (x @ np.array(G, float)))) < 1e-13
+    rng = np.random.default_rng(2)
+    c = rng.standard_normal(dims) + 1j * rng.standard_normal(dims)
+    a = O.fft3d(np.conj(c), 1)
+    b = np.conj(O.fft3d(c, 1))
+    b = np.roll(np.flip(b, (0, 1, 2)), 1, (0, 1, 2))
+    assert rel_err(a, b) < 1e-13
+
+
+# ---------------------------------------------------------------- Y_lm, SU(2)
+def test_ylm_orthonormal_and_addition():
+    say("case: oracle Y_lm orthonormality, addition theorem")
+    L = O.lib()
+    ct, wt = np.polynomial.legendre.leggauss(12)
+    ph = np.arange(16) * 2 * np.pi / 16
+    pts, w = [], []
+    for c, a in zip(ct, wt):
+        s = np.sqrt(1 - c * c)
+        for p in ph:
+            pts.append([s * np.cos(p), s * np.sin(p), c])
+            w.append(a * 2 * np.pi / 16)
+    pts = np.array(pts) * 1.7
+    w = np.array(w)
+    lm = [(l, m) for l in range(4) for m in range(-l, l + 1)]
+    Y = np.array([[L.orc_Ylm(l, m, dptr(np.ascontiguousarray(p))) for p in pts] for l, m in lm])
+    S = (Y * w[None]) @ Y.T
+    # the normalisation carries ELPH_PI = 3.1415927: 1.5e-8 relative
+    assert np.max(np.abs(S - np.eye(len(lm)))) < 3e-8
+    assert np.max(np.abs(S - np.diag(np.diag(S)))) < 1e-13
+    rng = np.random.default_rng(3)
+    a, b = rng.standard_normal(3), rng.standard_normal(3)
+    ca = np.dot(a, b) / np.linalg.norm(a) / np.linalg.norm(b)
+    from scipy.special import eval_legendre
+    for l in range(4):
+        s = sum(L.orc_Ylm(l, m, dptr(a)) * L.orc_Ylm(l, m, dptr(b)) for m in range(-l, l + 1))
+        assert abs(s - (2 * l + 1) / (4 * np.pi) * eval_legendre(l, ca)) < 3e-8
+
+
+def test_su2_homomorphism():
+    """U is in SU(2) and reproduces its rotation: R_ij = Tr(s_i U s_j U^+)/2."""
+    say("case: oracle SU2mat reproduces R")
+    L = O.lib()
+    sig = [np.array([[0, 1], [1, 0]]), np.array([[0, -1j], [1j, 0]]), np.diag([1, -1])]
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        R, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        if np.linalg.det(R) < 0:
+            R = -R
+        for Rs in (R, -R):  # improper: the SU(2) of the proper part
+            U = np.zeros(4, np.complex128)
+            assert L.orc_SU2mat(dptr(np.ascontiguousarray(Rs)), 2, 0, 0, dptr(U)) == 0
+            U = U.reshape(2, 2)
+            assert np.allclose(U @ U.conj().T, np.eye(2), atol=1e-12)
+            assert abs(np.linalg.det(U) - 1) < 1e-12
+            M = np.array([[0.5 * np.trace(sig[i] @ U @ sig[j] @ U.conj().T).real
+                           for j in range(3)] for i in range(3)])
+            assert np.allclose(M, R, atol=2e-7)  # ELPH_PI in the Euler angles
+
+
+# ---------------------------------------------------------------- elphLocal
+def _rand_dv(s, seed):
+    rng = np.random.default_rng(seed)
+    sh = (s.nmodes, s.nmag) + tuple(s.fft_dims)
+    return rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
+
+
+@pytest.mark.parametrize("kind", ["id", "trev", "cubic", "spin2"])
+def test_elph_local_vs_realspace(kind):
+    """Oracle (G-space convolution, symmetry-rotated spheres, umklapp) against
+    the real-space sum over explicitly rotated Bloch functions."""
+    kw = dict(nbnds=3, kgrid=(2, 2, 2))
+    if kind == "cubic":
+        kw.update(syms="cubic", q=(0.0, 0.5, 0.0))
+    elif kind == "spin2":
+        kw.update(syms="trev", nspin=2, nmag=2)
+    else:
+        kw.update(syms=kind)
+    s = synth.tiny(**kw)
+    o = O.OracleSystem(s)
+    dv = _rand_dv(s, 11)
+    q = np.asarray(s.qpts[0])
+    pairs = synth.kq_pairs(s, 0)
+    say(f"case: oracle elphLocal vs real space, {kind}, {len(pairs)} k")
+    for i, pr in enumerate(pairs):
+        g = o.elph_local(0, *pr, dv)
+        ref = BF.elph_local(s, i, q, dv)
+        assert rel_err(g, ref) < 1e-11, (kind, i, pr)
+
+
+def test_elph_local_constant_dv():
+    """q = 0, dV_nu = c_nu: g[nu][n][m] = c_nu delta_nm (orthonormal bands),
+    for every symmetry-rotated and time-reversed k."""
+    s = synth.tiny(nbnds=4, syms="cubic", q=(0.0, 0.0, 0.0))
+    o = O.OracleSystem(s)
+    c = np.arange(1, s.nmodes + 1) * (0.5 - 0.25j)
+    dv = np.ones((s.nmodes, s.nmag) + tuple(s.fft_dims), np.complex128) * c[:, None, None, None, None]
+    say("case: oracle elphLocal constant dV closed form")
+    for pr in synth.kq_pairs(s, 0):
+        g = o.elph_local(0, *pr, dv)
+        ref = c[:, None, None, None] * np.eye(s.nbnds)[None, None]
+        assert np.max(np.abs(g - ref)) < 1e-12
+
+
+def test_elph_local_gauge_invariance():
+    """sum |g|^2 is unchanged by unitary band rotations of psi_k and psi_k+q."""
+    s = synth.tiny(nbnds=4, syms="id")
+    o = O.OracleSystem(s)
+    dv = _rand_dv(s, 12)
+    pr = synth.kq_pairs(s, 0)[3]
+    g0 = o.elph_local(0, *pr, dv)
+    rng = np.random.default_rng(9)
+    for ik in {pr[0], pr[1]}:
+        U, _ = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))
+        w = s.wfcs[ik].wfc
+        w[0] = np.einsum("ab,bsg->asg", U, w[0])
+    o2 = O.OracleSystem(s)
+    g1 = o2.elph_local(0, *pr, dv)
+    say("case: oracle elphLocal gauge invariance")
+    a = np.sum(np.abs(g0) ** 2, axis=(1, 2, 3))
+    b = np.sum(np.abs(g1) ** 2, axis=(1, 2, 3))
+    assert np.max(np.abs(a - b) / a) < 1e-12
+    assert rel_err(g1, g0) > 1e-3  # the elements themselves did change
+
+
+# ---------------------------------------------------------------- dVlocq
+def test_dvlocq_closed_form():
+    """Bare local dV against the Gaussian/erf closed form of V_loc(G): pins
+    the table, spline, long-range add-back, phases and mode contraction to
+    the table's interpolation accuracy."""
+    s = synth.tiny(nbnds=2)
+    o = O.OracleSystem(s)
+    say("case: oracle dVlocq vs closed form")
+    for q in (s.qpts[0], np.array([0.0, 0.0, 0.0]), np.array([0.25, -0.5, 0.125])):
+        got = o.dvlocq(0, qpt=q)
+        ref = BF.dvlocq_closed(s, q, s.eigvec(0))
+        assert rel_err(got, ref) < 2e-6, q
+
+
+def test_dvlocq_translation_covariance():
+    """Atoms moved by a grid vector t: dV'_q(x) = exp(-i q.t) dV_q(x - t)."""
+    s = synth.tiny(nbnds=2)
+    o = O.OracleSystem(s)
+    q = np.array([0.25, -0.5, 0.125])
+    shift = np.array([3, 1, 5])
+    t_crys = shift / np.array(s.fft_dims)
+    t = s.alat @ t_crys
+    s2 = dataclasses.replace(s, atomic_pos=s.atomic_pos + t[None, :])
+    o2 = O.OracleSystem(s2)
+    say("case: oracle dVlocq translation covariance")
+    a = o.dvlocq(0, qpt=q)
+    b = o2.dvlocq(0, qpt=q)
+    ph = np.exp(-1j * np.dot(s.blat @ q, t))
+    ref = ph * np.roll(a, tuple(shift), axis=(1, 2, 3))
+    assert rel_err(b, ref) < 1e-7  # blat carries the truncated pi
+
+
+# ---------------------------------------------------------------- non-local
+def test_nonlocal_vs_dense_operator():
+    """add_elphNonLocal against the dense V_NL(K, K') assembled with the
+    Legendre addition theorem and differentiated analytically."""
+    s = synth.tiny(nbnds=3, syms="id", kgrid=(2, 1, 2))
+    o = O.OracleSystem(s)
+    eig = s.eigvec(0)
+    say("case: oracle non-local vs dense operator")
+    for pr in synth.kq_pairs(s, 0):
+        ikq, ik = pr[0], pr[1]
+        z = np.zeros((s.nmodes, s.nspin, s.nbnds, s.nbnds), np.complex128)
+        g = o.add_nonlocal(0, *pr, z)
+        ref = BF.elph_nonlocal(s, ikq, ik, eig)
+        assert rel_err(g, ref) < 1e-6, pr  # ELPH_PI in the projector phases
+
+
+def test_nonlocal_finite_difference():
+    """q = 0: e2 d<V_NL>/dtau by central differences equals the oracle's
+    g_NL for unit displacement patterns (one atom, one direction)."""
+    s = synth.tiny(nbnds=3, syms="id", kgrid=(2, 1, 1), q=(0.0, 0.0, 0.0))
+    o = O.OracleSystem(s)
+    eye = np.eye(s.nmodes, dtype=np.complex128)
+    pr = synth.kq_pairs(s, 0)[1]
+    ikq, ik = pr[0], pr[1]
+    z = np.zeros((s.nmodes, s.nspin, s.nbnds, s.nbnds), np.complex128)
+    lib = O.lib()
+    g = z.copy()
+    assert lib.orc_add_elphNonLocal(o.wtab, o.lat, o.pp, ikq, ik, pr[2], pr[3], dptr(eye),
+                                    dptr(g)) == 0
+    say("case: oracle non-local finite differences")
+    h = 1e-5
+    for ia in range(s.natom):
+        for x in range(3):
+            p1 = s.atomic_pos.copy()
+            p2 = s.atomic_pos.copy()
+            p1[ia, x] += h
+            p2[ia, x] -= h
+            fd = (BF.vnl_expect(s, ikq, ik, p1)[ia] - BF.vnl_expect(s, ikq, ik, p2)[ia]) / (2 * h)
+            fd *= synth.E2
+            ref = g[3 * ia + x]
+            assert np.max(np.abs(fd - ref)) < 1e-5 * max(1.0, np.max(np.abs(ref)))
+
+
+# ---------------------------------------------------------------- q-star rotation
+def test_elph_q_rotate_identity_and_norm():
+    """D = 1 returns g (conj(g) under time reversal); unitary D keep the
+    Frobenius norm per (mode, spin)."""
+    say("case: oracle elph_q_rotate invariants")
+    rng = np.random.default_rng(6)
+    nm, ns, nb = 3, 2, 4
+    g = rng.standard_normal((nm, ns, nb, nb)) + 1j * rng.standard_normal((nm, ns, nb, nb))
+    I = np.ascontiguousarray(np.broadcast_to(np.eye(nb, dtype=np.complex128), (ns, nb, nb)))
+    L = O.lib()
+    for tr in (0, 1):
+        out = np.zeros_like(g)
+        L.orc_elph_q_rotate(dptr(I), dptr(g), dptr(I), nm, ns, nb, tr, dptr(out))
+        assert rel_err(out, np.conj(g) if tr else g) < 1e-15
+    U = []
+    for _ in range(2):
+        m = rng.standard_normal((ns, nb, nb)) + 1j * rng.standard_normal((ns, nb, nb))
+        U.append(np.ascontiguousarray(np.linalg.qr(m)[0]))
+    out = np.zeros_like(g)
+    L.orc_elph_q_rotate(dptr(U[0]), dptr(g), dptr(U[1]), nm, ns, nb, 0, dptr(out))
+    a = np.sum(np.abs(out) ** 2, axis=(2, 3))
+    b = np.sum(np.abs(g) ** 2, axis=(2, 3))
+    assert np.max(np.abs(a - b) / b) < 1e-13

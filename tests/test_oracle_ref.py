@@ -1,0 +1,128 @@
+"""The oracle's helper routines against the reference's own compiled code.
+
+tests/golden/ref_vectors.npz holds seeded inputs and the outputs of the
+reference's legendre/Ylm/simpson (src/common/numerical_func.c), spline
+(src/common/spline.c), SU2mat/su2rotate/apply_trans_wfc (src/wfc/),
+Vloc_Gspace (src/dvloc/VlocG_atom.c) and sum_VNL_KKp
+(src/nonloc/sum_VNL_KKp.c), written by tests/golden/make_ref_vectors.py from
+oracle/_ref/liblelph_ref.so.  When that library is present the same checks
+also run live on a second seed.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from letzelphc_amd.abi import dptr
+from oracle import oracle as O
+
+from .util import say
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(HERE, "golden", "ref_vectors.npz")
+
+
+def _close(a, b, tol=1e-13):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    scale = max(1.0, float(np.max(np.abs(b))))
+    assert float(np.max(np.abs(a - b))) <= tol * scale, (a, b)
+
+
+def check_against(f):
+    L = O.lib()
+    # legendre, Ylm
+    for i, (l, m) in enumerate(f["leg_lm"]):
+        got = [L.orc_legendre(int(l), int(m), float(x)) for x in f["leg_x"]]
+        _close(got, f["leg_out"][i])
+    for i, (l, m) in enumerate(f["ylm_lm"]):
+        got = []
+        for v in f["ylm_v"]:
+            vv = np.ascontiguousarray(v, np.float64)
+            got.append(L.orc_Ylm(int(l), int(m), dptr(vv)))
+        _close(got, f["ylm_out"][i])
+    # simpson
+    for n in (5, 6, 101, 1000):
+        fv = np.ascontiguousarray(f[f"simp{n}_f"])
+        dx = np.ascontiguousarray(f[f"simp{n}_dx"])
+        _close(L.orc_simpson(dptr(fv), dptr(dx), n), f[f"simp{n}_out"])
+    # spline
+    x = np.ascontiguousarray(f["spl_x"])
+    y = np.ascontiguousarray(f["spl_y"])
+    dy = np.zeros_like(x)
+    L.orc_prepare_spline(len(x), dptr(x), dptr(y), dptr(dy))
+    _close(dy, f["spl_dy"], 1e-12)
+    h = x[1] - x[0]
+    yi = [L.orc_spline_interpolate(float(t), int(np.floor(t / h)), dptr(x), dptr(y), dptr(dy))
+          for t in f["spl_xi"]]
+    _close(yi, f["spl_yi"], 1e-12)
+    # reciprocal vectors
+    lat = np.ascontiguousarray(f["rec_lat"])
+    b = np.zeros((3, 3))
+    L.orc_reciprocal_vecs(dptr(lat), dptr(b))
+    _close(b, f["rec_blat"])
+    # k+q indices
+    k = np.ascontiguousarray(f["kq_k"])
+    for i, q in enumerate(f["kq_q"]):
+        idx = np.zeros(len(k), np.int32)
+        qq = np.ascontiguousarray(q)
+        assert L.orc_get_KplusQ_idxs(len(k), dptr(k), dptr(qq),
+                                     idx.ctypes.data_as(C.POINTER(C.c_int))) == 0
+        np.testing.assert_array_equal(idx, f["kq_idx"][i])
+    # SU(2)
+    for i, r in enumerate(f["su2_R"]):
+        rr = np.ascontiguousarray(r)
+        for inv in (0, 1):
+            for tr in (0, 1):
+                m = np.zeros(4, np.complex128)
+                assert L.orc_SU2mat(dptr(rr), 2, inv, tr, dptr(m)) == 0
+                _close(m.reshape(2, 2), f["su2_out"][i, inv, tr], 1e-12)
+    w = f["su2rot_in"].copy()
+    su = np.ascontiguousarray(f["su2rot_su"])
+    for s in range(w.shape[0]):
+        ws = np.ascontiguousarray(w[s])
+        L.orc_su2rotate(2, w.shape[2], dptr(su), dptr(ws))
+        w[s] = ws
+    _close(w, f["su2rot_out"])
+    # translation phase
+    for cj in (0, 1):
+        t = np.ascontiguousarray(f["tr_in"]).copy()
+        L.orc_apply_trans_wfc(dptr(np.ascontiguousarray(f["tr_tau"])),
+                              dptr(np.ascontiguousarray(f["tr_k"])), t.shape[0], t.shape[1],
+                              dptr(np.ascontiguousarray(f["tr_g"])), dptr(t), cj)
+        _close(t, f["tr_out"][cj], 1e-12)
+    # V_loc(G) radial integral
+    v, r, rab = (np.ascontiguousarray(f[k]) for k in ("vl_v", "vl_r", "vl_rab"))
+    work = np.zeros(len(r) + 1)
+    for ic, cut in enumerate((b"3", b"2")):
+        got = [L.orc_Vloc_Gspace(dptr(work), cut, float(G), len(r), dptr(v), dptr(r), dptr(rab),
+                                 float(f["vl_Z"]), 1.0, float(f["vl_vol"])) for G in f["vl_G"]]
+        _close(got, f["vl_out"][ic], 1e-12)
+    # sum_VNL_KKp
+    for nspinor in (1, 2):
+        K = np.ascontiguousarray(f[f"vnl{nspinor}_K"])
+        Kp = np.ascontiguousarray(f[f"vnl{nspinor}_Kp"])
+        fc = np.ascontiguousarray(f[f"vnl{nspinor}_f"])
+        o = np.ascontiguousarray(f[f"vnl{nspinor}_in"]).copy()
+        nspin, nb = K.shape[1], K.shape[2]
+        L.orc_sum_VNL_KKp(dptr(K), dptr(Kp), dptr(fc), nspin, nb, nspinor, dptr(o))
+        _close(o, f[f"vnl{nspinor}_out"])
+
+
+def test_oracle_vs_reference_fixture():
+    say("case: oracle helpers vs reference fixture")
+    with np.load(FIXTURE, allow_pickle=False) as z:
+        f = {k: z[k] for k in z.files}
+    check_against(f)
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_LIB), reason="oracle/_ref not built")
+def test_oracle_vs_reference_live():
+    say("case: oracle helpers vs reference library, second seed")
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "make_ref_vectors", os.path.join(HERE, "golden", "make_ref_vectors.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    check_against(mod.make(mod.load_ref(), seed=99))
