@@ -30,6 +30,7 @@ if ROOT not in sys.path:
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "bf16x6": 2500.0}
 PRODUCTS = {"fp64": 1, "fp32": 1, "bf16x3": 3, "bf16x6": 6}
+TOL = 1e-6  # |dg|max / |g|max of a headline variant against the GPU FP64 path
 
 
 def log(msg):
@@ -108,7 +109,20 @@ def main():
     ap.add_argument("--nk", type=int, default=0, help="pairs per step (0: all k)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-cores", type=int, default=0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 3 device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_probe > 0:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n}), flush=True)
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "runs on one GPU only; the (k,q) pairs of a q shard "
@@ -144,7 +158,7 @@ def main():
         err = rel_err(gv, g64)
         results[v] = (wv, cv, err)
         log(f"{v}: |dg|/|g|max vs FP64 = {err:.3e}")
-        if head is None and err <= 1e-6:
+        if err <= TOL and (head is None or np.mean(wv) < np.mean(results[head][0])):
             head = v
     if head is None:
         head = "fp64"
@@ -160,6 +174,14 @@ def main():
     achieved = flop_launch / t_launch / 1e12
     peak = PEAK_TFLOPS[head]
     t64_launch = float(np.mean(c64)) / len(pairs) / 1e3
+
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "k_contract_split_bf16x3_c2.md")
+    if args.config == "c2" and os.path.exists(prof):
+        import re
+        m = re.search(rf"traffic_bytes_per_launch\[{head}\] = (\d+)", open(prof).read())
+        if m:
+            traffic = int(m.group(1))
 
     cpu = None
     if not args.no_cpu:
@@ -179,7 +201,7 @@ def main():
                 "pseudopotentials)",
         "config": {"workload": f"{args.config}: 16-atom cubic a=20.5 bohr, ecut 40 Ry, FFT "
                                f"{s.fft_dims[0]}x{s.fft_dims[1]}x{s.fft_dims[2]}, "
-                               f"{len(pairs)} (k,k+q) pairs of q={tuple(s.qpts[0])}, "
+                               f"{len(pairs)} (k,k+q) pairs of q={tuple(float(x) for x in s.qpts[0])}, "
                                f"{s.nmodes} modes, {s.nbnds} bands, npw {min(npw)}..{max(npw)}",
                    "variant": head, "pairs": len(pairs)},
         "variants": {k: {"g_per_s": nel / float(np.mean(v[0])), "rel_err_vs_fp64": v[2],
@@ -189,9 +211,13 @@ def main():
                  "contract_tflops": flop_launch / t64_launch / 1e12},
         "ratio_vs_fp64": value / v64,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": f"k_contract ({head}, {PRODUCTS[head]} MFMA product(s) per "
-                               f"complex term; achieved counts the 8 useful flops per MAC)"},
+                     "frac": achieved / peak, "traffic": traffic,
+                     "kernel": (("k_contract_split" if head.startswith("bf16") else "k_contract")
+                                + f" ({head}, {PRODUCTS[head]} MFMA product(s) per real product; "
+                                  "achieved counts the 8 useful flops per complex MAC)"),
+                     "traffic_source": ("FETCH_SIZE + WRITE_SIZE PMC passes, copy-calibrated, "
+                                        "profiles/k_contract_split_bf16x3_c2.md")
+                     if traffic else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
