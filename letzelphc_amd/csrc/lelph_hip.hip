@@ -19,6 +19,7 @@
 #include "fft.hpp"
 #include "nonloc.hpp"
 #include "split.hpp"
+#include "splitpc.hpp"
 #include "wfc.hpp"
 
 namespace
@@ -494,7 +495,8 @@ static int run_local_split_t(lelph_ctx* c, const double* qpt, const lelph_pair& 
     const int64_t nBp = presplit_elems(nspin, L.nspinor, nchunks, NP, nbp);
     LELPH_TRY(c->presplit.ensure(sizeof(__bf16) * (size_t)nBp));
     hipLaunchKernelGGL((k_presplit_B<NP>),
-                       dim3((unsigned)cdiv(nchunks * SP_KC, 256), (unsigned)(nspin * L.nspinor * nbp)),
+                       dim3((unsigned)nchunks, (unsigned)cdiv((int64_t)nbp * 4, 256),
+                            (unsigned)(nspin * L.nspinor)),
                        dim3(256), 0, c->stream, c->box_kq.as<c32>(), nb, nbp, L.nspinor, c->nf,
                        nchunks, c->presplit.as<__bf16>());
     HIP_TRY(hipGetLastError());
@@ -506,8 +508,12 @@ static int run_local_split_t(lelph_ctx* c, const double* qpt, const lelph_pair& 
     sa.nspin = nspin;
     sa.nspinor = L.nspinor;
     sa.nmag = nmag_eff;
-    sa.ntn = nbp / SP_TR;
-    sa.ntm = (int)cdiv(nb, SP_TC);
+    const bool fast = L.nspinor == 1 && nmag_eff != 4 && c->nf % SP_KC == 0;
+    // producer/consumer kernel: bf16x3 on the aligned path with enough bands
+    // to fill its 64 x 128 tiles
+    const bool pc = fast && NP == 2 && nb >= 64;
+    sa.ntn = pc ? (int)cdiv(nb, PC_TR) : nbp / SP_TR;
+    sa.ntm = pc ? nbp / PC_TC : (int)cdiv(nb, SP_TC);
     sa.N = c->nf;
     sa.nchunks = nchunks;
     // nspin * nsplit must be a multiple of 8 (XCD grouping); ~768+ blocks
@@ -519,10 +525,19 @@ static int run_local_split_t(lelph_ctx* c, const double* qpt, const lelph_pair& 
     sa.chunks_per_split = cdiv(nchunks, ns);
     const int64_t gsz = (int64_t)nm * nspin * nb * nb;
     LELPH_TRY(c->part.ensure(sizeof(z64) * (size_t)(ns * gsz)));
-    const size_t lds = split_lds_bytes<NP>();
+    const size_t lds = pc     ? pc_lds_bytes<NP>()
+                       : fast ? split_lds_bytes<NP, true>()
+                              : split_lds_bytes<NP, false>();
     const int64_t nblk = (int64_t)nspin * ns * per_group;
-    const bool fast = L.nspinor == 1 && nmag_eff != 4 && c->nf % SP_KC == 0;
-    if (fast)
+    if (pc)
+    {
+        HIP_TRY(hipFuncSetAttribute((const void*)k_contract_split_pc<NP>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_contract_split_pc<NP>), dim3((unsigned)nblk), dim3(SP_THREADS), lds,
+                           c->stream, c->box_k.as<c32>(), c->presplit.as<__bf16>(),
+                           c->dv32.as<c32>(), c->part.as<z64>(), sa);
+    }
+    else if (fast)
     {
         HIP_TRY(hipFuncSetAttribute((const void*)k_contract_split<NP, NPROD, true>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

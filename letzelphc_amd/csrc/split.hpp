@@ -10,23 +10,30 @@
 // FP32 summation chain spans more than SP_FLUSH * 32 grid points.
 //
 // Data flow per (k, k+q) pair:
-//   k_presplit_B : psi_k+q (FP32 box) -> bf16 pieces in an MFMA-ready,
-//                  chunk-major layout Bp[spin][spinor][chunk][piece][re/im]
-//                  [m (padded to 128)][32 points]; written once per pair.
+//   k_presplit_B : psi_k+q (FP32 box) -> bf16 pieces, chunk-major
+//                  Bp[spin][spinor][chunk][piece][re/im][kq][m][8]: for one
+//                  chunk of 32 points, kq = point / 8 and 8 consecutive points
+//                  per 16-byte unit; m padded to a multiple of 128.
 //   k_contract_split : one block = one mode nu x 128 k bands (rows) x 64 k+q
 //                  bands (columns) x one range of grid points (split-K).
 //                  8 waves, each 16 rows x 64 columns.  Per chunk of 32 points
-//                  the block stages A = dV_nu psi_k (FP32, formed while
-//                  staging) and the B pieces into LDS, double buffered, with
-//                  the next chunk prefetched into registers during the MFMAs;
-//                  each wave splits its A fragment once per chunk and reuses
-//                  it across its 4 column blocks.
+//                  the B pieces (64 columns) and the 32 dV values are staged
+//                  into LDS, double buffered; each wave forms its A fragment
+//                  dV_nu psi_k in FP32, splits it once and reuses it across
+//                  its 4 column blocks.
+//   Aligned fast path (one spinor, diagonal potential, N a multiple of 32):
+//   the B pieces and dV go global -> LDS directly (global_load_lds), and the
+//   A fragment (each wave's own 16 rows) goes global -> registers, one chunk
+//   ahead; LDS then holds only B and dV.  Otherwise A = dV psi_k is staged
+//   element by element through registers into LDS.
 //   Blocks are ordered so that all tiles of one K range run on one XCD
 //   (block b runs on XCD b % 8): the psi tiles of that range are fetched
 //   into that XCD's L2 once.
 //
 // 16x16x32 fragment layout (wave64): A: lane l holds A[l%16][8(l/16)+0..7];
 // B: lane l holds B[8(l/16)+0..7][l%16]; D: lane l, v holds D[4(l/16)+v][l%16].
+// The k-group-major B layout makes the 16 lanes of one k group read 256
+// consecutive LDS bytes: the B fragment reads are free of bank conflicts.
 #pragma once
 
 #include "common.hpp"
@@ -34,6 +41,7 @@
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float flt4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) char lds_char;
 
 #define SP_KC 32      // grid points per chunk
 #define SP_TR 128     // rows (k bands) of a block tile
@@ -69,32 +77,47 @@ static inline int64_t presplit_elems(int nspin, int nspinor, int64_t nchunks, in
     return (int64_t)nspin * nspinor * nchunks * NP * 2 * nbp * SP_KC;
 }
 
-// psi_k+q box (nspin, nb, nspinor, N) FP32 -> Bp pieces.  Grid: x over the
-// padded points (nchunks * 32), y over (spin, spinor, row m < nbp); rows
-// m >= nb and points >= N are zero.
+// psi_k+q box (nspin, nb, nspinor, N) FP32 -> Bp pieces.  Thread t of
+// chunk c (grid x) and (spin, spinor) (grid z), t over grid y and the block:
+// row m = t / 4, k group kq = t % 4; it converts points c*32 + 8 kq .. +7 of
+// row m and writes one 16-byte unit per piece plane.  Rows m >= nb and
+// points >= N are zero.
 template <int NP>
 __global__ void k_presplit_B(const c32* __restrict__ box, int nb, int nbp, int nspinor,
                              int64_t N, int64_t nchunks, __bf16* __restrict__ Bp)
 {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nchunks * SP_KC) return;
-    const int64_t c = r >> 5;
-    const int kk = (int)(r & 31);
-    const int y = blockIdx.y;
-    const int m = y % nbp;
-    const int sp = (y / nbp) % nspinor;
-    const int is = y / (nbp * nspinor);
-    c32 v = {0, 0};
-    if (m < nb && r < N) v = box[((int64_t)(is * nb + m) * nspinor + sp) * N + r];
-    __bf16 pr[NP], pi[NP];
-    split_bf16<NP>(v.x, pr);
-    split_bf16<NP>(v.y, pi);
-    const int64_t base = (((int64_t)(is * nspinor + sp) * nchunks + c) * NP) * 2;
+    const int t = blockIdx.y * blockDim.x + threadIdx.x;
+    if (t >= nbp * 4) return;
+    const int m = t >> 2, kq = t & 3;
+    const int64_t c = blockIdx.x;
+    const int ss = blockIdx.z; // spin * nspinor + spinor
+    const int is = ss / nspinor, sp = ss % nspinor;
+    const int64_t r0 = c * SP_KC + kq * 8;
+    bf16x8 pr[NP], pi[NP];
+    const c32* row = box + ((int64_t)(is * nb + m) * nspinor + sp) * N;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+    {
+        c32 v = {0, 0};
+        if (m < nb && r0 + j < N) v = row[r0 + j];
+        __bf16 a[NP], b[NP];
+        split_bf16<NP>(v.x, a);
+        split_bf16<NP>(v.y, b);
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+        {
+            pr[p][j] = a[p];
+            pi[p][j] = b[p];
+        }
+    }
+    const int64_t plane = (int64_t)nbp * SP_KC; // bf16 per plane
+    __bf16* base = Bp + ((int64_t)ss * nchunks + c) * NP * 2 * plane + (int64_t)kq * nbp * 8 +
+                   (int64_t)m * 8;
 #pragma unroll
     for (int p = 0; p < NP; ++p)
     {
-        Bp[((base + 2 * p + 0) * nbp + m) * SP_KC + kk] = pr[p];
-        Bp[((base + 2 * p + 1) * nbp + m) * SP_KC + kk] = pi[p];
+        *reinterpret_cast<bf16x8*>(base + (2 * p) * plane) = pr[p];
+        *reinterpret_cast<bf16x8*>(base + (2 * p + 1) * plane) = pi[p];
     }
 }
 
@@ -110,18 +133,22 @@ template <int NP>
 struct SplitStage
 {
     static constexpr int B16 = NP * 2 * SP_TC * SP_KC * 2 / 16 / SP_THREADS; // uint4 per thread
-    uint4 psi[4]; // 8 complex: psi_k (or, for 2x2 spinor potentials, dV psi_k)
-    c32 dvv;      // dV_nu at point tid of the chunk (tid < SP_KC), else unused
+    uint4 psi[4]; // 8 complex: dV psi_k (generic path)
+    c32 dvv;      // 1 (the product is already formed)
     uint4 b[B16 > 0 ? B16 : 1];
 };
 
-// Block stage of chunk c: psi_k rows [n0, n0+128) x 32 points (thread ->
-// row tid/4, 8 points), the 32 dV values, and the B pieces.
-// FAST (one spinor, diagonal potential, N a multiple of 32): 16-byte loads,
-// no bounds tests; rows n >= nb read row nb-1 (their results are never
-// stored) and the product dV psi is formed when the wave builds its A
-// fragment.  Otherwise the product is staged element by element and dV = 1.
-template <int NP, bool FAST>
+// bf16 offset of the B tile (piece plane 0, kq 0, column m0) of chunk c
+__device__ inline int64_t btile_base(const SplitArgs& a, int NP, int is, int sp, int m0, int64_t c)
+{
+    return (((int64_t)(is * a.nspinor + sp) * a.nchunks + c) * NP * 2) * a.nbp * SP_KC +
+           (int64_t)m0 * 8;
+}
+
+// Generic-path block stage of chunk c: A = dV psi_k for rows [n0, n0+128)
+// x 32 points (thread -> row tid/4, 8 points), element by element with
+// bounds tests, and the B pieces (LDS slot s of a plane = kq * 64 + row).
+template <int NP>
 __device__ inline void stage_load(SplitStage<NP>& st, const c32* __restrict__ psik,
                                   const __bf16* __restrict__ Bp, const c32* __restrict__ dv,
                                   const SplitArgs& a, int v, int is, int sp, int n0, int m0,
@@ -131,42 +158,27 @@ __device__ inline void stage_load(SplitStage<NP>& st, const c32* __restrict__ ps
     const int n = n0 + arow;
     const int64_t r0 = c * SP_KC + aoff;
     const int64_t N = a.N;
-    if constexpr (FAST)
-    {
-        const int nn = n < a.nb ? n : a.nb - 1;
-        const uint4* pk = reinterpret_cast<const uint4*>(psik + (int64_t)(is * a.nb + nn) * N + r0);
+    c32 vals[8];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) st.psi[k] = pk[k];
-        if (tid < SP_KC)
-        {
-            const c32* V = dv + ((int64_t)v * a.nmag + (a.nmag == 2 ? is : 0)) * N;
-            st.dvv = V[c * SP_KC + tid];
-        }
-    }
-    else
+    for (int k = 0; k < 8; ++k)
     {
-        c32 vals[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-        {
-            const int64_t r = r0 + k;
-            c32 val = {0, 0};
-            if (n < a.nb && r < N)
-                val = load_dvpsi<float>(psik, dv, v, is, n, sp, r, a.nb, a.nspinor, a.nmag, N);
-            vals[k] = val;
-        }
-        __builtin_memcpy(st.psi, vals, sizeof(vals));
-        st.dvv = {1.f, 0.f};
+        const int64_t r = r0 + k;
+        c32 val = {0, 0};
+        if (n < a.nb && r < N)
+            val = load_dvpsi<float>(psik, dv, v, is, n, sp, r, a.nb, a.nspinor, a.nmag, N);
+        vals[k] = val;
     }
-    const uint4* src = reinterpret_cast<const uint4*>(
-        Bp + ((((int64_t)(is * a.nspinor + sp) * a.nchunks + c) * NP * 2) * a.nbp + m0) * SP_KC);
-    constexpr int PLANE16 = SP_TC * SP_KC * 2 / 16; // uint4 per plane of the tile
+    __builtin_memcpy(st.psi, vals, sizeof(vals));
+    st.dvv = {1.f, 0.f};
+    const __bf16* bt = Bp + btile_base(a, NP, is, sp, m0, c);
 #pragma unroll
     for (int q = 0; q < SplitStage<NP>::B16; ++q)
     {
         const int idx = tid + q * SP_THREADS;
-        const int plane = idx / PLANE16, within = idx % PLANE16;
-        st.b[q] = src[(int64_t)plane * (a.nbp * SP_KC * 2 / 16) + within];
+        const int plane = idx >> 8, within = idx & 255;
+        const int kq = within >> 6, row = within & 63;
+        st.b[q] = *reinterpret_cast<const uint4*>(bt + (int64_t)plane * a.nbp * SP_KC +
+                                                  (int64_t)kq * a.nbp * 8 + row * 8);
     }
 }
 
@@ -184,53 +196,43 @@ __device__ inline void stage_store(const SplitStage<NP>& st, c32* sA, __bf16* sB
     for (int q = 0; q < SplitStage<NP>::B16; ++q) db[tid + q * SP_THREADS] = st.b[q];
 }
 
-typedef __attribute__((address_space(3))) char lds_char;
-
-// FAST staging of chunk c straight from global memory into LDS
+// Fast-path staging of chunk c straight from global memory into LDS
 // (global_load_lds_dwordx4: each wave instruction fills 1 KB of LDS, lane l
-// its bytes [16 l, 16 l + 16)).  A: the padded tile, 128 rows x 272 bytes,
-// as 34 slices; pad slots reload the row's first 16 bytes.  B: NP*2 planes of
-// 64 rows x 64 bytes.  D: the 32 dV values (wave 0, lanes 0..15).  The
-// caller waits (s_waitcnt) and synchronises before reading the buffer.
+// its bytes [16 l, 16 l + 16)): NP*2 planes x 4 k groups of 64 columns x
+// 16 B, and the 32 dV values (wave 0, lanes 0..15).  The caller waits
+// (s_waitcnt) and synchronises before reading the buffer.
 template <int NP>
-__device__ inline void stage_async(const c32* __restrict__ psik, const __bf16* __restrict__ Bp,
-                                   const c32* __restrict__ dv, const SplitArgs& a, int v, int is,
-                                   int n0, int m0, int64_t c, lds_char* LA, lds_char* LB,
-                                   lds_char* LD, int wave, int lane)
+__device__ inline void stage_async(const __bf16* __restrict__ Bp, const c32* __restrict__ dv,
+                                   const SplitArgs& a, int v, int is, int m0, int64_t c,
+                                   lds_char* LB, lds_char* LD, int wave, int lane)
 {
-    const int64_t N = a.N;
-    constexpr int RB = SP_AL * 8;             // bytes per padded LDS row of A
-    constexpr int ASL = SP_TR * RB / 1024;    // 1 KB slices of the A tile
-    static_assert(SP_TR * RB % 1024 == 0, "A tile must be whole 1 KB slices");
+    const __bf16* bt = Bp + btile_base(a, NP, is, 0, m0, c);
+    static_assert(NP * 2 * 4 % 8 == 0, "B planes must spread evenly over 8 waves");
 #pragma unroll
-    for (int i = wave; i < ASL; i += 8)
+    for (int k = 0; k < NP * 2 * 4 / 8; ++k)
     {
-        const int o = i * 1024 + lane * 16;
-        const int row = o / RB;
-        int col = o - row * RB;
-        if (col >= SP_KC * 8) col = 0;
-        const int n = n0 + row;
-        const int nn = n < a.nb ? n : a.nb - 1;
-        const char* src =
-            reinterpret_cast<const char*>(psik + (int64_t)(is * a.nb + nn) * N + c * SP_KC) + col;
-        __builtin_amdgcn_global_load_lds(src, LA + i * 1024, 16, 0, 0);
-    }
-    const __bf16* bsrc =
-        Bp + (((int64_t)is * a.nchunks + c) * NP * 2 * a.nbp + m0) * SP_KC;
-#pragma unroll
-    for (int i = wave; i < NP * 2 * 4; i += 8)
-    {
-        const int plane = i >> 2, part = i & 3;
-        const char* src = reinterpret_cast<const char*>(bsrc + (int64_t)plane * a.nbp * SP_KC) +
-                          part * 1024 + lane * 16;
+        const int i = wave + 8 * k;
+        const int plane = i >> 2, kq = i & 3;
+        const __bf16* src = bt + (int64_t)plane * a.nbp * SP_KC + (int64_t)kq * a.nbp * 8 + lane * 8;
         __builtin_amdgcn_global_load_lds(src, LB + i * 1024, 16, 0, 0);
     }
     if (wave == 0 && lane < SP_KC / 2)
     {
-        const c32* V = dv + ((int64_t)v * a.nmag + (a.nmag == 2 ? is : 0)) * N + c * SP_KC;
+        const c32* V = dv + ((int64_t)v * a.nmag + (a.nmag == 2 ? is : 0)) * a.N + c * SP_KC;
         __builtin_amdgcn_global_load_lds(reinterpret_cast<const char*>(V) + lane * 16, LD, 16, 0,
                                          0);
     }
+}
+
+template <int NP, bool FAST>
+__host__ __device__ constexpr int split_a_bytes()
+{
+    return FAST ? 0 : SP_TR * SP_AL * 8;
+}
+template <int NP>
+__host__ __device__ constexpr int split_b_bytes()
+{
+    return NP * 2 * SP_TC * SP_KC * 2;
 }
 
 template <int NP, int NPROD, bool FAST>
@@ -238,13 +240,15 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split(
     const c32* __restrict__ psik, const __bf16* __restrict__ Bp, const c32* __restrict__ dv,
     z64* __restrict__ part, SplitArgs a)
 {
-    constexpr int BELEMS = NP * 2 * SP_TC * SP_KC; // bf16 per B stage
+    constexpr int A_BYTES = split_a_bytes<NP, FAST>();
+    constexpr int B_BYTES = split_b_bytes<NP>();
     extern __shared__ __attribute__((aligned(16))) char sp_smem[];
+    lds_char* L = (lds_char*)sp_smem;
     c32* sA0 = reinterpret_cast<c32*>(sp_smem);
-    c32* sA1 = sA0 + SP_TR * SP_AL;
-    __bf16* sB0 = reinterpret_cast<__bf16*>(sA1 + SP_TR * SP_AL);
-    __bf16* sB1 = sB0 + BELEMS;
-    c32* sD0 = reinterpret_cast<c32*>(sB1 + BELEMS);
+    c32* sA1 = reinterpret_cast<c32*>(sp_smem + A_BYTES);
+    __bf16* sB0 = reinterpret_cast<__bf16*>(sp_smem + 2 * A_BYTES);
+    __bf16* sB1 = reinterpret_cast<__bf16*>(sp_smem + 2 * A_BYTES + B_BYTES);
+    c32* sD0 = reinterpret_cast<c32*>(sp_smem + 2 * A_BYTES + 2 * B_BYTES);
     c32* sD1 = sD0 + SP_KC;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -275,23 +279,29 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split(
             di[cb][e] = 0.0;
         }
 
+    // fast path: this lane's A row (rows >= nb read row nb-1; their results
+    // are never stored) and 8 points 8(lane/16)..+7 of each chunk
+    const int arow = min(n0 + wave * 16 + (lane & 15), a.nb - 1);
+    const uint4* arow_p = reinterpret_cast<const uint4*>(
+        psik + (int64_t)(is * a.nb + arow) * a.N + 8 * (lane >> 4));
+    uint4 areg[4], anext[4];
+
     SplitStage<NP> st;
-    lds_char* L = (lds_char*)sp_smem;
-    constexpr int A_BYTES = SP_TR * SP_AL * 8;
-    constexpr int B_BYTES = BELEMS * 2;
     int nfl = 0;
     for (int sp = 0; sp < a.nspinor; ++sp)
     {
         if (c0 >= c1) break;
         if constexpr (FAST)
         {
-            stage_async<NP>(psik, Bp, dv, a, v, is, n0, m0, c0, L, L + 2 * A_BYTES,
-                            L + 2 * A_BYTES + 2 * B_BYTES, wave, lane);
+            stage_async<NP>(Bp, dv, a, v, is, m0, c0, L + 2 * A_BYTES, L + 2 * A_BYTES + 2 * B_BYTES,
+                            wave, lane);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) areg[k] = arow_p[c0 * (SP_KC / 2) + k];
             __builtin_amdgcn_s_waitcnt(0);
         }
         else
         {
-            stage_load<NP, FAST>(st, psik, Bp, dv, a, v, is, sp, n0, m0, c0, tid);
+            stage_load<NP>(st, psik, Bp, dv, a, v, is, sp, n0, m0, c0, tid);
             stage_store<NP>(st, sA0, sB0, sD0, tid);
         }
         __syncthreads();
@@ -302,14 +312,17 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split(
             if (more)
             {
                 if constexpr (FAST)
-                    stage_async<NP>(psik, Bp, dv, a, v, is, n0, m0, c + 1,
-                                    L + (cur ? 0 : A_BYTES), L + 2 * A_BYTES + (cur ? 0 : B_BYTES),
+                {
+                    stage_async<NP>(Bp, dv, a, v, is, m0, c + 1,
+                                    L + 2 * A_BYTES + (cur ? 0 : B_BYTES),
                                     L + 2 * A_BYTES + 2 * B_BYTES + (cur ? 0 : SP_KC * 8), wave,
                                     lane);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) anext[k] = arow_p[(c + 1) * (SP_KC / 2) + k];
+                }
                 else
-                    stage_load<NP, FAST>(st, psik, Bp, dv, a, v, is, sp, n0, m0, c + 1, tid);
+                    stage_load<NP>(st, psik, Bp, dv, a, v, is, sp, n0, m0, c + 1, tid);
             }
-            const c32* sA = cur ? sA1 : sA0;
             const __bf16* sb = cur ? sB1 : sB0;
             const c32* sD = cur ? sD1 : sD0;
 
@@ -317,12 +330,20 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split(
             // A = dV_nu psi_k formed here in FP32, then split
             bf16x8 are[NP], aim[NP];
             {
-                const c32* s = sA + (wave * 16 + (lane & 15)) * SP_AL + 8 * (lane >> 4);
+                c32 av[8];
+                if constexpr (FAST)
+                    __builtin_memcpy(av, areg, sizeof(av));
+                else
+                {
+                    const c32* s = (cur ? sA1 : sA0) + (wave * 16 + (lane & 15)) * SP_AL + 8 * (lane >> 4);
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) av[k] = s[k];
+                }
                 const c32* d = sD + 8 * (lane >> 4);
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
                 {
-                    const c32 val = cmul(d[k], s[k]);
+                    const c32 val = cmul(d[k], av[k]);
                     __bf16 pr[NP], pi[NP];
                     split_bf16<NP>(val.x, pr);
                     split_bf16<NP>(val.y, pi);
@@ -334,17 +355,20 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split(
                     }
                 }
             }
-            const int ko = 8 * (lane >> 4);
+            bf16x8 nre[NP];
+#pragma unroll
+            for (int p = 0; p < NP; ++p) nre[p] = neg_bf16x8(are[p]);
+            // B fragment: LDS slot kq * 64 + column of each piece plane
+            const __bf16* bq = sb + ((lane >> 4) * SP_TC + (lane & 15)) * 8;
 #pragma unroll
             for (int cb = 0; cb < 4; ++cb)
             {
                 bf16x8 br[NP], bi[NP];
-                const int ro = cb * 16 + (lane & 15);
 #pragma unroll
                 for (int p = 0; p < NP; ++p)
                 {
-                    br[p] = *reinterpret_cast<const bf16x8*>(sb + ((2 * p) * SP_TC + ro) * SP_KC + ko);
-                    bi[p] = *reinterpret_cast<const bf16x8*>(sb + ((2 * p + 1) * SP_TC + ro) * SP_KC + ko);
+                    br[p] = *reinterpret_cast<const bf16x8*>(bq + (2 * p) * SP_TC * SP_KC + cb * 128);
+                    bi[p] = *reinterpret_cast<const bf16x8*>(bq + (2 * p + 1) * SP_TC * SP_KC + cb * 128);
                 }
 #pragma unroll
                 for (int p1 = 0; p1 < NP; ++p1)
@@ -357,7 +381,7 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split(
                         accr[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(are[p1], br[p2], accr[cb], 0, 0, 0);
                         acci[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aim[p1], br[p2], acci[cb], 0, 0, 0);
                         accr[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aim[p1], bi[p2], accr[cb], 0, 0, 0);
-                        acci[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(neg_bf16x8(are[p1]), bi[p2], acci[cb], 0, 0, 0);
+                        acci[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nre[p1], bi[p2], acci[cb], 0, 0, 0);
                     }
             }
             if (++nfl == SP_FLUSH)
@@ -378,7 +402,11 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split(
             {
                 // the other buffer was last read before the previous barrier
                 if constexpr (FAST)
+                {
                     __builtin_amdgcn_s_waitcnt(0);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) areg[k] = anext[k];
+                }
                 else
                     stage_store<NP>(st, cur ? sA0 : sA1, cur ? sB0 : sB1, cur ? sD0 : sD1, tid);
                 __syncthreads();
@@ -403,8 +431,8 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split(
         }
 }
 
-template <int NP>
+template <int NP, bool FAST>
 static inline size_t split_lds_bytes()
 {
-    return 2 * (sizeof(c32) * (SP_TR * SP_AL + SP_KC) + sizeof(__bf16) * (size_t)NP * 2 * SP_TC * SP_KC);
+    return 2 * ((size_t)split_a_bytes<NP, FAST>() + split_b_bytes<NP>() + sizeof(c32) * SP_KC);
 }

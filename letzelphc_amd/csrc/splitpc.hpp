@@ -1,0 +1,300 @@
+// splitpc.hpp -- producer/consumer form of the bf16x3 split contraction.
+//
+// Same arithmetic as k_contract_split (split.hpp): A = dV_nu psi_k split into
+// 2 bf16 pieces, B = psi_k+q pieces (presplit), 3 piece products per real
+// product on v_mfma_f32_16x16x32_bf16, FP32 chains of SP_FLUSH * 32 points
+// flushed into FP64.  The difference is the division of labour inside a
+// block, which lets the VALU work of the split overlap the MFMAs:
+//
+//   block tile: one mode x 64 k bands (rows) x 128 k+q bands (columns) x one
+//   K range; 8 waves, 4 producers and 4 consumers (wave w runs on SIMD
+//   w % 4, so every SIMD holds one of each).
+//   producer p: loads its 16 rows of psi_k and the chunk's dV straight into
+//     registers (two chunks ahead), forms dV psi_k in FP32, splits it and
+//     writes the A fragments of row block p into LDS in MFMA lane order; it
+//     also issues the global -> LDS copies (global_load_lds) of the next
+//     chunk's B pieces.
+//   consumer p: reads the A fragments of row block p and the B fragments of
+//     the 8 column blocks from LDS and runs the 96 MFMAs of the chunk.
+//   One barrier per chunk; A pieces and B are double buffered, so chunk c+1
+//   is produced while chunk c is consumed.
+//
+// LDS per buffer: A pieces [piece][re/im][row block 4][64 lanes x 16 B]
+// (8 KB per piece) and B [piece][re/im][kq 4][column 128][16 B] (16 KB per
+// piece); both are read as 1 KB contiguous per wave instruction.
+// Requirements (checked by the host): one spinor, diagonal potential, N a
+// multiple of 32, bands padded to 128 in the presplit layout.
+#pragma once
+
+#include "split.hpp"
+
+#define PC_TR 64  // rows of a block tile
+#define PC_TC 128 // columns of a block tile
+
+template <int NP>
+__host__ __device__ constexpr int pc_a_bytes()
+{
+    return NP * 2 * 4 * 1024;
+}
+template <int NP>
+__host__ __device__ constexpr int pc_b_bytes()
+{
+    return NP * 2 * 4 * 2048;
+}
+template <int NP>
+static inline size_t pc_lds_bytes()
+{
+    return 2 * (size_t)(pc_a_bytes<NP>() + pc_b_bytes<NP>());
+}
+
+// one chunk of this lane's A row and dV: 8 complex each
+struct PcRaw
+{
+    uint4 a[4];
+    uint4 d[4];
+};
+
+__device__ inline void pc_load_raw(PcRaw& r, const uint4* arow, const uint4* drow, int64_t c)
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r.a[k] = arow[c * (SP_KC / 2) + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r.d[k] = drow[c * (SP_KC / 2) + k];
+}
+
+// dV psi_k -> NP bf16 pieces (re, im) -> LDS A fragment slots of row block rb
+template <int NP>
+__device__ inline void pc_produce(const PcRaw& r, lds_char* LA, int rb, int lane)
+{
+    c32 av[8], dv[8];
+    __builtin_memcpy(av, r.a, sizeof(av));
+    __builtin_memcpy(dv, r.d, sizeof(dv));
+    bf16x8 are[NP], aim[NP];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+    {
+        const c32 val = cmul(dv[k], av[k]);
+        __bf16 pr[NP], pi[NP];
+        split_bf16<NP>(val.x, pr);
+        split_bf16<NP>(val.y, pi);
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+        {
+            are[p][k] = pr[p];
+            aim[p][k] = pi[p];
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+    {
+        *reinterpret_cast<__attribute__((address_space(3))) bf16x8*>(
+            LA + ((2 * p) * 4 + rb) * 1024 + lane * 16) = are[p];
+        *reinterpret_cast<__attribute__((address_space(3))) bf16x8*>(
+            LA + ((2 * p + 1) * 4 + rb) * 1024 + lane * 16) = aim[p];
+    }
+}
+
+// B pieces of chunk c, columns [m0, m0+128): NP*2 planes x 4 k groups x 2
+// halves of 64 columns, one 1 KB global_load_lds each, spread over the 4
+// producer waves (pw = 0..3).
+template <int NP>
+__device__ inline void pc_stage_B(const __bf16* __restrict__ Bp, const SplitArgs& a, int is,
+                                  int m0, int64_t c, lds_char* LB, int pw, int lane)
+{
+    const __bf16* bt = Bp + btile_base(a, NP, is, 0, m0, c);
+#pragma unroll
+    for (int k = 0; k < NP * 2 * 8 / 4; ++k)
+    {
+        const int i = pw + 4 * k;
+        const int plane = i >> 3, kq = (i >> 1) & 3, half = i & 1;
+        const __bf16* src = bt + (int64_t)plane * a.nbp * SP_KC + (int64_t)kq * a.nbp * 8 +
+                            (half * 64 + lane) * 8;
+        __builtin_amdgcn_global_load_lds(src, LB + i * 1024, 16, 0, 0);
+    }
+}
+
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// accesses (lgkmcnt) and synchronises, but unlike __syncthreads it does not
+// wait for outstanding global loads, so register prefetches stay in flight
+// across it.  Global -> LDS copies are waited for explicitly (vmcnt) before
+// it.  The memory clobber keeps the compiler from moving LDS accesses
+// across the barrier.
+__device__ inline void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <int NP>
+__global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
+    const c32* __restrict__ psik, const __bf16* __restrict__ Bp, const c32* __restrict__ dv,
+    z64* __restrict__ part, SplitArgs a)
+{
+    constexpr int A_BYTES = pc_a_bytes<NP>();
+    constexpr int B_BYTES = pc_b_bytes<NP>();
+    extern __shared__ __attribute__((aligned(16))) char pc_smem[];
+    lds_char* L = (lds_char*)pc_smem;
+    // buffer u: A pieces at L + u*A_BYTES, B at L + 2*A_BYTES + u*B_BYTES
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool producer = wave >= 4;
+    const int w4 = wave & 3;
+    // XCD-aware decode: group g = (spin, split) runs on XCD g % 8
+    const int per_group = a.nm * a.ntn * a.ntm;
+    const int b = blockIdx.x;
+    const int x = b & 7, j = b >> 3;
+    const int g = x + 8 * (j / per_group);
+    const int tile = j % per_group;
+    const int v = tile / (a.ntn * a.ntm);
+    const int tn = (tile / a.ntm) % a.ntn, tm = tile % a.ntm;
+    const int is = g / a.nsplit, split = g % a.nsplit;
+    if (is >= a.nspin) return;
+    const int n0 = tn * PC_TR, m0 = tm * PC_TC;
+    const int64_t c0 = (int64_t)split * a.chunks_per_split;
+    const int64_t c1 = (c0 + a.chunks_per_split < a.nchunks) ? c0 + a.chunks_per_split : a.nchunks;
+    if (c0 >= c1) return; // uniform over the block
+
+    if (producer)
+    {
+        // this lane's row (rows >= nb read row nb-1; their results are never
+        // stored) and points 8(lane/16)..+7 of every chunk
+        const int arow = min(n0 + 16 * w4 + (lane & 15), a.nb - 1);
+        const uint4* ap = reinterpret_cast<const uint4*>(
+            psik + (int64_t)(is * a.nb + arow) * a.N + 8 * (lane >> 4));
+        const uint4* dp = reinterpret_cast<const uint4*>(
+            dv + ((int64_t)v * a.nmag + (a.nmag == 2 ? is : 0)) * a.N + 8 * (lane >> 4));
+        // two register sets alternate: chunk c+1 is split from one while
+        // chunk c+2 loads into the other.  Loads are unconditional (clamped
+        // to the last chunk) so every set stays in its registers.  In each
+        // step the LDS writes precede the global -> LDS copies in program
+        // order, so they never wait on them; the wait before the barrier
+        // lets the 8 raw loads (issued last) stay in flight.
+        PcRaw r0, r1;
+        const int64_t clast = c1 - 1;
+        pc_stage_B<NP>(Bp, a, is, m0, c0, L + 2 * A_BYTES, w4, lane);
+        pc_load_raw(r0, ap, dp, c0);
+        pc_produce<NP>(r0, L, w4, lane);
+        pc_load_raw(r1, ap, dp, c0 + 1 < c1 ? c0 + 1 : clast);
+        __builtin_amdgcn_s_waitcnt(0);
+        lds_barrier();
+        int64_t c = c0;
+        for (; c + 2 < c1; c += 2)
+        {
+            // chunk c (buffer 0 in use): split c+1 into buffer 1
+            pc_produce<NP>(r1, L + A_BYTES, w4, lane);
+            pc_stage_B<NP>(Bp, a, is, m0, c + 1, L + 2 * A_BYTES + B_BYTES, w4, lane);
+            // the raw loads must be the last 8 vector-memory operations
+            // before the wait: keep the scheduler from hoisting them
+            __builtin_amdgcn_sched_barrier(0);
+            pc_load_raw(r0, ap, dp, c + 2);
+            __builtin_amdgcn_s_waitcnt(0x0078); // vmcnt(8) lgkmcnt(0)
+            lds_barrier();
+            // chunk c+1 (buffer 1 in use): split c+2 into buffer 0
+            pc_produce<NP>(r0, L, w4, lane);
+            pc_stage_B<NP>(Bp, a, is, m0, c + 2, L + 2 * A_BYTES, w4, lane);
+            __builtin_amdgcn_sched_barrier(0);
+            pc_load_raw(r1, ap, dp, c + 3 < c1 ? c + 3 : clast);
+            __builtin_amdgcn_s_waitcnt(0x0078);
+            lds_barrier();
+        }
+        if (c + 1 < c1)
+        {
+            pc_produce<NP>(r1, L + A_BYTES, w4, lane);
+            pc_stage_B<NP>(Bp, a, is, m0, c + 1, L + 2 * A_BYTES + B_BYTES, w4, lane);
+            __builtin_amdgcn_s_waitcnt(0);
+            lds_barrier(); // chunk c
+            lds_barrier(); // chunk c+1
+        }
+        else
+            lds_barrier(); // chunk c, the last
+        __builtin_amdgcn_s_waitcnt(0);
+        return;
+    }
+
+    // consumer: rows n0 + 16 w4 + .., columns m0 .. m0 + 127 (8 blocks)
+    flt4v accr[8], acci[8];
+    double dr[8][4], di[8][4];
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            accr[cb][e] = 0.f;
+            acci[cb][e] = 0.f;
+            dr[cb][e] = 0.0;
+            di[cb][e] = 0.0;
+        }
+    lds_barrier();
+    int u = 0, nfl = 0;
+    for (int64_t c = c0; c < c1; ++c)
+    {
+        const lds_char* LA = L + u * A_BYTES;
+        const lds_char* LB = L + 2 * A_BYTES + u * B_BYTES;
+        bf16x8 are[NP], aim[NP], nre[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+        {
+            are[p] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8*>(
+                LA + ((2 * p) * 4 + w4) * 1024 + lane * 16);
+            aim[p] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8*>(
+                LA + ((2 * p + 1) * 4 + w4) * 1024 + lane * 16);
+            nre[p] = neg_bf16x8(are[p]);
+        }
+        // B fragment of column block cb: plane, kq = lane/16, column cb*16 + lane%16
+        const lds_char* bq = LB + ((lane >> 4) * PC_TC + (lane & 15)) * 16;
+#pragma unroll
+        for (int cb = 0; cb < 8; ++cb)
+        {
+            bf16x8 br[NP], bi[NP];
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+            {
+                br[p] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8*>(
+                    bq + (2 * p) * 8192 + cb * 256);
+                bi[p] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8*>(
+                    bq + (2 * p + 1) * 8192 + cb * 256);
+            }
+#pragma unroll
+            for (int p1 = 0; p1 < NP; ++p1)
+#pragma unroll
+                for (int p2 = 0; p2 < NP; ++p2)
+                {
+                    if (p1 + p2 > 1) continue; // the 3 products of the 2-piece split
+                    // C += a conj(b): re = ar br + ai bi, im = ai br - ar bi
+                    accr[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(are[p1], br[p2], accr[cb], 0, 0, 0);
+                    acci[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aim[p1], br[p2], acci[cb], 0, 0, 0);
+                    accr[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aim[p1], bi[p2], accr[cb], 0, 0, 0);
+                    acci[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nre[p1], bi[p2], acci[cb], 0, 0, 0);
+                }
+        }
+        if (++nfl == SP_FLUSH)
+        {
+            nfl = 0;
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                {
+                    dr[cb][e] += (double)accr[cb][e];
+                    di[cb][e] += (double)acci[cb][e];
+                    accr[cb][e] = 0.f;
+                    acci[cb][e] = 0.f;
+                }
+        }
+        lds_barrier();
+        u ^= 1;
+    }
+    const int row = n0 + 16 * w4;
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const int rr = row + 4 * (lane >> 4) + e;
+            const int cc = m0 + cb * 16 + (lane & 15);
+            if (rr < a.nb && cc < a.nb)
+            {
+                const int64_t o = ((((int64_t)split * a.nm + v) * a.nspin + is) * a.nb + rr) * a.nb + cc;
+                part[o] = {dr[cb][e] + (double)accr[cb][e], di[cb][e] + (double)acci[cb][e]};
+            }
+        }
+}

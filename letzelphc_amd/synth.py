@@ -384,10 +384,20 @@ def config(cfg: str, nbnds=None, kgrid=None, with_wfc=True, **kw) -> System:
 
 
 def tiny(nspin=1, nspinor=1, nmag=1, nbnds=4, fft=(12, 12, 12), kgrid=(2, 2, 2),
-         syms="trev", ecut=6.0, seed=7, q=(0.0, 0.0, 0.5), with_wfc=True) -> System:
-    """A small fcc cell for branch coverage (spin, spinors, symmetries)."""
+         syms="trev", ecut=6.0, seed=7, q=(0.0, 0.0, 0.5), with_wfc=True,
+         dimension="3") -> System:
+    """A small fcc cell for branch coverage (spin, spinors, symmetries);
+    dimension "2": a hexagonal slab cell (a = 5, c = 20 bohr, 12x12x48)
+    with the 2-D Coulomb cut-off."""
     a = 8.0
     I3 = np.eye(3)
+    if dimension == "2":
+        sl = [(I3, np.zeros(3), 0), (-I3, np.zeros(3), 1)]
+        return make_system("tiny2d", seed, hex_cell(5.0, 20.0),
+                           [[1 / 3, 2 / 3, 0.5], [2 / 3, 1 / 3, 0.56]], [0, 1], [3.0, 5.0],
+                           (12, 12, 48), (kgrid[0], kgrid[1], 1), ecut, nbnds, sl, 1,
+                           [(0.5, 0.0, 0.0)], nspin=nspin, nspinor=nspinor, nmag=nmag,
+                           dimension="2", with_wfc=with_wfc)
     if syms == "id":
         sl, tr = [(I3, np.zeros(3), 0)], 0
     elif syms == "trev":

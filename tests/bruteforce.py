@@ -134,7 +134,7 @@ def vnl_expect(s, ikq, ik, pos):
     return out
 
 
-def vloc_total_G(s, t, G):
+def vloc_total_G(s, t, G, cutoff=None):
     """Closed form of V_loc(|G|) for the synthetic local potential
     V(r) = -e2 Z erf(r)/r + A exp(-r^2/b):
     (1/Omega) [-4pi e2 Z exp(-G^2/4)/G^2 + A (pi b)^(3/2) exp(-b G^2/4)]."""
@@ -143,22 +143,38 @@ def vloc_total_G(s, t, G):
     G = np.asarray(G, np.float64)
     with np.errstate(divide="ignore", invalid="ignore"):
         lr = np.where(G > 1e-12, -4 * np.pi * synth.E2 * Z * np.exp(-G * G / 4) / (G * G), 0.0)
+    if cutoff is not None:
+        lr = lr * cutoff
     sr = ps.loc_A * (np.pi * ps.loc_b) ** 1.5 * np.exp(-ps.loc_b * G * G / 4)
     return (lr + sr) / s.volume
+
+
+def slab_cutoff(qg, c):
+    """Coulomb cut-off factor of a slab of height c (|z| < c/2), Ismail-Beigi,
+    Phys. Rev. B 73, 233103 (2006):
+    1 + exp(-G_p z_c) ((G_z/G_p) sin(G_z z_c) - cos(G_z z_c)), z_c = c/2,
+    for in-plane |G_p| > 0."""
+    gp = np.linalg.norm(qg[..., :2], axis=-1)
+    gz = qg[..., 2]
+    zc = 0.5 * c
+    assert np.all(gp > 1e-8), "G_p = 0 not covered by this closed form"
+    return 1 + np.exp(-gp * zc) * ((gz / gp) * np.sin(gz * zc) - np.cos(gz * zc))
 
 
 def dvlocq_closed(s, q, eig):
     """(nm, Nx, Ny, Nz) bare local dV_q in the mode basis from the closed-form
     V_loc: coefficients sum_{a,x} eig (-i)(q+G)_x exp(-i (q+G).tau_a) V(|q+G|)
-    on the box, unnormalised backward transform."""
+    on the box, unnormalised backward transform.  For 2-D systems the
+    long-range (erf) part carries the slab cut-off."""
     dims = s.fft_dims
     mi = [synth.miller_axis(n) for n in dims]
     gx, gy, gz = np.meshgrid(mi[0] + q[0], mi[1] + q[1], mi[2] + q[2], indexing="ij")
     qg = np.stack([gx, gy, gz], -1) @ s.blat.T
     qn = np.linalg.norm(qg, axis=-1)
     coef = np.zeros((s.nmodes,) + tuple(dims), np.complex128)
+    cf = slab_cutoff(qg, s.alat[2, 2]) if s.dimension == "2" else None
     for ia in range(s.natom):
-        V = vloc_total_G(s, int(s.atom_type[ia]), qn)
+        V = vloc_total_G(s, int(s.atom_type[ia]), qn, cf)
         ph = np.exp(-1j * (qg @ s.atomic_pos[ia]))
         for x in range(3):
             coef += eig[:, 3 * ia + x][:, None, None, None] * ((-1j) * qg[..., x] * ph * V)[None]

@@ -115,7 +115,7 @@ def test_compute_elphq_reduced(c0, c0ctx, prec, tol):
 
 
 @pytest.mark.parametrize("kind", ["spin2", "spinor_nmag1", "spinor_nmag4", "cubic_syms",
-                                  "bare", "dfpt_local"])
+                                  "bare", "dfpt_local", "slab_2d"])
 def test_tiny_variants(kind):
     kw = dict(nbnds=4)
     kernel = "dfpt"
@@ -129,6 +129,8 @@ def test_tiny_variants(kind):
         kw.update(syms="cubic", q=(0.0, 0.5, 0.0))
     elif kind in ("bare", "dfpt_local"):
         kernel = kind
+    elif kind == "slab_2d":
+        kw.update(dimension="2")
     s = synth.tiny(**kw)
     o = O.OracleSystem(s)
     say(f"case: tiny {kind}")
@@ -179,3 +181,36 @@ def test_error_paths():
         ctx.compute_elphq(np.array([0.1234, 0.0, 0.0]))
     assert e.value.status == 7
     ctx.close()
+
+
+def test_c1_symmetry_pairs():
+    """The layered c1 cell (45x45x216, 12 operations incl. time reversal),
+    16 bands: pairs whose k and k+q need non-trivial and time-reversed
+    operations, FP64 to 1e-10 and the reduced variants (generic, unaligned
+    grid path) to 1e-6 / 5e-5."""
+    s = synth.config("c1", nbnds=16, kgrid=(6, 6, 1))
+    o = O.OracleSystem(s)
+    O.set_inner_threads(8)
+    pairs = synth.kq_pairs(s, 0)
+    pick, seen = [], set()
+    for pr in pairs:
+        key = (pr[2] > 0, pr[3] > 0, pr[2] >= 6 or pr[3] >= 6)
+        if key not in seen:
+            seen.add(key)
+            pick.append(pr)
+    say(f"case: c1 pairs {pick}")
+    dv = o.dv_total(0)
+    ctx = lelph.Context(s, "fp64")
+    ctx.set_dvscf(s.qpts[0], s.eigvec(0), s.dvscf(0), kernel="dfpt")
+    try:
+        for pr in pick:
+            ref = o.add_nonlocal(0, *pr, o.elph_local(0, *pr, dv))
+            for prec, tol in (("fp64", TOL64), ("fp32", TOL32), ("bf16x3", 5e-5)):
+                ctx.set_precision(prec)
+                g = ctx.gkq(*pr)
+                err = rel_err(g, ref)
+                say(f"  {pr} {prec}: {err:.3e}")
+                assert err < tol, (pr, prec)
+    finally:
+        O.set_inner_threads(1)
+        ctx.close()

@@ -185,6 +185,18 @@ def test_dvlocq_closed_form():
         assert rel_err(got, ref) < 2e-6, q
 
 
+def test_dvlocq_closed_form_slab():
+    """2-D systems: the long-range part carries the slab Coulomb cut-off."""
+    s = synth.tiny(nbnds=2, dimension="2")
+    o = O.OracleSystem(s)
+    say("case: oracle dVlocq vs closed form, 2-D cut-off")
+    got = o.dvlocq(0)
+    ref = BF.dvlocq_closed(s, s.qpts[0], s.eigvec(0))
+    assert rel_err(got, ref) < 2e-6
+    s3 = dataclasses.replace(s, dimension="3")
+    assert rel_err(O.OracleSystem(s3).dvlocq(0), ref) > 1e-4  # the cut-off is resolved
+
+
 def test_dvlocq_translation_covariance():
     """Atoms moved by a grid vector t: dV'_q(x) = exp(-i q.t) dV_q(x - t)."""
     s = synth.tiny(nbnds=2)
