@@ -16,8 +16,10 @@
 //     chunk's B pieces.
 //   consumer p: reads the A fragments of row block p and the B fragments of
 //     the 8 column blocks from LDS and runs the 96 MFMAs of the chunk.
-//   One barrier per chunk; A pieces and B are double buffered, so chunk c+1
-//   is produced while chunk c is consumed.
+//   One barrier per chunk; A pieces are double buffered (chunk c+1 is
+//   produced while chunk c is consumed) and B sits in a ring of three, its
+//   copies started two chunks ahead, so global latency hides behind two
+//   chunks of MFMA work.
 //
 // LDS per buffer: A pieces [piece][re/im][row block 4][64 lanes x 16 B]
 // (8 KB per piece) and B [piece][re/im][kq 4][column 128][16 B] (16 KB per
@@ -41,10 +43,12 @@ __host__ __device__ constexpr int pc_b_bytes()
 {
     return NP * 2 * 4 * 2048;
 }
+// A pieces double buffered, B pieces in a ring of 3 (copies run two chunks
+// ahead of their use)
 template <int NP>
 static inline size_t pc_lds_bytes()
 {
-    return 2 * (size_t)(pc_a_bytes<NP>() + pc_b_bytes<NP>());
+    return 2 * (size_t)pc_a_bytes<NP>() + 3 * (size_t)pc_b_bytes<NP>();
 }
 
 // one chunk of this lane's A row and dV: 8 complex each
@@ -95,17 +99,24 @@ __device__ inline void pc_produce(const PcRaw& r, lds_char* LA, int rb, int lane
 }
 
 // B pieces of chunk c, columns [m0, m0+128): NP*2 planes x 4 k groups x 2
-// halves of 64 columns, one 1 KB global_load_lds each, spread over the 4
-// producer waves (pw = 0..3).
+// halves of 64 columns, one 1 KB global_load_lds each, spread over the 2
+// copy waves (pw = 0, 1): PC_BCOPY instructions per wave.
+#define PC_BCOPY(NP) ((NP) * 2 * 8 / 2)
+
+// s_waitcnt immediate (gfx9 encoding): vmcnt(vm), expcnt not waited, lgkmcnt(0)
+__host__ __device__ constexpr int waitcnt_vm(int vm)
+{
+    return (vm & 15) | ((vm >> 4) << 14) | (7 << 4);
+}
 template <int NP>
 __device__ inline void pc_stage_B(const __bf16* __restrict__ Bp, const SplitArgs& a, int is,
                                   int m0, int64_t c, lds_char* LB, int pw, int lane)
 {
     const __bf16* bt = Bp + btile_base(a, NP, is, 0, m0, c);
 #pragma unroll
-    for (int k = 0; k < NP * 2 * 8 / 4; ++k)
+    for (int k = 0; k < PC_BCOPY(NP); ++k)
     {
-        const int i = pw + 4 * k;
+        const int i = pw + 2 * k;
         const int plane = i >> 3, kq = (i >> 1) & 3, half = i & 1;
         const __bf16* src = bt + (int64_t)plane * a.nbp * SP_KC + (int64_t)kq * a.nbp * 8 +
                             (half * 64 + lane) * 8;
@@ -124,6 +135,53 @@ __device__ inline void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// one chunk of this lane's two A rows and dV
+struct PcRaw2
+{
+    uint4 a[2][4];
+    uint4 d[4];
+};
+
+__device__ inline void pc_load_raw2(PcRaw2& r, const uint4* const* ap, const uint4* dp, int i)
+{
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r.a[h][k] = ap[h][i * (SP_KC / 2) + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r.d[k] = dp[i * (SP_KC / 2) + k];
+}
+
+template <int NP>
+__device__ inline void pc_produce2(const PcRaw2& r, lds_char* LA, int pw, int lane)
+{
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+    {
+        PcRaw one;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            one.a[k] = r.a[h][k];
+            one.d[k] = r.d[k];
+        }
+        pc_produce<NP>(one, LA, 2 * pw + h, lane);
+    }
+}
+
+// one split-wave step i: split raw(i+1) into A buffer (i+1)%2, load raw(i+3),
+// barrier (its lgkmcnt wait covers the LDS writes)
+template <int NP>
+__device__ __forceinline__ void pc_split_step(int i, int n, const PcRaw2& rp, PcRaw2& rl,
+                                              const uint4* const* ap, const uint4* dp,
+                                              lds_char* L, int pw, int lane)
+{
+    constexpr int A_BYTES = pc_a_bytes<NP>();
+    if (i < n && i + 1 < n) pc_produce2<NP>(rp, L + ((i + 1) & 1) * A_BYTES, pw, lane);
+    pc_load_raw2(rl, ap, dp, i + 3 < n ? i + 3 : n - 1);
+    if (i < n) lds_barrier();
+}
+
 template <int NP>
 __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
     const c32* __restrict__ psik, const __bf16* __restrict__ Bp, const c32* __restrict__ dv,
@@ -133,7 +191,7 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
     constexpr int B_BYTES = pc_b_bytes<NP>();
     extern __shared__ __attribute__((aligned(16))) char pc_smem[];
     lds_char* L = (lds_char*)pc_smem;
-    // buffer u: A pieces at L + u*A_BYTES, B at L + 2*A_BYTES + u*B_BYTES
+    // A buffer u at L + u*A_BYTES (u = 0, 1), B buffer t at L + 2*A_BYTES + t*B_BYTES (t = 0..2)
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool producer = wave >= 4;
@@ -153,59 +211,66 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
     const int64_t c1 = (c0 + a.chunks_per_split < a.nchunks) ? c0 + a.chunks_per_split : a.nchunks;
     if (c0 >= c1) return; // uniform over the block
 
+    const int n = (int)(c1 - c0); // chunks of this block
+    if (wave >= 6)
+    {
+        // copy waves: B(i+2) is started in step i, B(i+1) must have landed
+        // before the barrier that ends step i; a wave waits only for its own
+        // copies, PC_BCOPY per chunk
+        const int pw = wave - 6;
+        lds_char* LB = L + 2 * A_BYTES;
+        pc_stage_B<NP>(Bp, a, is, m0, c0, LB, pw, lane);
+        if (n > 1) pc_stage_B<NP>(Bp, a, is, m0, c0 + 1, LB + B_BYTES, pw, lane);
+        if (n > 1)
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(PC_BCOPY(NP))); // B(0) landed
+        else
+            __builtin_amdgcn_s_waitcnt(0);
+        lds_barrier();
+        int t3 = 2;
+        for (int i = 0; i < n; ++i)
+        {
+            if (i + 2 < n)
+            {
+                pc_stage_B<NP>(Bp, a, is, m0, c0 + i + 2, LB + t3 * B_BYTES, pw, lane);
+                __builtin_amdgcn_s_waitcnt(waitcnt_vm(PC_BCOPY(NP))); // B(i+1) landed
+            }
+            else
+                __builtin_amdgcn_s_waitcnt(0);
+            lds_barrier();
+            t3 = (t3 == 2) ? 0 : t3 + 1;
+        }
+        return;
+    }
     if (producer)
     {
-        // this lane's row (rows >= nb read row nb-1; their results are never
-        // stored) and points 8(lane/16)..+7 of every chunk
-        const int arow = min(n0 + 16 * w4 + (lane & 15), a.nb - 1);
-        const uint4* ap = reinterpret_cast<const uint4*>(
-            psik + (int64_t)(is * a.nb + arow) * a.N + 8 * (lane >> 4));
+        // split waves: rows 32 pw + lane%16 (+16) of the tile, points
+        // 8(lane/16)..+7 of every chunk; raw chunk i+3 loads while i+1 is
+        // split (three register sets, loads clamped to the last chunk so
+        // every set keeps fixed registers)
+        const int pw = wave - 4;
+        const uint4* ap[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+        {
+            const int arow = min(n0 + 32 * pw + 16 * h + (lane & 15), a.nb - 1);
+            ap[h] = reinterpret_cast<const uint4*>(psik + (int64_t)(is * a.nb + arow) * a.N +
+                                                   8 * (lane >> 4)) + c0 * (SP_KC / 2);
+        }
         const uint4* dp = reinterpret_cast<const uint4*>(
-            dv + ((int64_t)v * a.nmag + (a.nmag == 2 ? is : 0)) * a.N + 8 * (lane >> 4));
-        // two register sets alternate: chunk c+1 is split from one while
-        // chunk c+2 loads into the other.  Loads are unconditional (clamped
-        // to the last chunk) so every set stays in its registers.  In each
-        // step the LDS writes precede the global -> LDS copies in program
-        // order, so they never wait on them; the wait before the barrier
-        // lets the 8 raw loads (issued last) stay in flight.
-        PcRaw r0, r1;
-        const int64_t clast = c1 - 1;
-        pc_stage_B<NP>(Bp, a, is, m0, c0, L + 2 * A_BYTES, w4, lane);
-        pc_load_raw(r0, ap, dp, c0);
-        pc_produce<NP>(r0, L, w4, lane);
-        pc_load_raw(r1, ap, dp, c0 + 1 < c1 ? c0 + 1 : clast);
-        __builtin_amdgcn_s_waitcnt(0);
+            dv + ((int64_t)v * a.nmag + (a.nmag == 2 ? is : 0)) * a.N + 8 * (lane >> 4)) +
+            c0 * (SP_KC / 2);
+        PcRaw2 R0, R1, R2;
+        pc_load_raw2(R0, ap, dp, 0);
+        pc_produce2<NP>(R0, L, pw, lane);
+        pc_load_raw2(R1, ap, dp, n > 1 ? 1 : n - 1);
+        pc_load_raw2(R2, ap, dp, n > 2 ? 2 : n - 1);
         lds_barrier();
-        int64_t c = c0;
-        for (; c + 2 < c1; c += 2)
+        for (int i = 0; i < n; i += 3)
         {
-            // chunk c (buffer 0 in use): split c+1 into buffer 1
-            pc_produce<NP>(r1, L + A_BYTES, w4, lane);
-            pc_stage_B<NP>(Bp, a, is, m0, c + 1, L + 2 * A_BYTES + B_BYTES, w4, lane);
-            // the raw loads must be the last 8 vector-memory operations
-            // before the wait: keep the scheduler from hoisting them
-            __builtin_amdgcn_sched_barrier(0);
-            pc_load_raw(r0, ap, dp, c + 2);
-            __builtin_amdgcn_s_waitcnt(0x0078); // vmcnt(8) lgkmcnt(0)
-            lds_barrier();
-            // chunk c+1 (buffer 1 in use): split c+2 into buffer 0
-            pc_produce<NP>(r0, L, w4, lane);
-            pc_stage_B<NP>(Bp, a, is, m0, c + 2, L + 2 * A_BYTES, w4, lane);
-            __builtin_amdgcn_sched_barrier(0);
-            pc_load_raw(r1, ap, dp, c + 3 < c1 ? c + 3 : clast);
-            __builtin_amdgcn_s_waitcnt(0x0078);
-            lds_barrier();
+            pc_split_step<NP>(i, n, R1, R0, ap, dp, L, pw, lane);
+            pc_split_step<NP>(i + 1, n, R2, R1, ap, dp, L, pw, lane);
+            pc_split_step<NP>(i + 2, n, R0, R2, ap, dp, L, pw, lane);
         }
-        if (c + 1 < c1)
-        {
-            pc_produce<NP>(r1, L + A_BYTES, w4, lane);
-            pc_stage_B<NP>(Bp, a, is, m0, c + 1, L + 2 * A_BYTES + B_BYTES, w4, lane);
-            __builtin_amdgcn_s_waitcnt(0);
-            lds_barrier(); // chunk c
-            lds_barrier(); // chunk c+1
-        }
-        else
-            lds_barrier(); // chunk c, the last
         __builtin_amdgcn_s_waitcnt(0);
         return;
     }
@@ -224,11 +289,11 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
             di[cb][e] = 0.0;
         }
     lds_barrier();
-    int u = 0, nfl = 0;
-    for (int64_t c = c0; c < c1; ++c)
+    int u = 0, u3 = 0, nfl = 0;
+    for (int i = 0; i < n; ++i)
     {
         const lds_char* LA = L + u * A_BYTES;
-        const lds_char* LB = L + 2 * A_BYTES + u * B_BYTES;
+        const lds_char* LB = L + 2 * A_BYTES + u3 * B_BYTES;
         bf16x8 are[NP], aim[NP], nre[NP];
 #pragma unroll
         for (int p = 0; p < NP; ++p)
@@ -282,6 +347,7 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
         }
         lds_barrier();
         u ^= 1;
+        u3 = (u3 == 2) ? 0 : u3 + 1;
     }
     const int row = n0 + 16 * w4;
 #pragma unroll
