@@ -1,0 +1,223 @@
+// boxes.hpp -- pruned sphere -> real-space transform of the FP32 psi boxes.
+//
+// Same result as scatter + three full FFT passes (the inverse transform of
+// src/fft/bwdfft.c:13-120), with the work restricted to where the sphere
+// lives, as the reference does with its z sticks (bwdfft.c:63-110):
+//   1. the rotated sphere's G vectors are grouped by x plane (counting sort
+//      on the box index; integer atomics, the order inside a plane has no
+//      effect on the result);
+//   2. k_plane_z: one block per (occupied x plane, box): the plane's
+//      coefficients (SU(2) rotation, translation phase, time-reversal
+//      conjugate as k_scatter) are scattered into an LDS plane, transformed
+//      along z there and written out (the box needs no zeroing);
+//   3. the y pass runs only on occupied planes;
+//   4. the x pass treats the unoccupied planes as zeros without reading them.
+// HBM traffic per band drops from ~7 box volumes to ~2.
+#pragma once
+
+#include "fft.hpp"
+#include "wfc.hpp"
+
+// per-plane counts: a block histogram in LDS, then one global atomic per
+// touched plane (nx <= 1024)
+__global__ __launch_bounds__(256) void k_plane_count(const int* __restrict__ idx, int64_t npw,
+                                                     int nyz, int nx, int* __restrict__ cnt)
+{
+    __shared__ int h[1024];
+    for (int t = threadIdx.x; t < nx; t += 256) h[t] = 0;
+    __syncthreads();
+    const int64_t g0 = (int64_t)blockIdx.x * 4096;
+    for (int64_t g = g0 + threadIdx.x; g < npw && g < g0 + 4096; g += 256)
+        atomicAdd(&h[idx[g] / nyz], 1);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nx; t += 256)
+        if (h[t]) atomicAdd(&cnt[t], h[t]);
+}
+
+// exclusive scan of cnt (nx <= 1024) into off and cur (one block)
+__global__ void k_plane_scan(const int* __restrict__ cnt, int nx, int* __restrict__ off,
+                             int* __restrict__ cur)
+{
+    __shared__ int s[1024];
+    const int t = threadIdx.x;
+    s[t] = t < nx ? cnt[t] : 0;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1)
+    {
+        const int v = t >= d ? s[t - d] : 0;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    if (t < nx)
+    {
+        const int e = s[t] - cnt[t];
+        off[t] = e;
+        cur[t] = e;
+    }
+}
+
+// G order grouped by plane: each block reserves a range per plane with one
+// global atomic, then places its G vectors with LDS cursors
+__global__ __launch_bounds__(256) void k_plane_fill(const int* __restrict__ idx, int64_t npw,
+                                                    int nyz, int nx, int* __restrict__ cur,
+                                                    int* __restrict__ perm)
+{
+    __shared__ int h[1024];
+    for (int t = threadIdx.x; t < nx; t += 256) h[t] = 0;
+    __syncthreads();
+    const int64_t g0 = (int64_t)blockIdx.x * 4096;
+    for (int64_t g = g0 + threadIdx.x; g < npw && g < g0 + 4096; g += 256)
+        atomicAdd(&h[idx[g] / nyz], 1);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nx; t += 256)
+        if (h[t]) h[t] = atomicAdd(&cur[t], h[t]);
+    __syncthreads();
+    for (int64_t g = g0 + threadIdx.x; g < npw && g < g0 + 4096; g += 256)
+        perm[atomicAdd(&h[idx[g] / nyz], 1)] = (int)g;
+}
+
+// Stockham stages on tl contiguous lines of length N in LDS (runtime line
+// count; same butterflies as FftStagesC).
+template <typename T, int SIGN, int N, int S, int NT>
+struct FftStagesP
+{
+    static __device__ inline void run(cpx<T>*& in, cpx<T>*& out, const cpx<T>* W, int tid,
+                                      int tl)
+    {
+        constexpr int R = fft_radix_c(N, S);
+        if constexpr (R != 0)
+        {
+            constexpr int Ns = fft_ns_c(N, S);
+            constexpr int NBF = N / R;
+            constexpr int TWS = N / (Ns * R);
+            for (int w = tid; w < tl * NBF; w += NT)
+            {
+                const int t = w / NBF, j = w - t * NBF;
+                const int k = j % Ns;
+                cpx<T> v[R];
+                const cpx<T>* src = in + t * N + j;
+                v[0] = src[0];
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[r] = cmul(src[r * NBF], tw<T, SIGN>(W, r * k * TWS));
+                if constexpr (R <= 5)
+                    dft_small<T, SIGN>(v, R, W, N);
+                else
+                {
+                    cpx<T> o[R];
+#pragma unroll
+                    for (int s = 0; s < R; ++s)
+                    {
+                        cpx<T> acc = {0, 0};
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            acc = cadd(acc, cmul(v[r], tw<T, SIGN>(W, ((r * s) % R) * (N / R))));
+                        o[s] = acc;
+                    }
+#pragma unroll
+                    for (int s = 0; s < R; ++s) v[s] = o[s];
+                }
+                cpx<T>* dst = out + t * N + (j / Ns) * Ns * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
+            }
+            __syncthreads();
+            cpx<T>* tmp = in;
+            in = out;
+            out = tmp;
+            FftStagesP<T, SIGN, N, S + 1, NT>::run(in, out, W, tid, tl);
+        }
+    }
+};
+
+// LDS bytes of k_plane_z for a (ny, nz) plane
+static inline size_t plane_lds_bytes(int ny, int nz)
+{
+    return sizeof(c32) * ((size_t)2 * ny * nz + nz);
+}
+
+// One block per (x plane, box b = set * nspinor + sigma): gather the plane's
+// coefficients into LDS, backward FFT along z (e^{+i}, unnormalised), write
+// the plane.  Empty planes are skipped (never written).
+#define PZ_THREADS 512
+template <int NZ>
+__global__ __launch_bounds__(PZ_THREADS) void k_plane_z(
+    const z64* __restrict__ wfc, int nspinor, int64_t npw, const int* __restrict__ idx,
+    const z64* __restrict__ ph, SideGeom geo, const int* __restrict__ perm,
+    const int* __restrict__ off, const int* __restrict__ cnt, int ny, int64_t nf,
+    const c32* __restrict__ Wg, c32* __restrict__ box)
+{
+    extern __shared__ __attribute__((aligned(16))) char pz_smem[];
+    const int x = blockIdx.x;
+    const int n = cnt[x];
+    if (n == 0) return;
+    const int b = blockIdx.y;
+    const int64_t s = b / nspinor;
+    const int sig = b - (int)s * nspinor;
+    const int nyz = ny * NZ;
+    c32* p0 = reinterpret_cast<c32*>(pz_smem);
+    c32* p1 = p0 + nyz;
+    c32* W = p1 + nyz;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < NZ; t += PZ_THREADS) W[t] = Wg[t];
+    for (int t = tid; t < nyz; t += PZ_THREADS) p0[t] = c32{0.f, 0.f};
+    __syncthreads();
+    const z64* w = wfc + s * nspinor * npw;
+    const int o = off[x];
+    for (int k = tid; k < n; k += PZ_THREADS)
+    {
+        const int g = perm[o + k];
+        const z64 p = ph[g];
+        z64 c;
+        if (nspinor == 1)
+            c = cmul(w[g], p);
+        else
+        {
+            const z64 u = w[g], d = w[npw + g];
+            c = sig == 0 ? cadd(cmul(u, geo.su2[0]), cmul(d, geo.su2[1]))
+                         : cadd(cmul(u, geo.su2[2]), cmul(d, geo.su2[3]));
+            c = cmul(c, p);
+        }
+        if (geo.conj) c.y = -c.y;
+        p0[idx[g] - x * nyz] = c32{(float)c.x, (float)c.y};
+    }
+    __syncthreads();
+    c32* in = p0;
+    c32* out = p1;
+    FftStagesP<float, 1, NZ, 0, PZ_THREADS>::run(in, out, W, tid, ny);
+    c32* dst = box + (int64_t)b * nf + (int64_t)x * nyz;
+    for (int t = tid; t < nyz; t += PZ_THREADS) dst[t] = in[t];
+}
+
+// launch k_plane_z for a compile-time NZ; false if NZ is not compiled
+static bool plane_z_launch(int nz, int nx, int nbox, size_t lds, const z64* wfc, int nspinor,
+                           int64_t npw, const int* idx, const z64* ph, const SideGeom& geo,
+                           const int* perm, const int* off, const int* cnt, int ny, int64_t nf,
+                           const c32* W, c32* box, hipStream_t st)
+{
+#define PZ_CASE(NZV)                                                                              \
+    case NZV:                                                                                     \
+        if (hipFuncSetAttribute((const void*)k_plane_z<NZV>,                                      \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) \
+            return false;                                                                         \
+        hipLaunchKernelGGL(k_plane_z<NZV>, dim3((unsigned)nx, (unsigned)nbox), dim3(PZ_THREADS), lds, st, \
+                           wfc, nspinor, npw, idx, ph, geo, perm, off, cnt, ny, nf, W, box);      \
+        return true;
+    switch (nz)
+    {
+        PZ_CASE(12)
+        PZ_CASE(24)
+        PZ_CASE(32)
+        PZ_CASE(36)
+        PZ_CASE(45)
+        PZ_CASE(48)
+        PZ_CASE(60)
+        PZ_CASE(64)
+        PZ_CASE(72)
+        PZ_CASE(96)
+        PZ_CASE(108)
+        PZ_CASE(216)
+        default: return false;
+    }
+#undef PZ_CASE
+}

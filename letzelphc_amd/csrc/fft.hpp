@@ -365,11 +365,16 @@ struct FftStagesC
 
 // One axis pass, compile-time N and TL (see k_fft_lines for the addressing).
 // Lines i >= n_inner of the last tile are computed on zeros and not stored.
+// Pruning (both optional): blocks whose outer index o has ocnt[o % omod] == 0
+// do nothing (their lines are all zero and are never read); elements j with
+// emask[j] == 0 are taken as zero without being read.
 template <typename T, int SIGN, int N, int TL>
 __global__ __launch_bounds__(256) void k_fft_lines_c(cpx<T>* __restrict__ data, int64_t n_inner,
                                                      int64_t ostride, int64_t lstride,
                                                      int64_t estride,
-                                                     const cpx<T>* __restrict__ Wg)
+                                                     const cpx<T>* __restrict__ Wg,
+                                                     const int* __restrict__ ocnt, int omod,
+                                                     const int* __restrict__ emask)
 {
     __shared__ cpx<T> W[N];
     __shared__ cpx<T> b0[TL * N];
@@ -377,6 +382,7 @@ __global__ __launch_bounds__(256) void k_fft_lines_c(cpx<T>* __restrict__ data, 
     const int tid = threadIdx.x;
     const int64_t tiles = (n_inner + TL - 1) / TL;
     const int64_t o = blockIdx.x / tiles;
+    if (ocnt && ocnt[o % omod] == 0) return;
     const int64_t i0 = (blockIdx.x - o * tiles) * TL;
     const int nl = (int)((n_inner - i0) < TL ? (n_inner - i0) : TL);
     cpx<T>* base = data + o * ostride + i0 * lstride;
@@ -387,7 +393,8 @@ __global__ __launch_bounds__(256) void k_fft_lines_c(cpx<T>* __restrict__ data, 
         for (int f = tid; f < TL * N; f += 256)
         {
             const int j = f / TL, t = f - j * TL;
-            b0[t * N + j] = (t < nl) ? base[(int64_t)j * estride + t] : cpx<T>{0, 0};
+            b0[t * N + j] = (t < nl && (!emask || emask[j])) ? base[(int64_t)j * estride + t]
+                                                             : cpx<T>{0, 0};
         }
     }
     else
@@ -395,8 +402,9 @@ __global__ __launch_bounds__(256) void k_fft_lines_c(cpx<T>* __restrict__ data, 
         for (int f = tid; f < TL * N; f += 256)
         {
             const int t = f / N, j = f - t * N;
-            b0[t * N + j] = (t < nl) ? base[(int64_t)t * lstride + (int64_t)j * estride]
-                                     : cpx<T>{0, 0};
+            b0[t * N + j] = (t < nl && (!emask || emask[j]))
+                                ? base[(int64_t)t * lstride + (int64_t)j * estride]
+                                : cpx<T>{0, 0};
         }
     }
     __syncthreads();
@@ -433,18 +441,31 @@ constexpr int fft_tl_c()
 template <typename T, int N>
 static bool fft_launch_c(cpx<T>* data, int64_t n_outer, int64_t n_inner, int64_t ostride,
                          int64_t lstride, int64_t estride, int sign, const cpx<T>* W,
-                         hipStream_t stream)
+                         hipStream_t stream, const int* ocnt, int omod, const int* emask)
 {
     constexpr int TL = fft_tl_c<T, N>();
     const int64_t nblk = cdiv(n_inner, TL) * n_outer;
     if (nblk > 0x7fffffffLL) return false;
     if (sign > 0)
         hipLaunchKernelGGL((k_fft_lines_c<T, 1, N, TL>), dim3((unsigned)nblk), dim3(256), 0,
-                           stream, data, n_inner, ostride, lstride, estride, W);
+                           stream, data, n_inner, ostride, lstride, estride, W, ocnt, omod,
+                           emask);
     else
         hipLaunchKernelGGL((k_fft_lines_c<T, -1, N, TL>), dim3((unsigned)nblk), dim3(256), 0,
-                           stream, data, n_inner, ostride, lstride, estride, W);
+                           stream, data, n_inner, ostride, lstride, estride, W, ocnt, omod,
+                           emask);
     return true;
+}
+
+// true if n has a compile-time line kernel
+static inline bool fft_has_c(int64_t n)
+{
+    switch (n)
+    {
+        case 12: case 24: case 32: case 36: case 45: case 48:
+        case 60: case 64: case 72: case 96: case 108: case 216: return true;
+        default: return false;
+    }
 }
 
 template <typename T>
@@ -454,24 +475,28 @@ static inline const cpx<T>* axis_tw(const FftAxis& a);
 // neighbours); other lengths use the generic kernel
 template <typename T>
 static bool fft_try_c(int n, cpx<T>* data, int64_t no, int64_t ni, int64_t os, int64_t ls,
-                      int64_t es, int sign, const cpx<T>* W, hipStream_t st)
+                      int64_t es, int sign, const cpx<T>* W, hipStream_t st,
+                      const int* ocnt = nullptr, int omod = 1, const int* emask = nullptr)
 {
+#define FFT_C_CASE(NV)                                                                        \
+    case NV: return fft_launch_c<T, NV>(data, no, ni, os, ls, es, sign, W, st, ocnt, omod, emask);
     switch (n)
     {
-        case 12: return fft_launch_c<T, 12>(data, no, ni, os, ls, es, sign, W, st);
-        case 24: return fft_launch_c<T, 24>(data, no, ni, os, ls, es, sign, W, st);
-        case 32: return fft_launch_c<T, 32>(data, no, ni, os, ls, es, sign, W, st);
-        case 36: return fft_launch_c<T, 36>(data, no, ni, os, ls, es, sign, W, st);
-        case 45: return fft_launch_c<T, 45>(data, no, ni, os, ls, es, sign, W, st);
-        case 48: return fft_launch_c<T, 48>(data, no, ni, os, ls, es, sign, W, st);
-        case 60: return fft_launch_c<T, 60>(data, no, ni, os, ls, es, sign, W, st);
-        case 64: return fft_launch_c<T, 64>(data, no, ni, os, ls, es, sign, W, st);
-        case 72: return fft_launch_c<T, 72>(data, no, ni, os, ls, es, sign, W, st);
-        case 96: return fft_launch_c<T, 96>(data, no, ni, os, ls, es, sign, W, st);
-        case 108: return fft_launch_c<T, 108>(data, no, ni, os, ls, es, sign, W, st);
-        case 216: return fft_launch_c<T, 216>(data, no, ni, os, ls, es, sign, W, st);
+        FFT_C_CASE(12)
+        FFT_C_CASE(24)
+        FFT_C_CASE(32)
+        FFT_C_CASE(36)
+        FFT_C_CASE(45)
+        FFT_C_CASE(48)
+        FFT_C_CASE(60)
+        FFT_C_CASE(64)
+        FFT_C_CASE(72)
+        FFT_C_CASE(96)
+        FFT_C_CASE(108)
+        FFT_C_CASE(216)
         default: return false;
     }
+#undef FFT_C_CASE
 }
 
 template <>

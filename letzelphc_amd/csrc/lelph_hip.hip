@@ -20,6 +20,9 @@
 #include "nonloc.hpp"
 #include "split.hpp"
 #include "splitpc.hpp"
+#include "boxes.hpp"
+
+#include <type_traits>
 #include "wfc.hpp"
 
 namespace
@@ -127,6 +130,8 @@ struct lelph_ctx
 
     // workspaces
     DBuf idx_k, idx_kq, ph_k, ph_kq, box_k, box_kq, part, g, gbatch, presplit;
+    // pruned transforms: per side x-plane counts, offsets, cursors, G order
+    DBuf pl_cnt[2], pl_off[2], pl_cur[2], pl_perm[2];
     DBuf nlK, nlKp, Y_K, Y_Kp, wr_K, wr_Kp, beta_K, beta_Kp, bb_K, bb_Kp, bbpart, ebuf;
 
     // per-pair events: [0] start, [1] boxes built, [2] contraction done,
@@ -298,6 +303,45 @@ static int ensure_dv32(lelph_ctx* c)
 }
 
 // psi_k(r), psi_kq(r) boxes: scatter + backward FFT
+// one side of the pruned FP32 transform (boxes.hpp): group the sphere by x
+// plane, gather + z transform per occupied plane, y pass on occupied planes,
+// x pass reading occupied planes only
+static int pruned_side(lelph_ctx* c, const DevWfc& w, const SideGeom& geo, const int* idx,
+                       const z64* ph, int side, c32* box)
+{
+    const lelph_lattice& L = c->lat;
+    const int Nx = (int)L.fft_dims[0], Ny = (int)L.fft_dims[1], Nz = (int)L.fft_dims[2];
+    const int nyz = Ny * Nz;
+    const int nbox = L.nspin * L.nbnds * L.nspinor;
+    LELPH_TRY(c->pl_cnt[side].ensure(sizeof(int) * Nx));
+    LELPH_TRY(c->pl_off[side].ensure(sizeof(int) * Nx));
+    LELPH_TRY(c->pl_cur[side].ensure(sizeof(int) * Nx));
+    LELPH_TRY(c->pl_perm[side].ensure(sizeof(int) * std::max<int64_t>(1, w.npw)));
+    int* cnt = c->pl_cnt[side].as<int>();
+    HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * Nx, c->stream));
+    const unsigned gb = (unsigned)cdiv(w.npw, 4096);
+    hipLaunchKernelGGL(k_plane_count, dim3(gb), dim3(256), 0, c->stream, idx, w.npw, nyz, Nx, cnt);
+    hipLaunchKernelGGL(k_plane_scan, dim3(1), dim3(1024), 0, c->stream, cnt, Nx,
+                       c->pl_off[side].as<int>(), c->pl_cur[side].as<int>());
+    hipLaunchKernelGGL(k_plane_fill, dim3(gb), dim3(256), 0, c->stream, idx, w.npw, nyz, Nx,
+                       c->pl_cur[side].as<int>(), c->pl_perm[side].as<int>());
+    HIP_TRY(hipGetLastError());
+    if (!plane_z_launch(Nz, Nx, nbox, plane_lds_bytes(Ny, Nz), w.wfc, L.nspinor, w.npw, idx, ph,
+                        geo, c->pl_perm[side].as<int>(), c->pl_off[side].as<int>(), cnt, Ny,
+                        c->nf, c->ax[2].w32, box, c->stream))
+        return LELPH_ERR_UNSUPPORTED;
+    HIP_TRY(hipGetLastError());
+    if (!fft_try_c<float>(Ny, box, (int64_t)nbox * Nx, Nz, nyz, 1, Nz, +1, c->ax[1].w32, c->stream,
+                          cnt, Nx, nullptr))
+        return LELPH_ERR_UNSUPPORTED;
+    HIP_TRY(hipGetLastError());
+    if (!fft_try_c<float>(Nx, box, nbox, nyz, c->nf, 1, nyz, +1, c->ax[0].w32, c->stream, nullptr,
+                          1, cnt))
+        return LELPH_ERR_UNSUPPORTED;
+    HIP_TRY(hipGetLastError());
+    return LELPH_OK;
+}
+
 template <typename T>
 static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
 {
@@ -319,15 +363,32 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
     LELPH_TRY(c->idx_kq.ensure(sizeof(int) * c->npw_max));
     LELPH_TRY(c->ph_k.ensure(sizeof(z64) * c->npw_max));
     LELPH_TRY(c->ph_kq.ensure(sizeof(z64) * c->npw_max));
-    HIP_TRY(hipMemsetAsync(c->box_k.p, 0, bb, c->stream));
-    HIP_TRY(hipMemsetAsync(c->box_kq.p, 0, bb, c->stream));
     const int Nx = (int)L.fft_dims[0], Ny = (int)L.fft_dims[1], Nz = (int)L.fft_dims[2];
+    // FP32 boxes: pruned transforms (boxes.hpp) when every axis has a
+    // compile-time line kernel and a y-z plane fits in LDS
+    const bool pruned = std::is_same<T, float>::value && fft_has_c(Nx) && fft_has_c(Ny) &&
+                        fft_has_c(Nz) && Nx <= 1024 && plane_lds_bytes(Ny, Nz) <= 160 * 1024 &&
+                        nbox <= 65535;
+    if (!pruned)
+    {
+        HIP_TRY(hipMemsetAsync(c->box_k.p, 0, bb, c->stream));
+        HIP_TRY(hipMemsetAsync(c->box_kq.p, 0, bb, c->stream));
+    }
     hipLaunchKernelGGL(k_gindex, dim3((unsigned)cdiv(wk.npw, 256)), dim3(256), 0, c->stream,
                        wk.gvec, wk.npw, gk, Nx, Ny, Nz, c->idx_k.as<int>(), c->ph_k.as<z64>(),
                        c->d_err);
     hipLaunchKernelGGL(k_gindex, dim3((unsigned)cdiv(wkq.npw, 256)), dim3(256), 0, c->stream,
                        wkq.gvec, wkq.npw, gkq, Nx, Ny, Nz, c->idx_kq.as<int>(),
                        c->ph_kq.as<z64>(), c->d_err);
+    if constexpr (std::is_same<T, float>::value)
+        if (pruned)
+        {
+            LELPH_TRY(pruned_side(c, wk, gk, c->idx_k.as<int>(), c->ph_k.as<z64>(), 0,
+                                  c->box_k.as<c32>()));
+            LELPH_TRY(pruned_side(c, wkq, gkq, c->idx_kq.as<int>(), c->ph_kq.as<z64>(), 1,
+                                  c->box_kq.as<c32>()));
+            return LELPH_OK;
+        }
     hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)cdiv(nsb * wk.npw, 256)), dim3(256), 0,
                        c->stream, wk.wfc, nsb, L.nspinor, wk.npw, c->idx_k.as<int>(),
                        c->ph_k.as<z64>(), gk, c->nf, c->box_k.as<cpx<T>>());
