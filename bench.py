@@ -176,7 +176,7 @@ def main():
     t64_launch = float(np.mean(c64)) / len(pairs) / 1e3
 
     traffic = None
-    prof = os.path.join(ROOT, "profiles", "k_contract_split_bf16x3_c2.md")
+    prof = os.path.join(ROOT, "profiles", "k_contract_split_pc_bf16x3_c2.md")
     if args.config == "c2" and os.path.exists(prof):
         import re
         m = re.search(rf"traffic_bytes_per_launch\[{head}\] = (\d+)", open(prof).read())
@@ -212,11 +212,12 @@ def main():
         "ratio_vs_fp64": value / v64,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
-                     "kernel": (("k_contract_split" if head.startswith("bf16") else "k_contract")
+                     "kernel": ({"bf16x3": "k_contract_split_pc", "bf16x6": "k_contract_split"}
+                                .get(head, "k_contract")
                                 + f" ({head}, {PRODUCTS[head]} MFMA product(s) per real product; "
                                   "achieved counts the 8 useful flops per complex MAC)"),
                      "traffic_source": ("FETCH_SIZE + WRITE_SIZE PMC passes, copy-calibrated, "
-                                        "profiles/k_contract_split_bf16x3_c2.md")
+                                        "profiles/k_contract_split_pc_bf16x3_c2.md")
                      if traffic else None},
         "cpu_baseline": cpu,
     }

@@ -2,7 +2,7 @@
 //
 // Same arithmetic as k_contract_split (split.hpp): A = dV_nu psi_k split into
 // 2 bf16 pieces, B = psi_k+q pieces (presplit), 3 piece products per real
-// product on v_mfma_f32_16x16x32_bf16, FP32 chains of SP_FLUSH * 32 points
+// product on v_mfma_f32_16x16x32_bf16, FP32 chains of PC_FLUSH * 32 points
 // flushed into FP64.  The difference is the division of labour inside a
 // block, which lets the VALU work of the split overlap the MFMAs:
 //
@@ -31,6 +31,7 @@
 #include "split.hpp"
 
 #define PC_TR 64  // rows of a block tile
+#define PC_FLUSH 8 // chunks per FP32 accumulation chain (256 points)
 #define PC_TC 128 // columns of a block tile
 
 template <int NP>
@@ -331,7 +332,7 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
                     acci[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nre[p1], bi[p2], acci[cb], 0, 0, 0);
                 }
         }
-        if (++nfl == SP_FLUSH)
+        if (++nfl == PC_FLUSH)
         {
             nfl = 0;
 #pragma unroll
