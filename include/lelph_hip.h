@@ -211,6 +211,14 @@ int lelph_elph_q_rotate(const double* Dmats_l, const double* elph_mat_q,
                         const double* Dmats_r, int nmodes, int nspin,
                         int nbnds, int tim_rev, double* elph_mat_Sq);
 
+/* electronic_reps (src/symmetries/elec_reps.c:16-413): representation
+ * matrices Dkmn (nspin, nbnds, nbnds) = <R k, b | U(R) | k, a> of the
+ * symmetry (Rsym cartesian row major, tauR cartesian fractional translation,
+ * tim_revR) at full-BZ point ikBZ.  R k must be on the k grid
+ * (LELPH_ERR_SYM otherwise). */
+int lelph_electronic_reps(lelph_ctx* ctx, const double Rsym[9], const double tauR[3],
+                          int tim_revR, int64_t ikBZ, double* Dkmn);
+
 /* get_KplusQ_idxs (src/common/numerical_func.c:471-524). */
 int lelph_get_kplusq_idxs(int64_t nbz, const double* kpoints_crys,
                           const double qpt_crys[3], int* kplusq_idxs);

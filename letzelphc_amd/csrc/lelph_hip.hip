@@ -21,6 +21,7 @@
 #include "split.hpp"
 #include "splitpc.hpp"
 #include "boxes.hpp"
+#include "dmats.hpp"
 
 #include <type_traits>
 #include "wfc.hpp"
@@ -132,6 +133,8 @@ struct lelph_ctx
     DBuf idx_k, idx_kq, ph_k, ph_kq, box_k, box_kq, part, g, gbatch, presplit;
     // pruned transforms: per side x-plane counts, offsets, cursors, G order
     DBuf pl_cnt[2], pl_off[2], pl_cur[2], pl_perm[2];
+    // electronic representation matrices
+    DBuf dm_map, dm_ph, dm_pos, dm_wRS, dm_wS2, dm_part, dm_out;
     DBuf nlK, nlKp, Y_K, Y_Kp, wr_K, wr_Kp, beta_K, beta_Kp, bb_K, bb_Kp, bbpart, ebuf;
 
     // per-pair events: [0] start, [1] boxes built, [2] contraction done,
@@ -1268,6 +1271,134 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
                            c->stream));
     LELPH_TRY(check_err(c));
     return finish_timings(c, npairs);
+}
+
+// electronic_reps (src/symmetries/elec_reps.c:16-413)
+int lelph_electronic_reps(lelph_ctx* c, const double Rsym[9], const double tauR[3], int tim_revR,
+                          int64_t ikBZ, double* Dkmn)
+{
+    if (!c || !Rsym || !tauR || !Dkmn) return LELPH_ERR_ARG;
+    const lelph_lattice& L = c->lat;
+    if (c->kpt_bz.empty()) return LELPH_ERR_STATE;
+    if (ikBZ < 0 || ikBZ >= L.nkpts_BZ) return LELPH_ERR_ARG;
+    const double* alat = L.alat_vec;
+    const double* blat = L.blat_vec;
+    // R k_BZ (crystal) and its index in the full-BZ list (:34-53)
+    const double* kbc = c->kpt_bz.data() + 3 * ikBZ;
+    double kbz[3], t[3], Rk[3];
+    for (int i = 0; i < 3; ++i)
+        kbz[i] = (blat[3 * i] * kbc[0] + blat[3 * i + 1] * kbc[1] + blat[3 * i + 2] * kbc[2]) /
+                 (2 * LELPH_RPI);
+    matvec3(Rsym, kbz, false, t);
+    matvec3(alat, t, true, Rk);
+    int64_t iRk = -1;
+    for (int64_t i = 0; i < L.nkpts_BZ && iRk < 0; ++i)
+    {
+        double s2 = 0;
+        for (int j = 0; j < 3; ++j)
+        {
+            double d = c->kpt_bz[3 * i + j] - Rk[j];
+            d -= rint(d);
+            s2 += d * d;
+        }
+        if (sqrt(s2) < LELPH_EPS) iRk = i;
+    }
+    if (iRk < 0) return LELPH_ERR_SYM;
+    const int ik1 = c->kmap[2 * ikBZ], iS1 = c->kmap[2 * ikBZ + 1];
+    const int ik2 = c->kmap[2 * iRk], iS2 = c->kmap[2 * iRk + 1];
+    if (ik1 < 0 || ik2 < 0 || ik1 >= (int)c->wf.size() || ik2 >= (int)c->wf.size())
+        return LELPH_ERR_ARG;
+    const DevWfc& w1 = c->wf[ik1];
+    const DevWfc& w2 = c->wf[ik2];
+    if (!w1.wfc || !w2.wfc) return LELPH_ERR_STATE;
+    const lelph_symmetry& sy1 = c->syms[iS1];
+    const lelph_symmetry& sy2 = c->syms[iS2];
+    const double* k1 = c->kpt_ibz.data() + 3 * ik1;
+    const double* k2 = c->kpt_ibz.data() + 3 * ik2;
+    // geometry (:83-176)
+    DmGeom geo{};
+    matvec3(sy1.Rmat, k1, false, t);
+    matvec3(alat, t, true, geo.kcrys);
+    matvec3(blat, sy1.tau, true, geo.tau1c);
+    matvec3(blat, tauR, true, geo.tauRc);
+    matvec3(blat, sy2.tau, true, geo.tau2c);
+    for (int i = 0; i < 3; ++i)
+    {
+        geo.tau1c[i] /= 2 * LELPH_RPI;
+        geo.tauRc[i] /= 2 * LELPH_RPI;
+        geo.tau2c[i] /= 2 * LELPH_RPI;
+    }
+    double RS1[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            RS1[3 * i + j] = Rsym[3 * i] * sy1.Rmat[j] + Rsym[3 * i + 1] * sy1.Rmat[3 + j] +
+                             Rsym[3 * i + 2] * sy1.Rmat[6 + j];
+    double ulm[3], Rkc[3];
+    matvec3(sy2.Rmat, k2, false, ulm);
+    matvec3(RS1, k1, false, Rkc);
+    matvec3(alat, Rkc, true, geo.Rkcrys);
+    for (int i = 0; i < 3; ++i) ulm[i] -= Rkc[i];
+    matvec3(alat, ulm, true, geo.ulmc);
+    // cartesian G -> crystal: M = alat^T S (rotate_Gvecs.c:15-81)
+    const double* Ss[3] = {sy1.Rmat, RS1, sy2.Rmat};
+    double* Ms[3] = {geo.M1, geo.M2, geo.M3};
+    for (int m = 0; m < 3; ++m)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                Ms[m][3 * i + j] = alat[i] * Ss[m][j] + alat[3 + i] * Ss[m][3 + j] +
+                                   alat[6 + i] * Ss[m][6 + j];
+    geo.tr1 = sy1.time_rev;
+    geo.trR = tim_revR ? 1 : 0;
+    geo.tr2 = sy2.time_rev;
+    LELPH_TRY(su2mat(sy1.Rmat, L.nspinor, sy1.time_rev, geo.su1));
+    LELPH_TRY(su2mat(Rsym, L.nspinor, tim_revR, geo.suR));
+    LELPH_TRY(su2mat(sy2.Rmat, L.nspinor, sy2.time_rev, geo.su2));
+    // device work
+    const int Nx = (int)L.fft_dims[0], Ny = (int)L.fft_dims[1], Nz = (int)L.fft_dims[2];
+    const int64_t nsets = (int64_t)L.nspin * L.nbnds, Lk = (int64_t)L.nspinor * w1.npw;
+    const int nb = L.nbnds;
+    LELPH_TRY(c->dm_map.ensure(sizeof(int) * c->nf));
+    LELPH_TRY(c->dm_ph.ensure(sizeof(z64) * 3 * std::max<int64_t>(1, w1.npw)));
+    LELPH_TRY(c->dm_pos.ensure(sizeof(int) * std::max<int64_t>(1, w2.npw)));
+    LELPH_TRY(c->dm_wRS.ensure(sizeof(z64) * nsets * Lk));
+    LELPH_TRY(c->dm_wS2.ensure(sizeof(z64) * nsets * Lk));
+    HIP_TRY(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+    HIP_TRY(hipMemsetAsync(c->dm_map.p, 0xFF, sizeof(int) * c->nf, c->stream));
+    HIP_TRY(hipMemsetAsync(c->dm_wS2.p, 0, sizeof(z64) * nsets * Lk, c->stream));
+    hipLaunchKernelGGL(k_dm_side1, dim3((unsigned)cdiv(w1.npw, 256)), dim3(256), 0, c->stream,
+                       w1.gvec, w1.npw, geo, Nx, Ny, Nz, c->dm_map.as<int>(), c->dm_ph.as<z64>(),
+                       c->d_err);
+    hipLaunchKernelGGL(k_dm_side2, dim3((unsigned)cdiv(w2.npw, 256)), dim3(256), 0, c->stream,
+                       w2.gvec, w2.npw, geo, Nx, Ny, Nz, c->dm_map.as<int>(), c->dm_pos.as<int>());
+    hipLaunchKernelGGL(k_dm_rot1, dim3((unsigned)cdiv(nsets * w1.npw, 256)), dim3(256), 0,
+                       c->stream, w1.wfc, nsets, L.nspinor, w1.npw, c->dm_ph.as<z64>(), geo,
+                       c->dm_wRS.as<z64>());
+    hipLaunchKernelGGL(k_dm_rot2, dim3((unsigned)cdiv(nsets * w2.npw, 256)), dim3(256), 0,
+                       c->stream, w2.wfc, nsets, L.nspinor, w2.npw, w1.npw, c->dm_pos.as<int>(),
+                       c->dm_ph.as<z64>(), geo, c->dm_wS2.as<z64>());
+    HIP_TRY(hipGetLastError());
+    // D[s] = wS2[s] (nb x Lk) . wRS[s]^T (:377-394), split over the sphere
+    const int64_t nchunks = cdiv(Lk, CT_KC);
+    const int64_t tiles = cdiv(nb, GZ_MB) * cdiv(nb, GZ_NB);
+    int64_t cps;
+    int nsplit;
+    pick_splits(tiles, nchunks, (int64_t)nb * nb, &cps, &nsplit);
+    LELPH_TRY(c->dm_part.ensure(sizeof(z64) * (size_t)nsplit * nb * nb));
+    LELPH_TRY(c->dm_out.ensure(sizeof(z64) * (size_t)L.nspin * nb * nb));
+    for (int is = 0; is < L.nspin; ++is)
+    {
+        hipLaunchKernelGGL(k_zgemm_nt<double>, dim3((unsigned)tiles, (unsigned)nsplit), dim3(256),
+                           0, c->stream, c->dm_wS2.as<z64>() + is * nb * Lk,
+                           c->dm_wRS.as<z64>() + is * nb * Lk, nb, nb, Lk, 0, cps,
+                           c->dm_part.as<z64>());
+        hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)cdiv((int64_t)nb * nb, 256)), dim3(256),
+                           0, c->stream, c->dm_part.as<z64>(), nsplit, (int64_t)nb * nb, 1.0,
+                           c->dm_out.as<z64>() + (int64_t)is * nb * nb);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(Dkmn, c->dm_out.p, sizeof(z64) * L.nspin * nb * nb,
+                           hipMemcpyDeviceToHost, c->stream));
+    return check_err(c);
 }
 
 int lelph_internal_grid(const lelph_ctx* c, int64_t* nbz, const double** kcrys, const int** kmap)

@@ -80,6 +80,8 @@ def lib():
         L.lelph_get_kplusq_idxs.argtypes = [C.c_int64, c_double_p, c_double_p,
                                             C.POINTER(C.c_int)]
         L.lelph_last_timings.argtypes = [C.c_void_p, c_double_p, C.c_int]
+        L.lelph_electronic_reps.argtypes = [C.c_void_p, c_double_p, c_double_p, C.c_int,
+                                            C.c_int64, c_double_p]
         _lib = L
     return _lib
 
@@ -216,6 +218,17 @@ class Context:
         out = np.zeros((nbz,) + self._gshape(), np.complex128)
         _check(lib().lelph_compute_elphq(self.h, dptr(q), int(non_loc), int(kminusq), dptr(out)),
                "lelph_compute_elphq")
+        return out
+
+    def electronic_reps(self, Rsym, tauR, tim_rev, ikbz) -> np.ndarray:
+        """D (nspin, nb, nb) = <R k, b | U(R) | k, a> at full-BZ point ikbz
+        (electronic_reps, src/symmetries/elec_reps.c:16-413)."""
+        s = self.sys
+        R = np.ascontiguousarray(Rsym, np.float64).reshape(9)
+        t = np.ascontiguousarray(tauR, np.float64).reshape(3)
+        out = np.zeros((s.nspin, s.nbnds, s.nbnds), np.complex128)
+        _check(lib().lelph_electronic_reps(self.h, dptr(R), dptr(t), int(bool(tim_rev)),
+                                           int(ikbz), dptr(out)), "electronic_reps")
         return out
 
     def last_timings(self) -> dict:

@@ -1323,3 +1323,141 @@ void orc_sum_VNL_KKp(const double* K, const double* Kp, const double* fcoeff,
     sum_VNL_KKp((const cplx*)K, (const cplx*)Kp, (const cplx*)fcoeff, nspin,
                 nb, nspinor, (cplx*)out);
 }
+
+/* ======================================================================
+ * find_gvecs_idxs (src/wfc/gsort.c:49-95): index of every search vector in
+ * the list (G compared after rint, ordered x, then y, then z), -1 if absent;
+ * uses the z-stick ordering of sort_pw above.
+ * ==================================================================== */
+void orc_find_gvecs_idxs(int64_t nsearch, const double* search, int64_t npw,
+                         const double* gvecs, int64_t* out)
+{
+    gkey* k = malloc(sizeof(gkey) * (npw > 0 ? npw : 1));
+    for (int64_t i = 0; i < npw; ++i)
+    {
+        k[i].x = (long long)rint(gvecs[3 * i]);
+        k[i].y = (long long)rint(gvecs[3 * i + 1]);
+        k[i].z = (long long)rint(gvecs[3 * i + 2]);
+        k[i].i = i;
+    }
+    qsort(k, npw, sizeof(gkey), gkey_cmp);
+    for (int64_t s = 0; s < nsearch; ++s)
+    {
+        gkey q = {(long long)rint(search[3 * s]), (long long)rint(search[3 * s + 1]),
+                  (long long)rint(search[3 * s + 2]), 0};
+        gkey* hit = bsearch(&q, k, npw, sizeof(gkey), gkey_cmp);
+        out[s] = hit ? hit->i : -1;
+    }
+    free(k);
+}
+
+/* ======================================================================
+ * electronic_reps (src/symmetries/elec_reps.c:16-413), one rank:
+ * Dkmn[s][b][a] = <S2 k2, b | U(R) U(S1) | k1, a> with k_BZ = S1 k1 and
+ * R k_BZ = S2 k2 (up to a G vector).
+ * ==================================================================== */
+int orc_electronic_reps(const orc_wfc* wfcs, const lelph_lattice* lat,
+                        const double* Rsym, const double* tauR, int tim_revR,
+                        int64_t ikBZ, double* Dkmn_d)
+{
+    cplx* Dkmn = (cplx*)Dkmn_d;
+    const double* alat = lat->alat_vec;
+    const double* blat = lat->blat_vec;
+    const int64_t nspin = lat->nspin, nb = lat->nbnds, nspinor = lat->nspinor;
+    /* R k_BZ in crystal coordinates (:34-46); k_BZ cartesian from its
+     * crystal coordinates, without 2 pi */
+    const double* kbc = lat->kpt_fullBZ_crys + 3 * ikBZ;
+    double kbz[3];
+    for (int i = 0; i < 3; ++i)
+        kbz[i] = (blat[3 * i] * kbc[0] + blat[3 * i + 1] * kbc[1] + blat[3 * i + 2] * kbc[2]) /
+                 (2 * RPI);
+    double Rk_tmp[3], Rk_vec[3];
+    matvec3(Rsym, kbz, false, Rk_tmp);
+    matvec3(alat, Rk_tmp, true, Rk_vec);
+    const int64_t iRk = find_kidx(lat->nkpts_BZ, lat->kpt_fullBZ_crys, Rk_vec);
+    if (iRk < 0) return LELPH_ERR_SYM;
+    /* :69-80 */
+    const int ik1 = lat->kmap[2 * ikBZ], iS1 = lat->kmap[2 * ikBZ + 1];
+    const int ik2 = lat->kmap[2 * iRk], iS2 = lat->kmap[2 * iRk + 1];
+    const double* S1 = lat->syms[iS1].Rmat;
+    const double* tau1 = lat->syms[iS1].tau;
+    const int tr1 = lat->syms[iS1].time_rev;
+    const double* S2 = lat->syms[iS2].Rmat;
+    const double* tau2 = lat->syms[iS2].tau;
+    const int tr2 = lat->syms[iS2].time_rev;
+    const double* k1 = lat->kpt_iredBZ + 3 * ik1;
+    const double* k2 = lat->kpt_iredBZ + 3 * ik2;
+    /* :83-103 */
+    double kcrys[3], Rkcrys[3], t3[3];
+    matvec3(S1, k1, false, t3);
+    matvec3(alat, t3, true, kcrys);
+    double tau1c[3], tauRc[3], tau2c[3];
+    matvec3(blat, tau1, true, tau1c);
+    matvec3(blat, tauR, true, tauRc);
+    matvec3(blat, tau2, true, tau2c);
+    for (int i = 0; i < 3; ++i)
+    {
+        tau1c[i] /= 2 * RPI;
+        tauRc[i] /= 2 * RPI;
+        tau2c[i] /= 2 * RPI;
+    }
+    const int64_t npw1 = wfcs[ik1].npw, npw2 = wfcs[ik2].npw;
+    const cplx* w1 = (const cplx*)wfcs[ik1].wfc;
+    const cplx* w2 = (const cplx*)wfcs[ik2].wfc;
+    /* :121-127 */
+    cplx su1[4] = {1, 0, 0, 1}, suR[4] = {1, 0, 0, 1}, su2[4] = {1, 0, 0, 1};
+    int err;
+    if ((err = orc_SU2mat(S1, nspinor, 0, tr1, (double*)su1))) return err;
+    if ((err = orc_SU2mat(Rsym, nspinor, 0, tim_revR, (double*)suR))) return err;
+    if ((err = orc_SU2mat(S2, nspinor, 0, tr2, (double*)su2))) return err;
+    /* :129-176 */
+    double* G_S1k1 = malloc(sizeof(double) * 3 * npw1);
+    double* G_RS1k1 = malloc(sizeof(double) * 3 * npw1);
+    double* G_S2k2 = malloc(sizeof(double) * 3 * npw2);
+    double RS1[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            RS1[3 * i + j] = Rsym[3 * i] * S1[j] + Rsym[3 * i + 1] * S1[3 + j] +
+                             Rsym[3 * i + 2] * S1[6 + j];
+    double ulm[3], Rkc[3];
+    matvec3(S2, k2, false, ulm);
+    matvec3(RS1, k1, false, Rkc);
+    matvec3(alat, Rkc, true, Rkcrys);
+    for (int i = 0; i < 3; ++i) ulm[i] -= Rkc[i];
+    rotateGvecs(wfcs[ik1].gvec, S1, npw1, alat, true, NULL, G_S1k1);
+    rotateGvecs(wfcs[ik1].gvec, RS1, npw1, alat, true, NULL, G_RS1k1);
+    rotateGvecs(wfcs[ik2].gvec, S2, npw2, alat, true, ulm, G_S2k2);
+    /* :256-262 */
+    int64_t* idx = malloc(sizeof(int64_t) * (npw2 > 0 ? npw2 : 1));
+    orc_find_gvecs_idxs(npw2, G_S2k2, npw1, G_RS1k1, idx);
+    /* :304-373 */
+    const int64_t nsets = nspin * nb, L = nspinor * npw1;
+    cplx* wRS = malloc(sizeof(cplx) * nsets * L);
+    cplx* wS2 = calloc(nsets * L, sizeof(cplx));
+    for (int64_t s = 0; s < nsets; ++s)
+    {
+        cplx* a = wRS + s * L;
+        memcpy(a, w1 + s * L, sizeof(cplx) * L);
+        su2rotate(nspinor, npw1, su1, a);
+        apply_trans_wfc(tau1c, kcrys, nspinor, npw1, G_S1k1, a, tr1);
+        su2rotate(nspinor, npw1, suR, a);
+        apply_trans_wfc(tauRc, Rkcrys, nspinor, npw1, G_RS1k1, a, tim_revR);
+        cplx* b = wS2 + s * L;
+        for (int64_t sp = 0; sp < nspinor; ++sp)
+            for (int64_t i = 0; i < npw2; ++i)
+                if (idx[i] >= 0) b[sp * npw1 + idx[i]] = w2[s * nspinor * npw2 + sp * npw2 + i];
+        su2rotate(nspinor, npw1, su2, b);
+        apply_trans_wfc(tau2c, Rkcrys, nspinor, npw1, G_RS1k1, b, !tr2);
+    }
+    /* :377-394 */
+    for (int64_t is = 0; is < nspin; ++is)
+        zgemm_rm('N', 'T', wS2 + is * nb * L, wRS + is * nb * L, Dkmn + is * nb * nb, 1.0,
+                 0.0, L, L, nb, nb, nb, L);
+    free(G_S1k1);
+    free(G_RS1k1);
+    free(G_S2k2);
+    free(idx);
+    free(wRS);
+    free(wS2);
+    return 0;
+}

@@ -106,6 +106,14 @@ void orc_apply_trans_wfc(const double* tr, const double* kvec, int64_t nsets,
 void orc_sum_VNL_KKp(const double* K, const double* Kp, const double* fcoeff,
                      int64_t nspin, int64_t nb, int64_t nspinor, double* out);
 
+/* G-vector matching (src/wfc/gsort.c:49-95) */
+void orc_find_gvecs_idxs(int64_t nsearch, const double* search, int64_t npw,
+                         const double* gvecs, int64_t* out);
+/* electronic_reps (src/symmetries/elec_reps.c:16-413): Dkmn (nspin,nb,nb) */
+int orc_electronic_reps(const orc_wfc* wfcs, const lelph_lattice* lat,
+                        const double* Rsym, const double* tauR, int tim_revR,
+                        int64_t ikBZ, double* Dkmn);
+
 #ifdef __cplusplus
 }
 #endif

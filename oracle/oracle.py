@@ -85,6 +85,10 @@ def lib():
                                           c_double_p, c_double_p, C.c_int]
         L.orc_sum_VNL_KKp.argtypes = [c_double_p, c_double_p, c_double_p, C.c_int64,
                                       C.c_int64, C.c_int64, c_double_p]
+        L.orc_find_gvecs_idxs.argtypes = [C.c_int64, c_double_p, C.c_int64, c_double_p,
+                                          C.POINTER(C.c_int64)]
+        L.orc_electronic_reps.argtypes = [C.POINTER(OrcWfc), C.POINTER(Lattice), c_double_p,
+                                          c_double_p, C.c_int, C.c_int64, c_double_p]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_inner_threads.argtypes = [C.c_int]
         _lib = L
@@ -193,6 +197,15 @@ class OracleSystem:
         check(lib().orc_compute_elphq(dptr(q), self.wtab, self.lat, self.pp,
                                       dptr(s.eigvec(iq)), dptr(dv), int(non_loc),
                                       int(kminusq), k_first, k_count, dptr(out)))
+        return out
+
+    def electronic_reps(self, Rsym, tauR, tim_rev, ikbz) -> np.ndarray:
+        s = self.sys
+        R = np.ascontiguousarray(Rsym, np.float64).reshape(9)
+        t = np.ascontiguousarray(tauR, np.float64).reshape(3)
+        out = np.zeros((s.nspin, s.nbnds, s.nbnds), np.complex128)
+        check(lib().orc_electronic_reps(self.wtab, self.lat, dptr(R), dptr(t),
+                                        int(bool(tim_rev)), int(ikbz), dptr(out)))
         return out
 
     def __del__(self):

@@ -180,3 +180,46 @@ def dvlocq_closed(s, q, eig):
             coef += eig[:, 3 * ia + x][:, None, None, None] * ((-1j) * qg[..., x] * ph * V)[None]
     N = int(np.prod(dims))
     return np.fft.ifftn(coef, axes=(1, 2, 3)) * N
+
+
+def dmat(s, R, trR, ikbz):
+    """<R k, b| U(R) |k, a> (nspin, nb, nb) for nspinor == 1 and zero
+    fractional translations, by explicit G matching: U(R) U(S1) psi_k1 has
+    coefficients conj^(tr1 xor trR)(c_k1) at R S1 G; U(S2) psi_k2 has
+    conj^tr2(c_k2) at S2 G + G0, G0 = S2 k2 - R S1 k1 (crystal)."""
+    assert s.nspinor == 1
+    kb = s.kpt_bz_crys
+    Rc = synth._sym_crys(np.asarray(R), s.alat, s.blat)
+    Rk = Rc @ kb[ikbz]
+    d = kb - Rk[None, :]
+    d = d - np.rint(d)
+    j = int(np.argmin(np.linalg.norm(d, axis=1)))
+    assert np.linalg.norm(d[j]) < 1e-5
+    ik1, is1 = (int(x) for x in s.kmap[ikbz])
+    ik2, is2 = (int(x) for x in s.kmap[j])
+    S1c, S2c = sym_crys(s, is1), sym_crys(s, is2)
+    tr1, tr2 = int(s.syms[is1][2]), int(s.syms[is2][2])
+    k1, k2 = s.kpt_ibz_crys[ik1], s.kpt_ibz_crys[ik2]
+    ulm = S2c @ k2 - Rc @ S1c @ k1
+    assert np.allclose(ulm, np.rint(ulm), atol=1e-6)
+    w1, w2 = s.wfcs[ik1], s.wfcs[ik2]
+    Ga = np.rint(w1.gint @ (Rc @ S1c).T).astype(np.int64)
+    Gb = np.rint(w2.gint @ S2c.T + ulm[None, :]).astype(np.int64)
+    ca = w1.wfc[:, :, 0, :]
+    if tr1 ^ int(bool(trR)):
+        ca = np.conj(ca)
+    cb = w2.wfc[:, :, 0, :]
+    if tr2:
+        cb = np.conj(cb)
+    pos = {tuple(g): i for i, g in enumerate(Ga)}
+    ib, ia = [], []
+    for i, g in enumerate(Gb):
+        p = pos.get(tuple(g))
+        if p is not None:
+            ib.append(i)
+            ia.append(p)
+    ib, ia = np.array(ib), np.array(ia)
+    D = np.zeros((s.nspin, s.nbnds, s.nbnds), np.complex128)
+    for sp in range(s.nspin):
+        D[sp] = np.conj(cb[sp][:, ib]) @ ca[sp][:, ia].T
+    return D

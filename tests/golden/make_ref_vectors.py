@@ -6,6 +6,7 @@ own helper routines, compiled from its sources into oracle/_ref/liblelph_ref.so
   prepare_spline, spline_interpolate src/common/spline.c:13-139
   reciprocal_vecs                    src/common/numerical_func.c
   get_KplusQ_idxs                    src/common/numerical_func.c:471-524
+  find_gvecs_idxs                    src/wfc/gsort.c:49-95
   SU2mat                             src/wfc/su2mat.c:16-117
   su2rotate                          src/wfc/su2rotate.c:13-41
   apply_trans_wfc                    src/wfc/trans_wfc.c:15-52
@@ -59,6 +60,7 @@ def load_ref():
     L.Vloc_Gspace.argtypes = [dp, C.c_char, C.c_double, ll, dp, dp, dp, C.c_double,
                               C.c_double, C.c_double]
     L.sum_VNL_KKp.argtypes = [dp, dp, dp, ll, ll, ll, dp]
+    L.find_gvecs_idxs.argtypes = [ll, dp, ll, dp, C.POINTER(ll)]
     return L
 
 
@@ -194,6 +196,16 @@ def make(L, seed=2024):
                       nspin, nb, nspinor, o.ctypes.data_as(dp))
         f.update({f"vnl{nspinor}_K": K, f"vnl{nspinor}_Kp": Kp, f"vnl{nspinor}_f": fc,
                   f"vnl{nspinor}_in": o0, f"vnl{nspinor}_out": o})
+    # G-vector matching: a shuffled integer sphere, searched with a rotated
+    # copy in which some vectors are missing and some are new
+    g = np.stack(np.meshgrid(*[np.arange(-3, 4)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    g = g[np.sum(g * g, 1) <= 9].astype(np.float64)
+    gl = np.ascontiguousarray(g[rng.permutation(len(g))])
+    sr = g[:, [1, 0, 2]] * np.array([1.0, -1.0, 1.0]) + rng.choice([0, 0, 0, 1], (len(g), 1))
+    sr = np.ascontiguousarray(sr[rng.permutation(len(sr))] + 1e-9 * rng.standard_normal(sr.shape))
+    out = np.zeros(len(sr), np.int64)
+    L.find_gvecs_idxs(len(sr), P(sr), len(gl), P(gl), out.ctypes.data_as(C.POINTER(ll)))
+    f.update(gs_list=gl, gs_search=sr, gs_idx=out)
     return f
 
 

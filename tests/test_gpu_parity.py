@@ -214,3 +214,29 @@ def test_c1_symmetry_pairs():
     finally:
         O.set_inner_threads(1)
         ctx.close()
+
+
+@pytest.mark.parametrize("kind", ["cubic", "spinor", "c1"])
+def test_electronic_reps(kind):
+    """Representation matrices D = <Rk,b|U(R)|k,a> against the oracle for
+    every symmetry of the system as R (time-reversed ones included)."""
+    if kind == "cubic":
+        s = synth.tiny(nbnds=4, syms="cubic", q=(0.0, 0.5, 0.0))
+        ks = range(len(s.kpt_bz_crys))
+    elif kind == "spinor":
+        s = synth.tiny(nbnds=4, nspinor=2, nmag=1, syms="cubic", q=(0.0, 0.5, 0.0))
+        ks = range(len(s.kpt_bz_crys))
+    else:
+        s = synth.config("c1", nbnds=8, kgrid=(6, 6, 1))
+        ks = range(0, len(s.kpt_bz_crys), 5)
+    o = O.OracleSystem(s)
+    ctx = lelph.Context(s, "fp64")
+    say(f"case: electronic_reps {kind}, {len(s.syms)} symmetries")
+    try:
+        for R, tau, tr in s.syms:
+            for ik in ks:
+                got = ctx.electronic_reps(R, tau, tr, ik)
+                ref = o.electronic_reps(R, tau, tr, ik)
+                assert rel_err(got, ref) < TOL64, (kind, ik)
+    finally:
+        ctx.close()

@@ -281,3 +281,34 @@ def test_elph_q_rotate_identity_and_norm():
     a = np.sum(np.abs(out) ** 2, axis=(2, 3))
     b = np.sum(np.abs(g) ** 2, axis=(2, 3))
     assert np.max(np.abs(a - b) / b) < 1e-13
+
+
+# ---------------------------------------------------------------- D matrices
+@pytest.mark.parametrize("kind", ["cubic", "trev", "spin2"])
+def test_electronic_reps_vs_matching(kind):
+    """electronic_reps against explicit G matching of the rotated spheres,
+    for every symmetry of the system taken as R and every full-BZ k."""
+    kw = dict(nbnds=3)
+    if kind == "cubic":
+        kw.update(syms="cubic", q=(0.0, 0.5, 0.0))
+    elif kind == "trev":
+        kw.update(syms="trev")
+    else:
+        kw.update(syms="cubic", q=(0.0, 0.5, 0.0), nspin=2, nmag=2)
+    s = synth.tiny(**kw)
+    o = O.OracleSystem(s)
+    say(f"case: oracle electronic_reps vs G matching, {kind}")
+    for R, tau, tr in s.syms:
+        for ik in range(len(s.kpt_bz_crys)):
+            D = o.electronic_reps(R, tau, tr, ik)
+            ref = BF.dmat(s, R, tr, ik)
+            assert np.max(np.abs(D - ref)) < 1e-12, (kind, ik)
+
+
+def test_electronic_reps_identity():
+    """R = E: D is the identity (orthonormal bands, same state)."""
+    s = synth.tiny(nbnds=4, syms="cubic", q=(0.0, 0.5, 0.0))
+    o = O.OracleSystem(s)
+    for ik in range(len(s.kpt_bz_crys)):
+        D = o.electronic_reps(np.eye(3), np.zeros(3), 0, ik)
+        assert np.max(np.abs(D[0] - np.eye(4))) < 1e-12

@@ -4,7 +4,8 @@ tests/golden/ref_vectors.npz holds seeded inputs and the outputs of the
 reference's legendre/Ylm/simpson (src/common/numerical_func.c), spline
 (src/common/spline.c), SU2mat/su2rotate/apply_trans_wfc (src/wfc/),
 Vloc_Gspace (src/dvloc/VlocG_atom.c) and sum_VNL_KKp
-(src/nonloc/sum_VNL_KKp.c), written by tests/golden/make_ref_vectors.py from
+(src/nonloc/sum_VNL_KKp.c) and find_gvecs_idxs (src/wfc/gsort.c), written by
+tests/golden/make_ref_vectors.py from
 oracle/_ref/liblelph_ref.so.  When that library is present the same checks
 also run live on a second seed.
 """
@@ -99,6 +100,15 @@ def check_against(f):
         got = [L.orc_Vloc_Gspace(dptr(work), cut, float(G), len(r), dptr(v), dptr(r), dptr(rab),
                                  float(f["vl_Z"]), 1.0, float(f["vl_vol"])) for G in f["vl_G"]]
         _close(got, f["vl_out"][ic], 1e-12)
+    # G-vector matching
+    if "gs_list" in f:
+        gl = np.ascontiguousarray(f["gs_list"])
+        sr = np.ascontiguousarray(f["gs_search"])
+        out = np.zeros(len(sr), np.int64)
+        L.orc_find_gvecs_idxs(len(sr), dptr(sr), len(gl), dptr(gl),
+                              out.ctypes.data_as(C.POINTER(C.c_int64)))
+        np.testing.assert_array_equal(out, f["gs_idx"])
+        assert (out < 0).any() and (out >= 0).any()
     # sum_VNL_KKp
     for nspinor in (1, 2):
         K = np.ascontiguousarray(f[f"vnl{nspinor}_K"])
