@@ -219,6 +219,13 @@ int lelph_elph_q_rotate(const double* Dmats_l, const double* elph_mat_q,
 int lelph_electronic_reps(lelph_ctx* ctx, const double Rsym[9], const double tauR[3],
                           int tim_revR, int64_t ikBZ, double* Dkmn);
 
+/* elph_q_rotate (src/symmetries/elphq_expand.c:16-73) on the device, same
+ * arguments and result as lelph_elph_q_rotate: g(Sk,Sq) (nmodes, nspin, nb,
+ * nb) from g(k,q) and the D-matrices of the two sides. */
+int lelph_elph_q_rotate_gpu(lelph_ctx* ctx, const double* Dmats_l, const double* elph_mat_q,
+                            const double* Dmats_r, int nmodes, int nspin, int nb, int tim_rev,
+                            double* elph_mat_Sq);
+
 /* get_KplusQ_idxs (src/common/numerical_func.c:471-524). */
 int lelph_get_kplusq_idxs(int64_t nbz, const double* kpoints_crys,
                           const double qpt_crys[3], int* kplusq_idxs);

@@ -155,3 +155,55 @@ __global__ void k_dm_rot2(const z64* __restrict__ w2, int64_t nsets, int nspinor
     o[p] = u;
     if (nspinor == 2) o[npw1 + p] = d;
 }
+
+// Batched complex FP64 GEMM for the q-star rotation (elph_q_rotate,
+// src/symmetries/elphq_expand.c:16-73): C[b] = opA(A[b]) opB(B[b]) with
+// opA in {A, conj(A)}, opB in {B, conj(B), B^T}; square n x n matrices, batch
+// strides sa/sb/sc (elements; 0 = the same matrix for every batch entry),
+// 32 x 32 tiles through LDS.
+template <bool CONJA, bool CONJB, bool TRANSB>
+__global__ __launch_bounds__(256) void k_zgemm_bat(const z64* __restrict__ A, int64_t sa,
+                                                   const z64* __restrict__ B, int64_t sb,
+                                                   z64* __restrict__ Cm, int64_t sc, int n)
+{
+    __shared__ z64 ta[32][33];
+    __shared__ z64 tb[32][33];
+    const int tiles = (n + 31) / 32;
+    const int bi = blockIdx.x / tiles, bj = blockIdx.x % tiles;
+    const int64_t bt = blockIdx.y;
+    const z64* a = A + bt * sa;
+    const z64* b = B + bt * sb;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 32 x 8
+    z64 acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int k0 = 0; k0 < n; k0 += 32)
+    {
+        for (int r = ty; r < 32; r += 8)
+        {
+            const int i = bi * 32 + r, k = k0 + tx;
+            z64 v = (i < n && k < n) ? a[(int64_t)i * n + k] : z64{0, 0};
+            if (CONJA) v.y = -v.y;
+            ta[r][tx] = v;
+            const int kk = k0 + r, j = bj * 32 + tx;
+            z64 w = {0, 0};
+            if (kk < n && j < n) w = TRANSB ? b[(int64_t)j * n + kk] : b[(int64_t)kk * n + j];
+            if (CONJB) w.y = -w.y;
+            tb[r][tx] = w;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < 32; ++k)
+        {
+            const z64 bv = tb[k][tx];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = cadd(acc[q], cmul(ta[ty + 8 * q][k], bv));
+        }
+        __syncthreads();
+    }
+    z64* c = Cm + bt * sc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+    {
+        const int i = bi * 32 + ty + 8 * q, j = bj * 32 + tx;
+        if (i < n && j < n) c[(int64_t)i * n + j] = acc[q];
+    }
+}

@@ -135,6 +135,8 @@ struct lelph_ctx
     DBuf pl_cnt[2], pl_off[2], pl_cur[2], pl_perm[2];
     // electronic representation matrices
     DBuf dm_map, dm_ph, dm_pos, dm_wRS, dm_wS2, dm_part, dm_out;
+    // q-star rotation
+    DBuf qr_dl, qr_dr, qr_g, qr_t, qr_out;
     DBuf nlK, nlKp, Y_K, Y_Kp, wr_K, wr_Kp, beta_K, beta_Kp, bb_K, bb_Kp, bbpart, ebuf;
 
     // per-pair events: [0] start, [1] boxes built, [2] contraction done,
@@ -1399,6 +1401,54 @@ int lelph_electronic_reps(lelph_ctx* c, const double Rsym[9], const double tauR[
     HIP_TRY(hipMemcpyAsync(Dkmn, c->dm_out.p, sizeof(z64) * L.nspin * nb * nb,
                            hipMemcpyDeviceToHost, c->stream));
     return check_err(c);
+}
+
+// elph_q_rotate (src/symmetries/elphq_expand.c:16-73) on the device:
+// gSq[m][s] = (D_l[s] op(g[m][s]) D_r[s]^H)^T = conj(D_r[s]) op'(g) D_l[s]^T,
+// op = transpose (conjugate transpose under time reversal), so op' = 1 (conj)
+int lelph_elph_q_rotate_gpu(lelph_ctx* c, const double* Dmats_l, const double* elph_mat_q,
+                            const double* Dmats_r, int nmodes, int nspin, int nb, int tim_rev,
+                            double* elph_mat_Sq)
+{
+    if (!c || !Dmats_l || !elph_mat_q || !Dmats_r || !elph_mat_Sq) return LELPH_ERR_ARG;
+    if (nmodes <= 0 || nspin <= 0 || nb <= 0 || nspin > 2) return LELPH_ERR_ARG;
+    const int64_t mat = (int64_t)nb * nb;
+    const int64_t gsz = (int64_t)nmodes * nspin * mat;
+    LELPH_TRY(c->qr_dl.ensure(sizeof(z64) * nspin * mat));
+    LELPH_TRY(c->qr_dr.ensure(sizeof(z64) * nspin * mat));
+    LELPH_TRY(c->qr_g.ensure(sizeof(z64) * gsz));
+    LELPH_TRY(c->qr_t.ensure(sizeof(z64) * gsz));
+    LELPH_TRY(c->qr_out.ensure(sizeof(z64) * gsz));
+    HIP_TRY(hipMemcpyAsync(c->qr_dl.p, Dmats_l, sizeof(z64) * nspin * mat, hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->qr_dr.p, Dmats_r, sizeof(z64) * nspin * mat, hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->qr_g.p, elph_mat_q, sizeof(z64) * gsz, hipMemcpyHostToDevice,
+                           c->stream));
+    const int tiles = (nb + 31) / 32;
+    for (int is = 0; is < nspin; ++is)
+    {
+        // batch over modes: matrix (m, is) sits at (m * nspin + is) * mat
+        const z64* dr = c->qr_dr.as<z64>() + is * mat;
+        const z64* dl = c->qr_dl.as<z64>() + is * mat;
+        const z64* g = c->qr_g.as<z64>() + is * mat;
+        z64* t = c->qr_t.as<z64>() + is * mat;
+        z64* o = c->qr_out.as<z64>() + is * mat;
+        const dim3 grid((unsigned)(tiles * tiles), (unsigned)nmodes);
+        if (tim_rev)
+            hipLaunchKernelGGL((k_zgemm_bat<true, true, false>), grid, dim3(256), 0, c->stream, dr,
+                               (int64_t)0, g, nspin * mat, t, nspin * mat, nb);
+        else
+            hipLaunchKernelGGL((k_zgemm_bat<true, false, false>), grid, dim3(256), 0, c->stream,
+                               dr, (int64_t)0, g, nspin * mat, t, nspin * mat, nb);
+        hipLaunchKernelGGL((k_zgemm_bat<false, false, true>), grid, dim3(256), 0, c->stream, t,
+                           nspin * mat, dl, (int64_t)0, o, nspin * mat, nb);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(elph_mat_Sq, c->qr_out.p, sizeof(z64) * gsz, hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LELPH_OK;
 }
 
 int lelph_internal_grid(const lelph_ctx* c, int64_t* nbz, const double** kcrys, const int** kmap)

@@ -31,6 +31,7 @@ EXPORTS = [
     "lelph_dvlocq", "lelph_set_dvscf", "lelph_get_dvscf", "lelph_elph_local",
     "lelph_add_elph_nonlocal", "lelph_gkq", "lelph_gkq_batch", "lelph_compute_elphq",
     "lelph_elph_q_rotate", "lelph_get_kplusq_idxs", "lelph_last_timings",
+    "lelph_electronic_reps", "lelph_elph_q_rotate_gpu",
 ]
 
 
@@ -82,6 +83,8 @@ def lib():
         L.lelph_last_timings.argtypes = [C.c_void_p, c_double_p, C.c_int]
         L.lelph_electronic_reps.argtypes = [C.c_void_p, c_double_p, c_double_p, C.c_int,
                                             C.c_int64, c_double_p]
+        L.lelph_elph_q_rotate_gpu.argtypes = [C.c_void_p, c_double_p, c_double_p, c_double_p,
+                                              C.c_int, C.c_int, C.c_int, C.c_int, c_double_p]
         _lib = L
     return _lib
 
@@ -229,6 +232,19 @@ class Context:
         out = np.zeros((s.nspin, s.nbnds, s.nbnds), np.complex128)
         _check(lib().lelph_electronic_reps(self.h, dptr(R), dptr(t), int(bool(tim_rev)),
                                            int(ikbz), dptr(out)), "electronic_reps")
+        return out
+
+    def elph_q_rotate(self, Dl, g, Dr, tim_rev: bool) -> np.ndarray:
+        """q-star rotation on the device (elph_q_rotate,
+        src/symmetries/elphq_expand.c:16-73); g (nmodes, nspin, nb, nb)."""
+        g = np.ascontiguousarray(g, np.complex128)
+        Dl = np.ascontiguousarray(Dl, np.complex128)
+        Dr = np.ascontiguousarray(Dr, np.complex128)
+        nm, ns, nb, _ = g.shape
+        out = np.zeros_like(g)
+        _check(lib().lelph_elph_q_rotate_gpu(self.h, dptr(Dl), dptr(g), dptr(Dr), nm, ns, nb,
+                                             int(bool(tim_rev)), dptr(out)),
+               "elph_q_rotate_gpu")
         return out
 
     def last_timings(self) -> dict:

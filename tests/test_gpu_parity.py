@@ -240,3 +240,26 @@ def test_electronic_reps(kind):
                 assert rel_err(got, ref) < TOL64, (kind, ik)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("nb", [5, 40, 70])
+def test_elph_q_rotate_gpu(nb):
+    """q-star rotation on the device against the oracle's elph_q_rotate
+    (both time-reversal branches; nb below, at and across the 32-wide tile)."""
+    from letzelphc_amd.abi import dptr
+    rng = np.random.default_rng(nb)
+    nm, ns = 6, 2
+    g = rng.standard_normal((nm, ns, nb, nb)) + 1j * rng.standard_normal((nm, ns, nb, nb))
+    Dl = rng.standard_normal((ns, nb, nb)) + 1j * rng.standard_normal((ns, nb, nb))
+    Dr = rng.standard_normal((ns, nb, nb)) + 1j * rng.standard_normal((ns, nb, nb))
+    s = synth.tiny(nbnds=4)
+    ctx = lelph.Context(s, "fp64")
+    say(f"case: elph_q_rotate_gpu nb={nb}")
+    try:
+        for tr in (0, 1):
+            got = ctx.elph_q_rotate(Dl, g, Dr, bool(tr))
+            ref = np.zeros_like(g)
+            O.lib().orc_elph_q_rotate(dptr(Dl), dptr(g), dptr(Dr), nm, ns, nb, tr, dptr(ref))
+            assert rel_err(got, ref) < TOL64, (nb, tr)
+    finally:
+        ctx.close()
