@@ -230,6 +230,95 @@ int lelph_elph_q_rotate_gpu(lelph_ctx* ctx, const double* Dmats_l, const double*
 int lelph_get_kplusq_idxs(int64_t nbz, const double* kpoints_crys,
                           const double qpt_crys[3], int* kplusq_idxs);
 
+/* ------------------------------------------------------------------------
+ * Fourier interpolation of dV_q onto arbitrary q
+ * (src/interpolation/interpolation_driver.c:190-420 coarse side,
+ * :790-835 fine side).
+ * ---------------------------------------------------------------------- */
+
+/* Parameters of dV_add_longrange (src/dvloc/dVpot_add_long_range.c:18-241)
+ * and dVlong_range_kernel (src/dvloc/dVlong_range_kernel.c:31-458).  A NULL
+ * array switches its term off, as in the reference. */
+typedef struct lelph_longrange
+{
+    const double* epsilon; /* (3,3) dielectric tensor, or NULL             */
+    const double* Zborn;   /* (natom,3,3) Born charges, or NULL            */
+    const double* Qpole;   /* (natom,3,3,3) quadrupoles, or NULL           */
+    const double* Zvals;   /* (natom) valence charges (monopole), or NULL  */
+    int only_induced;      /* 1: drop the monopole term                    */
+    double eta_bare;       /* Ewald parameter of the bare (monopole) term  */
+    double eta_induced;    /* Ewald parameter of the induced terms         */
+    double EcutRy;         /* G box cut-off of the long-range sum, Ry      */
+    int nmags_add[4];      /* add/subtract into dV[:, imag] if nonzero      */
+} lelph_longrange;
+
+/* dV_add_longrange (src/dvloc/dVpot_add_long_range.c:18-241) on the device:
+ * dvscf (nmodes, nmag, Nx, Ny, Nz) += sign * long-range potential at qpt
+ * (crystal), in the mode basis of eigVec (nmodes, 3*natom), or in the
+ * cartesian basis (mode = 3*atom + x) when eigVec is NULL. */
+int lelph_dv_longrange(lelph_ctx* ctx, const double qpt[3], const lelph_longrange* lr,
+                       const double* eigVec, int sign, double* dvscf);
+
+/* rotate_dvscf (src/symmetries/rotate_dvscf.c:22-368) on the device:
+ * dV_Sq(r) = e^{-i Sq.tau} dV_q(S^-1 (r - tau)), SU(2) sandwich for nmag 4,
+ * conjugate for time reversal.  composite_form as in the reference. */
+int lelph_rotate_dvscf(lelph_ctx* ctx, const double qpt[3], const double* dvscf_in,
+                       const lelph_symmetry* sym, int composite_form, double* dvscf_out);
+
+/* build_wigner_seitz_vectors (src/interpolation/wigner_seitz.c:27-202), host
+ * code: the superlattice vectors T that minimise |r_m - r_n - R - T| for
+ * every R of the grid (x slowest, miller order of get_miller_idx) and every
+ * (m, n) pair.  rvec_m / rvec_n NULL means one vector at the origin.
+ * ws_degen (nR, nrvec_m, nrvec_n) gets the degeneracies; ws_vecs (cap, 3)
+ * the vectors in crystal units of the unit cell.  *nws = sum of ws_degen;
+ * LELPH_ERR_RANGE if that exceeds cap. */
+int lelph_wigner_seitz(const int64_t grid[3], const double lat_vecs[9], double eps,
+                       const double* rvec_m, int64_t nrvec_m, const double* rvec_n,
+                       int64_t nrvec_n, int64_t cap, int64_t* ws_vecs, int64_t* ws_degen,
+                       int64_t* nws);
+
+/* Starts an interpolation on the context's lattice: coarse Gamma-centred
+ * q grid qgrid, atomic masses (natom, or NULL: no mass normalisation, as
+ * mass_normalize_pol_vecs with NULL), long-range parameters (NULL: none).
+ * Device memory: nq * nmodes * nmag * N complex doubles. */
+int lelph_interp_begin(lelph_ctx* ctx, const int64_t qgrid[3], const double* masses,
+                       const lelph_longrange* lr);
+
+/* One coarse q (crystal; any point of the full coarse grid, each exactly
+ * once): dvscf (nmodes, nmag, N) in the mode basis of eigVec, as
+ * get_dvscf_dyn_qe returns it.  Subtracts the long-range part, changes to the
+ * cartesian basis (dVscf_change_basis 'C' with the mass-unnormalised
+ * eigenvectors) and multiplies by e^{iqr} (interpolation_driver.c:316-320,
+ * 380-412).  The long-range-free mode-basis potential stays on the device
+ * for lelph_interp_add_star. */
+int lelph_interp_add_coarse(lelph_ctx* ctx, const double qpt[3], const double* eigVec,
+                            const double* dvscf);
+
+/* A star member S q of the last coarse q added: rotate_dvscf of its
+ * long-range-free potential with sym, then the same basis change with
+ * eig_Sq (rotate_eig_vecs of the caller) and e^{iSq.r}
+ * (interpolation_driver.c:329-363, 380-412).  qpt_Sq is S q in crystal
+ * units. */
+int lelph_interp_add_star(lelph_ctx* ctx, const lelph_symmetry* sym, const double qpt_Sq[3],
+                          const double* eig_Sq);
+
+/* q -> R: fft_q2R (src/interpolation/interpolation_utilities.c:25-76) over
+ * the coarse grid once every coarse point is in. */
+int lelph_interp_finish(lelph_ctx* ctx);
+
+/* R -> q for nq fine points (crystal): fft_R2q_dvscf
+ * (interpolation_utilities.c:165-238) with the Wigner-Seitz weights,
+ * e^{-iqr}, long range added back in the cartesian basis
+ * (interpolation_driver.c:822-835).  dvscf_out (nq, nmodes, nmag, N),
+ * cartesian basis (mode = 3*atom + x), as write_dvscf_qe writes it. */
+int lelph_interp_dvscf(lelph_ctx* ctx, int64_t nq, const double* qpts, double* dvscf_out);
+
+/* The interpolated potential at qpt made the context's current dV_q
+ * without leaving the device: lelph_interp_dvscf, then lelph_set_dvscf's
+ * mode rotation with eigVec (identity patterns) and the bare local part. */
+int lelph_interp_set_dvscf(lelph_ctx* ctx, const double qpt[3], const double* eigVec,
+                           int kernel);
+
 /* Device time of the last elph_local / nonlocal / gkq / batch call, summed
  * over its pairs (HIP events on the context stream), milliseconds:
  *   [0] sphere scatter + backward FFTs of psi_k and psi_k+q

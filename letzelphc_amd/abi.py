@@ -64,6 +64,55 @@ class Pair(C.Structure):
                 ("ksym", C.c_int)]
 
 
+class LongRange(C.Structure):
+    """lelph_longrange: parameters of dV_add_longrange
+    (src/dvloc/dVpot_add_long_range.c:18-241)."""
+    _fields_ = [("epsilon", c_double_p), ("Zborn", c_double_p), ("Qpole", c_double_p),
+                ("Zvals", c_double_p), ("only_induced", C.c_int), ("eta_bare", C.c_double),
+                ("eta_induced", C.c_double), ("EcutRy", C.c_double), ("nmags_add", C.c_int * 4)]
+
+
+@dataclass
+class LrParams:
+    """Host-side long-range parameters; `pack()` returns a LongRange whose
+    pointers stay valid while this object lives."""
+    epsilon: np.ndarray | None = None
+    Zborn: np.ndarray | None = None
+    Qpole: np.ndarray | None = None
+    Zvals: np.ndarray | None = None
+    only_induced: bool = False
+    eta_bare: float = 1.0
+    eta_induced: float = 1.0
+    EcutRy: float = 30.0
+    nmags_add: tuple = (1, 0, 0, 0)
+    _keep: list = field(default_factory=list)
+
+    def pack(self) -> LongRange:
+        def p(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float64)
+            self._keep.append(a)
+            return dptr(a)
+        lr = LongRange()
+        lr.epsilon = p(self.epsilon)
+        lr.Zborn = p(self.Zborn)
+        lr.Qpole = p(self.Qpole)
+        lr.Zvals = p(self.Zvals)
+        lr.only_induced = int(bool(self.only_induced))
+        lr.eta_bare = float(self.eta_bare)
+        lr.eta_induced = float(self.eta_induced)
+        lr.EcutRy = float(self.EcutRy)
+        for i in range(4):
+            lr.nmags_add[i] = int(self.nmags_add[i]) if i < len(self.nmags_add) else 0
+        return lr
+
+
+def i64ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"] and a.dtype == np.int64
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
 def dptr(a: np.ndarray):
     """double* to a C-contiguous float64 / complex128 array."""
     assert a.flags["C_CONTIGUOUS"], "array must be C contiguous"

@@ -22,6 +22,7 @@
 #include "splitpc.hpp"
 #include "boxes.hpp"
 #include "dmats.hpp"
+#include "interp.hpp"
 
 #include <type_traits>
 #include "wfc.hpp"
@@ -137,6 +138,17 @@ struct lelph_ctx
     DBuf dm_map, dm_ph, dm_pos, dm_wRS, dm_wS2, dm_part, dm_out;
     // q-star rotation
     DBuf qr_dl, qr_dr, qr_g, qr_t, qr_out;
+    // Fourier interpolation of dV_q
+    bool ip_on = false, ip_ready = false, ip_last_ok = false, ip_has_lr = false;
+    int64_t ip_grid[3] = {0, 0, 0};
+    int64_t ip_nq = 0;
+    std::vector<double> ip_mass;
+    std::vector<char> ip_have;
+    double ip_last_q[3] = {0, 0, 0};
+    lelph_longrange ip_lr{};
+    std::vector<double> ip_eps, ip_zb, ip_qp, ip_zv;
+    DBuf ip_dvq, ip_dvR, ip_last, ip_tmp, ip_out, ip_ws, ip_off, ip_W, ip_P, ip_qs, ip_rot,
+        ip_eig, lr_box, lr_box2, lr_q, lr_zv, lr_zb, lr_qp, rot_idx, rot_sr;
     DBuf nlK, nlKp, Y_K, Y_Kp, wr_K, wr_Kp, beta_K, beta_Kp, bb_K, bb_Kp, bbpart, ebuf;
 
     // per-pair events: [0] start, [1] boxes built, [2] contraction done,
@@ -652,16 +664,24 @@ static int run_local_any(lelph_ctx* c, const double* qpt, const lelph_pair& pr, 
     }
 }
 
+// reads and clears the device error word (bit 0: G outside the box or table
+// range, bit 1: symmetry incompatible with the FFT grid, bit 2: rotated
+// index out of range)
+static int read_err(lelph_ctx* c, int* e)
+{
+    *e = 0;
+    HIP_TRY(hipMemcpyAsync(e, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (*e) HIP_TRY(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+    return LELPH_OK;
+}
+
 static int check_err(lelph_ctx* c)
 {
     int e = 0;
-    HIP_TRY(hipMemcpyAsync(&e, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (e)
-    {
-        HIP_TRY(hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
-        return LELPH_ERR_RANGE;
-    }
+    LELPH_TRY(read_err(c, &e));
+    if (e & 2) return LELPH_ERR_SYM;
+    if (e) return LELPH_ERR_RANGE;
     return LELPH_OK;
 }
 
@@ -714,7 +734,14 @@ static void free_ctx(lelph_ctx* c)
     DBuf* bufs[] = {&c->dv64, &c->dv32, &c->eig_d, &c->vloc_tmp, &c->rot_tmp, &c->idx_k,
                     &c->idx_kq, &c->ph_k, &c->ph_kq, &c->box_k, &c->box_kq, &c->part, &c->g,
                     &c->gbatch, &c->presplit, &c->nlK, &c->nlKp, &c->Y_K, &c->Y_Kp, &c->wr_K, &c->wr_Kp,
-                    &c->beta_K, &c->beta_Kp, &c->bb_K, &c->bb_Kp, &c->bbpart, &c->ebuf};
+                    &c->beta_K, &c->beta_Kp, &c->bb_K, &c->bb_Kp, &c->bbpart, &c->ebuf,
+                    &c->pl_cnt[0], &c->pl_cnt[1], &c->pl_off[0], &c->pl_off[1], &c->pl_cur[0],
+                    &c->pl_cur[1], &c->pl_perm[0], &c->pl_perm[1], &c->dm_map, &c->dm_ph,
+                    &c->dm_pos, &c->dm_wRS, &c->dm_wS2, &c->dm_part, &c->dm_out, &c->qr_dl,
+                    &c->qr_dr, &c->qr_g, &c->qr_t, &c->qr_out, &c->ip_dvq, &c->ip_dvR,
+                    &c->ip_last, &c->ip_tmp, &c->ip_out, &c->ip_ws, &c->ip_off, &c->ip_W,
+                    &c->ip_P, &c->ip_qs, &c->ip_rot, &c->ip_eig, &c->lr_box, &c->lr_box2,
+                    &c->lr_q, &c->lr_zv, &c->lr_zb, &c->lr_qp, &c->rot_idx, &c->rot_sr};
     for (DBuf* b : bufs) b->release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1093,6 +1120,8 @@ int lelph_dvlocq(lelph_ctx* c, const double qpt[3], const double* eigVec, double
     return LELPH_OK;
 }
 
+static int finish_dv(lelph_ctx* c, const double* qpt);
+
 int lelph_set_dvscf(lelph_ctx* c, const double qpt[3], const double* eigVec, const double* dvscf,
                     const double* patterns, int kernel)
 {
@@ -1139,7 +1168,14 @@ int lelph_set_dvscf(lelph_ctx* c, const double qpt[3], const double* eigVec, con
     }
     else
         HIP_TRY(hipMemsetAsync(c->dv64.p, 0, bytes, c->stream));
-    // every kernel adds the bare local part (set_kernel.c: bare_loc = true)
+    return finish_dv(c, qpt);
+}
+
+// every kernel adds the bare local part (set_kernel.c: bare_loc = true)
+static int finish_dv(lelph_ctx* c, const double* qpt)
+{
+    const lelph_lattice& L = c->lat;
+    const int nm = L.nmodes;
     LELPH_TRY(dvlocq_dev(c, qpt));
     hipLaunchKernelGGL(k_add_dvscf, dim3((unsigned)cdiv((int64_t)nm * c->nf, 256)), dim3(256), 0,
                        c->stream, c->dv64.as<z64>(), c->vloc_tmp.as<z64>(), nm, L.nmag, c->nf);
@@ -1449,6 +1485,457 @@ int lelph_elph_q_rotate_gpu(lelph_ctx* c, const double* Dmats_l, const double* e
                            c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return LELPH_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Fourier interpolation of dV_q                                             */
+/* ------------------------------------------------------------------------ */
+
+// get_fft_box (src/wfc/gen_fftbox.c:8-90) for one rank
+static void lr_fft_box(double EcutRy, const double* blat, int64_t* box)
+{
+    for (int i = 0; i < 3; ++i)
+    {
+        const double bn =
+            std::sqrt(blat[i] * blat[i] + blat[3 + i] * blat[3 + i] + blat[6 + i] * blat[6 + i]);
+        box[i] = 2 * (int64_t)std::rint(std::sqrt(EcutRy) / bn + 1) + 1;
+    }
+}
+
+// copies the long-range parameters (arrays owned by the context)
+static void lr_copy(lelph_ctx* c, const lelph_longrange* lr)
+{
+    const int na = c->lat.natom;
+    c->ip_lr = *lr;
+    auto cp = [](std::vector<double>& v, const double* p, size_t n) {
+        v.assign(p ? p : nullptr, p ? p + n : nullptr);
+    };
+    cp(c->ip_eps, lr->epsilon, 9);
+    cp(c->ip_zb, lr->Zborn, (size_t)9 * na);
+    cp(c->ip_qp, lr->Qpole, (size_t)27 * na);
+    cp(c->ip_zv, lr->Zvals, (size_t)na);
+    c->ip_lr.epsilon = lr->epsilon ? c->ip_eps.data() : nullptr;
+    c->ip_lr.Zborn = lr->Zborn ? c->ip_zb.data() : nullptr;
+    c->ip_lr.Qpole = lr->Qpole ? c->ip_qp.data() : nullptr;
+    c->ip_lr.Zvals = lr->Zvals ? c->ip_zv.data() : nullptr;
+}
+
+// dV_add_longrange (dVpot_add_long_range.c:18-241) for nq points at once:
+// dv (nq, nm, nmag, nf) on the device += sgn * long range; eig (device,
+// nm x nm) rotates to the mode basis and needs nq == 1.
+static int lr_apply(lelph_ctx* c, const lelph_longrange& lr, const double* qs, int64_t nq,
+                    const z64* eig, z64* dv, double sgn)
+{
+    const lelph_lattice& L = c->lat;
+    const int nm = L.nmodes, na = L.natom;
+    if (eig && nq != 1) return LELPH_ERR_ARG;
+    if (L.dimension != '3' && L.dimension != '2') return LELPH_OK; // no kernel in the reference
+    if (L.dimension == '2')
+        for (int64_t i = 0; i < nq; ++i)
+            if (std::fabs(qs[3 * i + 2]) > LELPH_EPS) return LELPH_ERR_ARG;
+    if (lr.Zborn && !lr.epsilon) return LELPH_ERR_ARG;
+    const double* zv = lr.only_induced ? nullptr : lr.Zvals;
+    if (!lr.epsilon && !zv) return LELPH_OK; // the kernel returns zeros
+    int mask = 0;
+    for (int m = 0; m < L.nmag && m < 4; ++m)
+        if (lr.nmags_add[m]) mask |= 1 << m;
+    if (!mask) return LELPH_OK;
+    LrGeom g{};
+    std::memcpy(g.blat, L.blat_vec, sizeof(g.blat));
+    if (lr.epsilon) std::memcpy(g.eps, lr.epsilon, sizeof(g.eps));
+    g.latz = L.alat_vec[8];
+    g.volume = L.volume;
+    g.EcutRy = lr.EcutRy;
+    g.eta_bare = lr.eta_bare;
+    g.eta_ind = lr.eta_induced;
+    g.has_eps = lr.epsilon != nullptr;
+    g.has_zv = zv != nullptr;
+    g.has_zb = lr.Zborn != nullptr;
+    g.has_qp = lr.Qpole != nullptr;
+    g.natom = na;
+    g.dim = L.dimension;
+    int64_t gb[3];
+    lr_fft_box(lr.EcutRy, L.blat_vec, gb);
+    for (int i = 0; i < 3; ++i)
+    {
+        g.N[i] = (int)L.fft_dims[i];
+        if (gb[i] > L.fft_dims[i]) gb[i] = L.fft_dims[i];
+    }
+    if (L.dimension == '2') gb[2] = L.fft_dims[2];
+    for (int i = 0; i < 3; ++i)
+    {
+        g.lo[i] = -(int)(gb[i] / 2);
+        g.hi[i] = (int)((gb[i] - 1) / 2);
+    }
+    if (zv) LELPH_TRY(c->lr_zv.ensure(sizeof(double) * na));
+    if (lr.Zborn) LELPH_TRY(c->lr_zb.ensure(sizeof(double) * 9 * na));
+    if (lr.Qpole) LELPH_TRY(c->lr_qp.ensure(sizeof(double) * 27 * na));
+    if (zv)
+        HIP_TRY(hipMemcpyAsync(c->lr_zv.p, zv, sizeof(double) * na, hipMemcpyHostToDevice,
+                               c->stream));
+    if (lr.Zborn)
+        HIP_TRY(hipMemcpyAsync(c->lr_zb.p, lr.Zborn, sizeof(double) * 9 * na,
+                               hipMemcpyHostToDevice, c->stream));
+    if (lr.Qpole)
+        HIP_TRY(hipMemcpyAsync(c->lr_qp.p, lr.Qpole, sizeof(double) * 27 * na,
+                               hipMemcpyHostToDevice, c->stream));
+    const int64_t nf = c->nf;
+    // q chunks keep the coefficient boxes within 2 GiB
+    const int64_t per_q = (int64_t)nm * nf * (int64_t)sizeof(z64);
+    int64_t chunk = std::max<int64_t>(1, std::min<int64_t>((int64_t)2 << 30, per_q * nq) / per_q);
+    chunk = std::min<int64_t>(chunk, std::max<int64_t>(1, 65535 / nm));
+    LELPH_TRY(c->lr_q.ensure(sizeof(double) * 3 * nq));
+    HIP_TRY(hipMemcpyAsync(c->lr_q.p, qs, sizeof(double) * 3 * nq, hipMemcpyHostToDevice,
+                           c->stream));
+    LELPH_TRY(c->lr_box.ensure((size_t)per_q * chunk));
+    if (eig) LELPH_TRY(c->lr_box2.ensure((size_t)per_q));
+    for (int64_t q0 = 0; q0 < nq; q0 += chunk)
+    {
+        const int64_t nqc = std::min(chunk, nq - q0);
+        hipLaunchKernelGGL(k_lr_box, dim3((unsigned)cdiv(nf, 256), (unsigned)nqc), dim3(256), 0,
+                           c->stream, g, c->lr_q.as<double>() + 3 * q0, c->d_apos,
+                           c->lr_zv.as<double>(), c->lr_zb.as<double>(), c->lr_qp.as<double>(),
+                           c->lr_box.as<z64>());
+        HIP_TRY(hipGetLastError());
+        z64* box = c->lr_box.as<z64>();
+        if (eig)
+        {
+            hipLaunchKernelGGL(k_mode_rotate, dim3((unsigned)cdiv((int64_t)nm * nf, 256)),
+                               dim3(256), 0, c->stream, box, eig, nm, nf, c->lr_box2.as<z64>());
+            HIP_TRY(hipGetLastError());
+            box = c->lr_box2.as<z64>();
+        }
+        LELPH_TRY(fft3d_batch<double>(c->ax, box, nqc * nm, +1, c->stream));
+        hipLaunchKernelGGL(k_lr_add, dim3((unsigned)cdiv(nf, 256), (unsigned)(nqc * nm)),
+                           dim3(256), 0, c->stream, box, dv + q0 * nm * L.nmag * nf, nm, L.nmag,
+                           nf, mask, sgn);
+        HIP_TRY(hipGetLastError());
+    }
+    return LELPH_OK;
+}
+
+int lelph_dv_longrange(lelph_ctx* c, const double qpt[3], const lelph_longrange* lr,
+                       const double* eigVec, int sign, double* dvscf)
+{
+    if (!c || !qpt || !lr || !dvscf) return LELPH_ERR_ARG;
+    const lelph_lattice& L = c->lat;
+    const int nm = L.nmodes;
+    const size_t bytes = sizeof(z64) * (size_t)nm * L.nmag * c->nf;
+    LELPH_TRY(c->ip_tmp.ensure(bytes));
+    HIP_TRY(hipMemcpyAsync(c->ip_tmp.p, dvscf, bytes, hipMemcpyHostToDevice, c->stream));
+    const z64* eig = nullptr;
+    if (eigVec)
+    {
+        LELPH_TRY(c->ip_eig.ensure(sizeof(z64) * nm * nm));
+        HIP_TRY(hipMemcpyAsync(c->ip_eig.p, eigVec, sizeof(z64) * nm * nm, hipMemcpyHostToDevice,
+                               c->stream));
+        eig = c->ip_eig.as<z64>();
+    }
+    LELPH_TRY(lr_apply(c, *lr, qpt, 1, eig, c->ip_tmp.as<z64>(), sign < 0 ? -1.0 : 1.0));
+    HIP_TRY(hipMemcpyAsync(dvscf, c->ip_tmp.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return check_err(c);
+}
+
+// rotate_dvscf on the device: out (nm, nmag, nf) from in
+static int rotate_dev(lelph_ctx* c, const double* qpt, const lelph_symmetry* sym, int composite,
+                      const z64* in, z64* out)
+{
+    const lelph_lattice& L = c->lat;
+    const double* blat = L.blat_vec;
+    const double tp = 2.0 * LELPH_RPI;
+    double tr[3], tmp[9], sr[9];
+    for (int i = 0; i < 3; ++i)
+    {
+        tr[i] = (blat[i] * sym->tau[0] + blat[3 + i] * sym->tau[1] + blat[6 + i] * sym->tau[2]) / tp;
+        if (sym->time_rev) tr[i] = -tr[i];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            double a = 0;
+            for (int k = 0; k < 3; ++k) a += sym->Rmat[k * 3 + i] * L.alat_vec[k * 3 + j];
+            tmp[i * 3 + j] = a / tp * (sym->time_rev ? -1.0 : 1.0);
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            double a = 0;
+            for (int k = 0; k < 3; ++k) a += blat[k * 3 + i] * tmp[k * 3 + j];
+            sr[i * 3 + j] = a;
+        }
+    const int64_t nf = c->nf;
+    LELPH_TRY(c->rot_sr.ensure(sizeof(sr)));
+    LELPH_TRY(c->rot_idx.ensure(sizeof(int64_t) * nf));
+    HIP_TRY(hipMemcpyAsync(c->rot_sr.p, sr, sizeof(sr), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_rot_index, dim3((unsigned)cdiv(nf, 256)), dim3(256), 0, c->stream,
+                       c->rot_sr.as<double>(), tr[0], tr[1], tr[2], (int)L.fft_dims[0],
+                       (int)L.fft_dims[1], (int)L.fft_dims[2], c->rot_idx.as<int64_t>(),
+                       c->d_err);
+    HIP_TRY(hipGetLastError());
+    int e = 0;
+    LELPH_TRY(read_err(c, &e));
+    if (e & 2) return LELPH_ERR_SYM;
+    if (e) return LELPH_ERR_RANGE;
+    double qc[3], Rq[3];
+    matvec3(blat, qpt, false, qc);
+    matvec3(sym->Rmat, qc, false, Rq);
+    const double a = -(Rq[0] * sym->tau[0] + Rq[1] * sym->tau[1] + Rq[2] * sym->tau[2]);
+    const z64 ph = {std::cos(a), std::sin(a)};
+    z64 su[4] = {{1, 0}, {0, 0}, {0, 0}, {1, 0}};
+    if (L.nmag == 4) LELPH_TRY(su2mat(sym->Rmat, L.nspinor, sym->time_rev, su));
+    hipLaunchKernelGGL(k_rot_gather, dim3((unsigned)cdiv((int64_t)L.nmodes * nf, 256)), dim3(256),
+                       0, c->stream, in, c->rot_idx.as<int64_t>(), L.nmodes, L.nmag, nf, ph,
+                       su[0], su[1], su[2], su[3], composite, sym->time_rev ? 1 : 0, out);
+    HIP_TRY(hipGetLastError());
+    return LELPH_OK;
+}
+
+int lelph_rotate_dvscf(lelph_ctx* c, const double qpt[3], const double* dvscf_in,
+                       const lelph_symmetry* sym, int composite_form, double* dvscf_out)
+{
+    if (!c || !qpt || !dvscf_in || !sym || !dvscf_out) return LELPH_ERR_ARG;
+    const lelph_lattice& L = c->lat;
+    const size_t bytes = sizeof(z64) * (size_t)L.nmodes * L.nmag * c->nf;
+    LELPH_TRY(c->ip_tmp.ensure(bytes));
+    LELPH_TRY(c->rot_tmp.ensure(bytes));
+    HIP_TRY(hipMemcpyAsync(c->ip_tmp.p, dvscf_in, bytes, hipMemcpyHostToDevice, c->stream));
+    LELPH_TRY(rotate_dev(c, qpt, sym, composite_form, c->ip_tmp.as<z64>(), c->rot_tmp.as<z64>()));
+    HIP_TRY(hipMemcpyAsync(dvscf_out, c->rot_tmp.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LELPH_OK;
+}
+
+int lelph_interp_begin(lelph_ctx* c, const int64_t qgrid[3], const double* masses,
+                       const lelph_longrange* lr)
+{
+    if (!c || !qgrid) return LELPH_ERR_ARG;
+    for (int i = 0; i < 3; ++i)
+        if (qgrid[i] < 1 || qgrid[i] > 1024) return LELPH_ERR_ARG;
+    const lelph_lattice& L = c->lat;
+    const int na = L.natom;
+    c->ip_on = c->ip_ready = c->ip_last_ok = false;
+    std::memcpy(c->ip_grid, qgrid, sizeof(c->ip_grid));
+    c->ip_nq = qgrid[0] * qgrid[1] * qgrid[2];
+    c->ip_mass.assign(masses ? masses : nullptr, masses ? masses + na : nullptr);
+    c->ip_has_lr = lr != nullptr;
+    if (lr) lr_copy(c, lr);
+    // Wigner-Seitz vectors of every (R, atom) (interpolation_driver.c:212-216)
+    const int64_t nR = c->ip_nq;
+    std::vector<int64_t> degen((size_t)nR * na);
+    int64_t cap = 16 * nR * na, nws = 0;
+    std::vector<int64_t> ws;
+    for (;;)
+    {
+        ws.resize((size_t)3 * cap);
+        const int st = lelph_wigner_seitz(qgrid, L.alat_vec, LELPH_EPS6, c->atomic_pos.data(), na,
+                                          nullptr, 0, cap, ws.data(), degen.data(), &nws);
+        if (st == LELPH_OK) break;
+        if (st != LELPH_ERR_RANGE) return st;
+        cap *= 4;
+    }
+    std::vector<int> wsi((size_t)3 * std::max<int64_t>(nws, 1));
+    for (int64_t i = 0; i < 3 * nws; ++i) wsi[i] = (int)ws[i];
+    std::vector<int64_t> off((size_t)nR * na + 1, 0);
+    for (int64_t i = 0; i < nR * na; ++i) off[i + 1] = off[i] + degen[i];
+    LELPH_TRY(c->ip_ws.ensure(sizeof(int) * wsi.size()));
+    LELPH_TRY(c->ip_off.ensure(sizeof(int64_t) * off.size()));
+    HIP_TRY(hipMemcpy(c->ip_ws.p, wsi.data(), sizeof(int) * wsi.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->ip_off.p, off.data(), sizeof(int64_t) * off.size(),
+                      hipMemcpyHostToDevice));
+    const size_t per = sizeof(z64) * (size_t)L.nmodes * L.nmag * c->nf;
+    c->ip_dvR.release();
+    LELPH_TRY(c->ip_dvq.ensure(per * c->ip_nq));
+    c->ip_have.assign((size_t)c->ip_nq, 0);
+    c->ip_on = true;
+    return LELPH_OK;
+}
+
+// slot of a coarse q in the grid (the order of Sorted_qpts_idxs)
+static int ip_slot(const lelph_ctx* c, const double* q, int64_t* slot)
+{
+    int64_t id[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        const double v = (q[i] - std::floor(q[i])) * c->ip_grid[i];
+        const double r = std::rint(v);
+        if (std::fabs(v - r) > 1e-4) return LELPH_ERR_ARG;
+        id[i] = ((int64_t)r) % c->ip_grid[i];
+    }
+    *slot = (id[0] * c->ip_grid[1] + id[1]) * c->ip_grid[2] + id[2];
+    return LELPH_OK;
+}
+
+// basis change with eig * M (mass_normalize_pol_vecs power +1), e^{iqr},
+// into the slot of q
+static int ip_store(lelph_ctx* c, const double* q, const double* eig, const z64* in)
+{
+    const lelph_lattice& L = c->lat;
+    const int nm = L.nmodes;
+    int64_t slot;
+    LELPH_TRY(ip_slot(c, q, &slot));
+    if (c->ip_have[slot]) return LELPH_ERR_ARG;
+    std::vector<z64> rot((size_t)nm * nm);
+    const z64* e = reinterpret_cast<const z64*>(eig);
+    for (int v = 0; v < nm; ++v)
+        for (int j = 0; j < nm; ++j)
+        {
+            const double m = c->ip_mass.empty() ? 1.0 : c->ip_mass[j / 3];
+            rot[(size_t)v * nm + j] = cscale(e[(size_t)v * nm + j], m);
+        }
+    LELPH_TRY(c->ip_rot.ensure(sizeof(z64) * nm * nm));
+    HIP_TRY(hipMemcpyAsync(c->ip_rot.p, rot.data(), sizeof(z64) * nm * nm, hipMemcpyHostToDevice,
+                           c->stream));
+    const int64_t ld = (int64_t)L.nmag * c->nf;
+    hipLaunchKernelGGL(k_coarse_cart, dim3((unsigned)cdiv(ld, 256)), dim3(256), 0, c->stream, in,
+                       c->ip_rot.as<z64>(), nm, L.nmag, (int)L.fft_dims[0], (int)L.fft_dims[1],
+                       (int)L.fft_dims[2], q[0], q[1], q[2],
+                       c->ip_dvq.as<z64>() + slot * nm * ld);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->ip_have[slot] = 1;
+    return LELPH_OK;
+}
+
+int lelph_interp_add_coarse(lelph_ctx* c, const double qpt[3], const double* eigVec,
+                            const double* dvscf)
+{
+    if (!c || !qpt || !eigVec || !dvscf) return LELPH_ERR_ARG;
+    if (!c->ip_on || c->ip_ready) return LELPH_ERR_STATE;
+    const lelph_lattice& L = c->lat;
+    const int nm = L.nmodes;
+    int64_t slot;
+    LELPH_TRY(ip_slot(c, qpt, &slot));
+    if (c->ip_have[slot]) return LELPH_ERR_ARG;
+    const size_t bytes = sizeof(z64) * (size_t)nm * L.nmag * c->nf;
+    LELPH_TRY(c->ip_last.ensure(bytes));
+    LELPH_TRY(c->ip_eig.ensure(sizeof(z64) * nm * nm));
+    c->ip_last_ok = false;
+    HIP_TRY(hipMemcpyAsync(c->ip_last.p, dvscf, bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->ip_eig.p, eigVec, sizeof(z64) * nm * nm, hipMemcpyHostToDevice,
+                           c->stream));
+    if (c->ip_has_lr)
+        LELPH_TRY(lr_apply(c, c->ip_lr, qpt, 1, c->ip_eig.as<z64>(), c->ip_last.as<z64>(), -1.0));
+    LELPH_TRY(ip_store(c, qpt, eigVec, c->ip_last.as<z64>()));
+    LELPH_TRY(check_err(c));
+    std::memcpy(c->ip_last_q, qpt, sizeof(c->ip_last_q));
+    c->ip_last_ok = true;
+    return LELPH_OK;
+}
+
+int lelph_interp_add_star(lelph_ctx* c, const lelph_symmetry* sym, const double qpt_Sq[3],
+                          const double* eig_Sq)
+{
+    if (!c || !sym || !qpt_Sq || !eig_Sq) return LELPH_ERR_ARG;
+    if (!c->ip_on || c->ip_ready || !c->ip_last_ok) return LELPH_ERR_STATE;
+    const lelph_lattice& L = c->lat;
+    const size_t bytes = sizeof(z64) * (size_t)L.nmodes * L.nmag * c->nf;
+    LELPH_TRY(c->ip_tmp.ensure(bytes));
+    // QE stores [V, Bx, By, Bz] (interpolation_driver.c:264-271)
+    LELPH_TRY(rotate_dev(c, c->ip_last_q, sym, 0, c->ip_last.as<z64>(), c->ip_tmp.as<z64>()));
+    return ip_store(c, qpt_Sq, eig_Sq, c->ip_tmp.as<z64>());
+}
+
+int lelph_interp_finish(lelph_ctx* c)
+{
+    if (!c) return LELPH_ERR_ARG;
+    if (!c->ip_on || c->ip_ready) return LELPH_ERR_STATE;
+    for (char h : c->ip_have)
+        if (!h) return LELPH_ERR_STATE;
+    const lelph_lattice& L = c->lat;
+    const int64_t nq = c->ip_nq;
+    const int64_t cols = (int64_t)L.nmodes * L.nmag * c->nf;
+    LELPH_TRY(c->ip_P.ensure(sizeof(z64) * nq * nq));
+    LELPH_TRY(c->ip_dvR.ensure(sizeof(z64) * nq * cols));
+    hipLaunchKernelGGL(k_q2r_phases, dim3((unsigned)cdiv(nq * nq, 256)), dim3(256), 0, c->stream,
+                       (int)c->ip_grid[0], (int)c->ip_grid[1], (int)c->ip_grid[2],
+                       c->ip_P.as<z64>());
+    HIP_TRY(hipGetLastError());
+    constexpr int MB = 16;
+    hipLaunchKernelGGL((k_phase_gemm<MB>), dim3((unsigned)cdiv(cols, 256), (unsigned)cdiv(nq, MB)),
+                       dim3(256), 0, c->stream, c->ip_P.as<z64>(), (int)nq, (int)nq, 1,
+                       c->ip_dvq.as<z64>(), 1, cols, c->ip_dvR.as<z64>(), nullptr, 0, 0, 0);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->ip_dvq.release();
+    c->ip_ready = true;
+    return LELPH_OK;
+}
+
+// R -> q for nq points into out (device, (nq, nm, nmag, nf))
+static int ip_r2q(lelph_ctx* c, int64_t nq, const double* qs, z64* out)
+{
+    const lelph_lattice& L = c->lat;
+    const int na = L.natom;
+    const int64_t nR = c->ip_nq;
+    const int64_t cpa = (int64_t)3 * L.nmag * c->nf;
+    LELPH_TRY(c->ip_qs.ensure(sizeof(double) * 3 * nq));
+    LELPH_TRY(c->ip_W.ensure(sizeof(z64) * nq * nR * na));
+    HIP_TRY(hipMemcpyAsync(c->ip_qs.p, qs, sizeof(double) * 3 * nq, hipMemcpyHostToDevice,
+                           c->stream));
+    hipLaunchKernelGGL(k_ws_weights, dim3((unsigned)cdiv(nq * nR * na, 256)), dim3(256), 0,
+                       c->stream, c->ip_qs.as<double>(), (int)nq, (int)c->ip_grid[0],
+                       (int)c->ip_grid[1], (int)c->ip_grid[2], na, c->ip_off.as<int64_t>(),
+                       c->ip_ws.as<int>(), c->ip_W.as<z64>());
+    HIP_TRY(hipGetLastError());
+    constexpr int MB = 16;
+    hipLaunchKernelGGL((k_phase_gemm<MB>),
+                       dim3((unsigned)(na * cdiv(cpa, 256)), (unsigned)cdiv(nq, MB)), dim3(256), 0,
+                       c->stream, c->ip_W.as<z64>(), (int)nq, (int)nR, na, c->ip_dvR.as<z64>(), na,
+                       cpa, out, c->ip_qs.as<double>(), (int)L.fft_dims[0], (int)L.fft_dims[1],
+                       (int)L.fft_dims[2]);
+    HIP_TRY(hipGetLastError());
+    if (c->ip_has_lr) LELPH_TRY(lr_apply(c, c->ip_lr, qs, nq, nullptr, out, 1.0));
+    return LELPH_OK;
+}
+
+int lelph_interp_dvscf(lelph_ctx* c, int64_t nq, const double* qpts, double* dvscf_out)
+{
+    if (!c || nq < 0 || (nq && (!qpts || !dvscf_out))) return LELPH_ERR_ARG;
+    if (!c->ip_ready) return LELPH_ERR_STATE;
+    const lelph_lattice& L = c->lat;
+    const size_t per = sizeof(z64) * (size_t)L.nmodes * L.nmag * c->nf;
+    // chunks of at most 2 GiB of output and 65535 grid rows
+    int64_t chunk = std::max<int64_t>(1, ((int64_t)2 << 30) / (int64_t)per);
+    chunk = std::min<int64_t>(chunk, 65535);
+    LELPH_TRY(c->ip_out.ensure(per * std::min(chunk, std::max<int64_t>(nq, 1))));
+    for (int64_t q0 = 0; q0 < nq; q0 += chunk)
+    {
+        const int64_t n = std::min(chunk, nq - q0);
+        LELPH_TRY(ip_r2q(c, n, qpts + 3 * q0, c->ip_out.as<z64>()));
+        HIP_TRY(hipMemcpyAsync(reinterpret_cast<char*>(dvscf_out) + per * q0, c->ip_out.p,
+                               per * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return check_err(c);
+}
+
+int lelph_interp_set_dvscf(lelph_ctx* c, const double qpt[3], const double* eigVec, int kernel)
+{
+    if (!c || !qpt || !eigVec) return LELPH_ERR_ARG;
+    if (kernel < LELPH_KERNEL_DFPT || kernel > LELPH_KERNEL_BARE_LOCAL) return LELPH_ERR_ARG;
+    if (!c->ip_ready) return LELPH_ERR_STATE;
+    const lelph_lattice& L = c->lat;
+    const int nm = L.nmodes;
+    const int64_t ld = (int64_t)L.nmag * c->nf;
+    const size_t bytes = sizeof(z64) * (size_t)nm * ld;
+    c->dv_set = false;
+    c->dv32_valid = false;
+    LELPH_TRY(set_eig(c, eigVec));
+    LELPH_TRY(c->dv64.ensure(bytes));
+    if (kernel == LELPH_KERNEL_DFPT || kernel == LELPH_KERNEL_DFPT_LOCAL)
+    {
+        LELPH_TRY(c->ip_out.ensure(bytes));
+        LELPH_TRY(ip_r2q(c, 1, qpt, c->ip_out.as<z64>()));
+        // identity patterns: eig @ I^H = eig (read_dvscf.c:121-127)
+        hipLaunchKernelGGL(k_mode_rotate, dim3((unsigned)cdiv((int64_t)nm * ld, 256)), dim3(256),
+                           0, c->stream, c->ip_out.as<z64>(), c->eig_d.as<z64>(), nm, ld,
+                           c->dv64.as<z64>());
+        HIP_TRY(hipGetLastError());
+    }
+    else
+        HIP_TRY(hipMemsetAsync(c->dv64.p, 0, bytes, c->stream));
+    return finish_dv(c, qpt);
 }
 
 int lelph_internal_grid(const lelph_ctx* c, int64_t* nbz, const double** kcrys, const int** kmap)

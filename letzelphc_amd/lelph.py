@@ -19,8 +19,8 @@ import os
 
 import numpy as np
 
-from .abi import (KERNELS, PRECISIONS, Lattice, Pair, Pseudo, c_double_p, dptr,
-                  pack_system)
+from .abi import (KERNELS, PRECISIONS, Lattice, LongRange, LrParams, Pair, Pseudo, Symmetry,
+                  c_double_p, dptr, i64ptr, pack_system)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblelph_hip.so")
@@ -31,7 +31,10 @@ EXPORTS = [
     "lelph_dvlocq", "lelph_set_dvscf", "lelph_get_dvscf", "lelph_elph_local",
     "lelph_add_elph_nonlocal", "lelph_gkq", "lelph_gkq_batch", "lelph_compute_elphq",
     "lelph_elph_q_rotate", "lelph_get_kplusq_idxs", "lelph_last_timings",
-    "lelph_electronic_reps", "lelph_elph_q_rotate_gpu",
+    "lelph_electronic_reps", "lelph_elph_q_rotate_gpu", "lelph_dv_longrange",
+    "lelph_rotate_dvscf", "lelph_wigner_seitz", "lelph_interp_begin", "lelph_interp_add_coarse",
+    "lelph_interp_add_star", "lelph_interp_finish", "lelph_interp_dvscf",
+    "lelph_interp_set_dvscf",
 ]
 
 
@@ -85,6 +88,20 @@ def lib():
                                             C.c_int64, c_double_p]
         L.lelph_elph_q_rotate_gpu.argtypes = [C.c_void_p, c_double_p, c_double_p, c_double_p,
                                               C.c_int, C.c_int, C.c_int, C.c_int, c_double_p]
+        i64p = C.POINTER(C.c_int64)
+        L.lelph_dv_longrange.argtypes = [C.c_void_p, c_double_p, C.POINTER(LongRange),
+                                         c_double_p, C.c_int, c_double_p]
+        L.lelph_rotate_dvscf.argtypes = [C.c_void_p, c_double_p, c_double_p,
+                                         C.POINTER(Symmetry), C.c_int, c_double_p]
+        L.lelph_wigner_seitz.argtypes = [i64p, c_double_p, C.c_double, c_double_p, C.c_int64,
+                                         c_double_p, C.c_int64, C.c_int64, i64p, i64p, i64p]
+        L.lelph_interp_begin.argtypes = [C.c_void_p, i64p, c_double_p, C.POINTER(LongRange)]
+        L.lelph_interp_add_coarse.argtypes = [C.c_void_p, c_double_p, c_double_p, c_double_p]
+        L.lelph_interp_add_star.argtypes = [C.c_void_p, C.POINTER(Symmetry), c_double_p,
+                                            c_double_p]
+        L.lelph_interp_finish.argtypes = [C.c_void_p]
+        L.lelph_interp_dvscf.argtypes = [C.c_void_p, C.c_int64, c_double_p, c_double_p]
+        L.lelph_interp_set_dvscf.argtypes = [C.c_void_p, c_double_p, c_double_p, C.c_int]
         _lib = L
     return _lib
 
@@ -247,6 +264,87 @@ class Context:
                "elph_q_rotate_gpu")
         return out
 
+    # ---- Fourier interpolation of dV_q ----
+    def dv_longrange(self, qpt, lr: LrParams, dvscf, eig=None, sign: int = 1) -> np.ndarray:
+        """dV_add_longrange (src/dvloc/dVpot_add_long_range.c:18-241) on the
+        device; returns dvscf + sign * long range."""
+        out = np.ascontiguousarray(dvscf, np.complex128).copy()
+        q = np.ascontiguousarray(qpt, np.float64)
+        e = None if eig is None else np.ascontiguousarray(eig, np.complex128)
+        plr = lr.pack()
+        _check(lib().lelph_dv_longrange(self.h, dptr(q), C.byref(plr),
+                                        None if e is None else dptr(e), int(sign), dptr(out)),
+               "lelph_dv_longrange")
+        return out
+
+    def rotate_dvscf(self, qpt, dvscf, sym, composite: bool = False) -> np.ndarray:
+        """rotate_dvscf (src/symmetries/rotate_dvscf.c:22-368) on the device;
+        sym = (R, tau, time_rev)."""
+        R, tau, tr = sym
+        s = Symmetry()
+        for i in range(9):
+            s.Rmat[i] = float(np.asarray(R, np.float64).reshape(9)[i])
+        for i in range(3):
+            s.tau[i] = float(np.asarray(tau, np.float64)[i])
+        s.time_rev = int(bool(tr))
+        din = np.ascontiguousarray(dvscf, np.complex128)
+        out = np.empty_like(din)
+        q = np.ascontiguousarray(qpt, np.float64)
+        _check(lib().lelph_rotate_dvscf(self.h, dptr(q), dptr(din), C.byref(s),
+                                        int(bool(composite)), dptr(out)), "lelph_rotate_dvscf")
+        return out
+
+    def interp_begin(self, qgrid, masses=None, lr: LrParams | None = None):
+        g = np.ascontiguousarray(qgrid, np.int64)
+        self._ip_mass = None if masses is None else np.ascontiguousarray(masses, np.float64)
+        self._ip_lr = lr
+        plr = None if lr is None else lr.pack()
+        self._ip_plr = plr
+        _check(lib().lelph_interp_begin(self.h, i64ptr(g),
+                                        None if self._ip_mass is None else dptr(self._ip_mass),
+                                        None if plr is None else C.byref(plr)),
+               "lelph_interp_begin")
+
+    def interp_add_coarse(self, qpt, eig, dvscf):
+        q = np.ascontiguousarray(qpt, np.float64)
+        _check(lib().lelph_interp_add_coarse(self.h, dptr(q),
+                                             dptr(np.ascontiguousarray(eig, np.complex128)),
+                                             dptr(np.ascontiguousarray(dvscf, np.complex128))),
+               "lelph_interp_add_coarse")
+
+    def interp_add_star(self, sym, qpt_Sq, eig_Sq):
+        R, tau, tr = sym
+        s = Symmetry()
+        for i in range(9):
+            s.Rmat[i] = float(np.asarray(R, np.float64).reshape(9)[i])
+        for i in range(3):
+            s.tau[i] = float(np.asarray(tau, np.float64)[i])
+        s.time_rev = int(bool(tr))
+        q = np.ascontiguousarray(qpt_Sq, np.float64)
+        _check(lib().lelph_interp_add_star(self.h, C.byref(s), dptr(q),
+                                           dptr(np.ascontiguousarray(eig_Sq, np.complex128))),
+               "lelph_interp_add_star")
+
+    def interp_finish(self):
+        _check(lib().lelph_interp_finish(self.h), "lelph_interp_finish")
+
+    def interp_dvscf(self, qpts) -> np.ndarray:
+        """(nq, nmodes, nmag, Nx, Ny, Nz) interpolated dV, cartesian basis."""
+        q = np.ascontiguousarray(np.atleast_2d(qpts), np.float64)
+        s = self.sys
+        out = np.empty((len(q), s.nmodes, s.nmag) + tuple(s.fft_dims), np.complex128)
+        _check(lib().lelph_interp_dvscf(self.h, len(q), dptr(q), dptr(out)),
+               "lelph_interp_dvscf")
+        return out
+
+    def interp_set_dvscf(self, qpt, eig, kernel: str = "dfpt"):
+        q = np.ascontiguousarray(qpt, np.float64)
+        _check(lib().lelph_interp_set_dvscf(self.h, dptr(q),
+                                            dptr(np.ascontiguousarray(eig, np.complex128)),
+                                            KERNELS[kernel]), "lelph_interp_set_dvscf")
+        self.q = q
+        self.eig = np.ascontiguousarray(eig, np.complex128)
+
     def last_timings(self) -> dict:
         """Device milliseconds of the last call, summed over its pairs."""
         t = np.zeros(6)
@@ -273,3 +371,28 @@ def kplusq_idxs(kpts_crys, q) -> np.ndarray:
                                        out.ctypes.data_as(C.POINTER(C.c_int))),
            "lelph_get_kplusq_idxs")
     return out
+
+
+def wigner_seitz(grid, lat, eps, rvec_m=None, rvec_n=None):
+    """build_wigner_seitz_vectors (src/interpolation/wigner_seitz.c:27-202),
+    host code of the library: (ws_vecs (nws,3) int64, ws_degen)."""
+    g = np.ascontiguousarray(grid, np.int64)
+    a = np.ascontiguousarray(lat, np.float64)
+    rm = None if rvec_m is None else np.ascontiguousarray(rvec_m, np.float64)
+    rn = None if rvec_n is None else np.ascontiguousarray(rvec_n, np.float64)
+    nm = 1 if rm is None else len(rm)
+    nn = 1 if rn is None else len(rn)
+    degen = np.zeros(int(np.prod(g)) * nm * nn, np.int64)
+    cap = 16 * len(degen)
+    while True:
+        vecs = np.zeros((cap, 3), np.int64)
+        n = C.c_int64(0)
+        rc = lib().lelph_wigner_seitz(i64ptr(g), dptr(a), float(eps),
+                                      None if rm is None else dptr(rm), nm,
+                                      None if rn is None else dptr(rn), nn, cap, i64ptr(vecs),
+                                      i64ptr(degen), C.byref(n))
+        if rc == 4:
+            cap = int(n.value)
+            continue
+        _check(rc, "lelph_wigner_seitz")
+        return vecs[: n.value].copy(), degen

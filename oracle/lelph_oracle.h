@@ -114,6 +114,34 @@ int orc_electronic_reps(const orc_wfc* wfcs, const lelph_lattice* lat,
                         const double* Rsym, const double* tauR, int tim_revR,
                         int64_t ikBZ, double* Dkmn);
 
+/* ---- Fourier interpolation of dV_q (interp_oracle.c) ---- */
+void orc_get_fft_box(double EcutRy, const double* blat, int64_t* box);
+/* returns the number of vectors, -1 if cap is too small, -2 if none found */
+int64_t orc_ws_vectors(const int64_t* grid, const double* lat, double eps,
+                       const double* rvec_m, int64_t nrvec_m,
+                       const double* rvec_n, int64_t nrvec_n, int64_t cap,
+                       int64_t* ws_vecs, int64_t* ws_degen);
+void orc_dVlong_range_kernel(const double* qpt, const double* gvecs, int64_t npw,
+                             const double* Zvals, const double* eps, const double* Zeu,
+                             const double* Qpole, int64_t natom, const double* atom_pos,
+                             char dimension, double volume, double zlat, double EcutRy,
+                             double eta_bare, double eta_ind, double* out);
+int orc_dV_add_longrange(const double* qpt, const lelph_lattice* lat,
+                         const lelph_longrange* lr, const double* eigVec, int sign,
+                         double* dvscf);
+void orc_multiply_eikr(double* pot, const double* q, const int64_t* N, int64_t nsets,
+                       int sign);
+void orc_dvscf_change_basis(double* dvscf, const double* rot, int64_t nsets, int64_t nm,
+                            int64_t nmag, int64_t nf, char op);
+int orc_rotate_dvscf(const double* qpt, const double* dvin, const lelph_symmetry* sym,
+                     const lelph_lattice* lat, int composite, double* dvout);
+void orc_fft_q2R(double* data, const int64_t* qgrid, int64_t nsets);
+int orc_fft_R2q_dvscf(const double* dataR, const double* q, const int64_t* qgrid,
+                      int64_t natom, int64_t nsets, const int64_t* ws, int64_t nws,
+                      const int64_t* degen, double* dataq);
+int orc_find_qpt_grid(int64_t nq, const double* q, int64_t* grid);
+void orc_sorted_qpts_idxs(int64_t nq, const double* q, int64_t* idx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,8 @@ import subprocess
 
 import numpy as np
 
-from letzelphc_amd.abi import Lattice, Pseudo, c_double_p, dptr, pack_system
+from letzelphc_amd.abi import (Lattice, LongRange, LrParams, Pseudo, Symmetry, c_double_p,
+                                dptr, i64ptr, pack_system)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liblelph_oracle.so")
@@ -89,6 +90,27 @@ def lib():
                                           C.POINTER(C.c_int64)]
         L.orc_electronic_reps.argtypes = [C.POINTER(OrcWfc), C.POINTER(Lattice), c_double_p,
                                           c_double_p, C.c_int, C.c_int64, c_double_p]
+        i64p = C.POINTER(C.c_int64)
+        L.orc_get_fft_box.argtypes = [C.c_double, c_double_p, i64p]
+        L.orc_ws_vectors.restype = C.c_int64
+        L.orc_ws_vectors.argtypes = [i64p, c_double_p, C.c_double, c_double_p, C.c_int64,
+                                     c_double_p, C.c_int64, C.c_int64, i64p, i64p]
+        L.orc_dVlong_range_kernel.argtypes = [c_double_p, c_double_p, C.c_int64, c_double_p,
+                                              c_double_p, c_double_p, c_double_p, C.c_int64,
+                                              c_double_p, C.c_char, C.c_double, C.c_double,
+                                              C.c_double, C.c_double, C.c_double, c_double_p]
+        L.orc_dV_add_longrange.argtypes = [c_double_p, C.POINTER(Lattice), C.POINTER(LongRange),
+                                           c_double_p, C.c_int, c_double_p]
+        L.orc_multiply_eikr.argtypes = [c_double_p, c_double_p, i64p, C.c_int64, C.c_int]
+        L.orc_dvscf_change_basis.argtypes = [c_double_p, c_double_p, C.c_int64, C.c_int64,
+                                             C.c_int64, C.c_int64, C.c_char]
+        L.orc_rotate_dvscf.argtypes = [c_double_p, c_double_p, C.POINTER(Symmetry),
+                                       C.POINTER(Lattice), C.c_int, c_double_p]
+        L.orc_fft_q2R.argtypes = [c_double_p, i64p, C.c_int64]
+        L.orc_fft_R2q_dvscf.argtypes = [c_double_p, c_double_p, i64p, C.c_int64, C.c_int64,
+                                        i64p, C.c_int64, i64p, c_double_p]
+        L.orc_find_qpt_grid.argtypes = [C.c_int64, c_double_p, i64p]
+        L.orc_sorted_qpts_idxs.argtypes = [C.c_int64, c_double_p, i64p]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_inner_threads.argtypes = [C.c_int]
         _lib = L
@@ -214,3 +236,145 @@ class OracleSystem:
                 lib().orc_free_vloc_table(C.byref(self.table))
             except Exception:
                 pass
+
+
+# ---------------------------------------------------------------------------
+# Fourier interpolation of dV_q (oracle/interp_oracle.c), composed in the
+# order of src/interpolation/interpolation_driver.c:190-420, 790-835.
+# ---------------------------------------------------------------------------
+
+def make_sym(R, tau, tr) -> Symmetry:
+    s = Symmetry()
+    Rf = np.asarray(R, np.float64).reshape(9)
+    for i in range(9):
+        s.Rmat[i] = float(Rf[i])
+    for i in range(3):
+        s.tau[i] = float(np.asarray(tau, np.float64)[i])
+    s.time_rev = int(bool(tr))
+    return s
+
+
+def ws_vectors(grid, lat, eps, rvec_m=None, rvec_n=None):
+    g = np.ascontiguousarray(grid, np.int64)
+    a = np.ascontiguousarray(lat, np.float64)
+    rm = None if rvec_m is None else np.ascontiguousarray(rvec_m, np.float64)
+    rn = None if rvec_n is None else np.ascontiguousarray(rvec_n, np.float64)
+    nm = 1 if rm is None else len(rm)
+    nn = 1 if rn is None else len(rn)
+    degen = np.zeros(int(np.prod(g)) * nm * nn, np.int64)
+    cap = 64 * len(degen)
+    vecs = np.zeros((cap, 3), np.int64)
+    n = lib().orc_ws_vectors(i64ptr(g), dptr(a), float(eps), None if rm is None else dptr(rm),
+                             nm, None if rn is None else dptr(rn), nn, cap, i64ptr(vecs),
+                             i64ptr(degen))
+    if n < 0:
+        raise RuntimeError(f"orc_ws_vectors failed ({n})")
+    return vecs[:n].copy(), degen
+
+
+def lr_kernel(qcart, gvecs, Zvals, eps, Zeu, Qpole, atom_pos, dimension, volume, zlat,
+              EcutRy, eta_bare, eta_ind):
+    """dVlong_range_kernel: (npw, natom, 3) complex."""
+    g = np.ascontiguousarray(gvecs, np.float64)
+    pos = np.ascontiguousarray(atom_pos, np.float64)
+    na = len(pos)
+    out = np.zeros((len(g), na, 3), np.complex128)
+    opt = lambda a: None if a is None else dptr(np.ascontiguousarray(a, np.float64))  # noqa
+    keep = [None if a is None else np.ascontiguousarray(a, np.float64)
+            for a in (Zvals, eps, Zeu, Qpole)]
+    lib().orc_dVlong_range_kernel(dptr(np.ascontiguousarray(qcart, np.float64)), dptr(g),
+                                  len(g), *(None if a is None else dptr(a) for a in keep), na,
+                                  dptr(pos), dimension.encode(), float(volume), float(zlat),
+                                  float(EcutRy), float(eta_bare), float(eta_ind), dptr(out))
+    del opt
+    return out
+
+
+class OracleInterp:
+    """The oracle's Fourier interpolation for a synth.System-like object."""
+
+    def __init__(self, sys, qgrid, masses=None, lr: LrParams | None = None):
+        self.sys = sys
+        self.packed = pack_system(sys)
+        self.grid = np.ascontiguousarray(qgrid, np.int64)
+        self.nq = int(np.prod(self.grid))
+        self.masses = masses
+        self.lr = lr
+        self.plr = None if lr is None else lr.pack()
+        self.ws, self.degen = ws_vectors(self.grid, sys.alat, 1e-6, sys.atomic_pos)
+        nm, nmag = sys.nmodes, sys.nmag
+        self.shape = (nm, nmag) + tuple(sys.fft_dims)
+        self.data = np.zeros((self.nq,) + self.shape, np.complex128)
+        self.have = np.zeros(self.nq, bool)
+        self.last = None
+        self.ready = False
+
+    def slot(self, q):
+        idx = []
+        for i in range(3):
+            v = (q[i] - np.floor(q[i])) * self.grid[i]
+            idx.append(int(np.rint(v)) % int(self.grid[i]))
+        return (idx[0] * self.grid[1] + idx[1]) * self.grid[2] + idx[2]
+
+    def longrange(self, q, dv, eig, sign):
+        dv = np.ascontiguousarray(dv, np.complex128).copy()
+        e = None if eig is None else np.ascontiguousarray(eig, np.complex128)
+        check(lib().orc_dV_add_longrange(dptr(np.ascontiguousarray(q, np.float64)),
+                                         C.byref(self.packed.lattice), C.byref(self.plr),
+                                         None if e is None else dptr(e), int(sign), dptr(dv)))
+        return dv
+
+    def _store(self, q, eig, dv):
+        s = self.sys
+        rot = np.ascontiguousarray(eig, np.complex128).copy()
+        if self.masses is not None:
+            rot = rot * np.repeat(np.asarray(self.masses, np.float64), 3)[None, :]
+        d = np.ascontiguousarray(dv, np.complex128).copy()
+        lib().orc_dvscf_change_basis(dptr(d), dptr(rot), 1, s.nmodes, s.nmag, s.nfft, b"C")
+        dims = np.ascontiguousarray(s.fft_dims, np.int64)
+        lib().orc_multiply_eikr(dptr(d), dptr(np.ascontiguousarray(q, np.float64)), i64ptr(dims),
+                                s.nmodes * s.nmag, 1)
+        i = self.slot(q)
+        assert not self.have[i]
+        self.data[i] = d
+        self.have[i] = True
+
+    def add_coarse(self, q, eig, dv):
+        d = np.ascontiguousarray(dv, np.complex128).copy()
+        if self.lr is not None:
+            d = self.longrange(q, d, eig, -1)
+        self.last = (np.asarray(q, np.float64), d)
+        self._store(q, eig, d)
+
+    def rotate(self, q, dv, sym, composite=False):
+        out = np.empty_like(dv)
+        check(lib().orc_rotate_dvscf(dptr(np.ascontiguousarray(q, np.float64)),
+                                     dptr(np.ascontiguousarray(dv, np.complex128)),
+                                     C.byref(make_sym(*sym)), C.byref(self.packed.lattice),
+                                     int(bool(composite)), dptr(out)))
+        return out
+
+    def add_star(self, sym, qSq, eigSq):
+        q0, d0 = self.last
+        self._store(qSq, eigSq, self.rotate(q0, d0, sym, False))
+
+    def finish(self):
+        assert self.have.all()
+        flat = np.ascontiguousarray(self.data.reshape(self.nq, -1))
+        lib().orc_fft_q2R(dptr(flat), i64ptr(self.grid), flat.shape[1])
+        self.dataR = flat
+        self.ready = True
+
+    def dvscf(self, q):
+        s = self.sys
+        out = np.zeros(self.shape, np.complex128)
+        check(lib().orc_fft_R2q_dvscf(dptr(self.dataR), dptr(np.ascontiguousarray(q, np.float64)),
+                                      i64ptr(self.grid), s.natom, s.nmag * s.nfft,
+                                      i64ptr(self.ws), len(self.ws), i64ptr(self.degen),
+                                      dptr(out)))
+        dims = np.ascontiguousarray(s.fft_dims, np.int64)
+        lib().orc_multiply_eikr(dptr(out), dptr(np.ascontiguousarray(q, np.float64)),
+                                i64ptr(dims), s.nmodes * s.nmag, -1)
+        if self.lr is not None:
+            out = self.longrange(q, out, None, 1)
+        return out

@@ -13,6 +13,13 @@ own helper routines, compiled from its sources into oracle/_ref/liblelph_ref.so
   Vloc_Gspace                        src/dvloc/VlocG_atom.c:10-57
   sum_VNL_KKp                        src/nonloc/sum_VNL_KKp.c:8-58
 
+and tests/golden/interp_vectors.npz:
+
+  build_wigner_seitz_vectors         src/interpolation/wigner_seitz.c:27-313
+  dVlong_range_kernel                src/dvloc/dVlong_range_kernel.c:31-458
+  multiply_eikr                      src/dvloc/multiply_eikr.c:14-64
+  rotate_dvscf                       src/symmetries/rotate_dvscf.c:22-368
+
 Run:  python tests/golden/make_ref_vectors.py
 The fixture is data (seeded inputs, the reference's outputs); the tests
 compare the oracle's restatement against it.
@@ -209,12 +216,150 @@ def make(L, seed=2024):
     return f
 
 
+def load_interp(L):
+    """Signatures of the interpolation helpers (and the flat glue of
+    oracle/ref_glue.c, which only fills the reference's structs)."""
+    L.rg_init.restype = C.c_int
+    L.rg_ws.restype = ll
+    L.rg_ws.argtypes = [C.POINTER(ll), dp, C.c_double, dp, ll, dp, ll, ll, C.POINTER(ll),
+                        C.POINTER(ll)]
+    L.dVlong_range_kernel.argtypes = [dp, dp, ll, dp, dp, dp, dp, ll, dp, C.c_char, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_double, dp]
+    L.rg_multiply_eikr.argtypes = [dp, dp, C.POINTER(ll), ll, C.c_int]
+    L.rg_rotate_dvscf.argtypes = [dp, dp, dp, dp, C.c_int, C.POINTER(ll), ll, ll, C.c_int, dp,
+                                  dp, C.c_int, dp]
+    if L.rg_init() != 0:
+        raise RuntimeError("MPI initialisation of the reference library failed")
+    return L
+
+
+def _lp(a):
+    return a.ctypes.data_as(C.POINTER(ll))
+
+
+WS_CASES = {
+    # name: (lattice a[:, i], grid, atoms (crystal), pair form)
+    "fcc": (lambda: np.array([[-5.13, 0, -5.13], [0, 5.13, 5.13], [5.13, 5.13, 0]]),
+            (4, 4, 4), [[0, 0, 0], [0.25, 0.25, 0.25]], False),
+    "fccdyn": (lambda: np.array([[-5.13, 0, -5.13], [0, 5.13, 5.13], [5.13, 5.13, 0]]),
+               (3, 3, 3), [[0, 0, 0], [0.25, 0.25, 0.25]], True),
+    "hex": (lambda: np.array([[5.9, -2.95, 0], [0, 5.9 * np.sqrt(3) / 2, 0], [0, 0, 38.0]]),
+            (6, 6, 1), [[1 / 3, 2 / 3, 0.5], [2 / 3, 1 / 3, 0.58], [2 / 3, 1 / 3, 0.42]], False),
+}
+
+
+def make_interp(L, seed=77):
+    """Wigner-Seitz vectors (src/interpolation/wigner_seitz.c:27),
+    dVlong_range_kernel (src/dvloc/dVlong_range_kernel.c:31), multiply_eikr
+    (src/dvloc/multiply_eikr.c:14) and rotate_dvscf
+    (src/symmetries/rotate_dvscf.c:22) on seeded inputs."""
+    rng = np.random.default_rng(seed)
+    f = {}
+    for name, (latf, grid, crys, pair) in WS_CASES.items():
+        lat = np.ascontiguousarray(latf())
+        pos = np.ascontiguousarray(np.asarray(crys, np.float64) @ lat.T)
+        g = np.ascontiguousarray(grid, np.int64)
+        na = len(pos)
+        nb = int(np.prod(g)) * na * (na if pair else 1)
+        cap = 64 * nb
+        vecs = np.zeros((cap, 3), np.int64)
+        degen = np.zeros(nb, np.int64)
+        n = L.rg_ws(_lp(g), P(lat), 1e-6, P(pos), na, P(pos) if pair else None,
+                    na if pair else 0, cap, _lp(vecs), _lp(degen))
+        assert n > 0
+        f.update({f"ws_{name}_lat": lat, f"ws_{name}_grid": g, f"ws_{name}_pos": pos,
+                  f"ws_{name}_pair": np.array(int(pair)), f"ws_{name}_vecs": vecs[:n].copy(),
+                  f"ws_{name}_degen": degen})
+    # long-range kernel: 3-D and 2-D, each with four sets of terms
+    lat3 = np.array([[-5.13, 0, -5.13], [0, 5.13, 5.13], [5.13, 5.13, 0]])
+    lat2 = np.array([[5.9, -2.95, 0], [0, 5.9 * np.sqrt(3) / 2, 0], [0, 0, 38.0]])
+    for dim, lat in (("3", lat3), ("2", lat2)):
+        blat = 2 * np.pi * np.linalg.inv(lat).T
+        m = rng.integers(-3, 4, (40, 3)).astype(np.float64)
+        q = rng.uniform(-0.5, 0.5, 3)
+        if dim == "2":
+            # the induced 2-D terms decay as 1 - tanh(|q+G|_par L/2): keep
+            # most in-plane q+G small so that they are not cut off
+            m[:, 2] = rng.integers(-12, 13, 40)
+            m[:30, :2] = 0
+            q = np.array([rng.uniform(-0.02, 0.02), rng.uniform(-0.02, 0.02), 0.0])
+        m[0] = -np.round(q)  # q + G small but not zero
+        gv = np.ascontiguousarray((m @ blat.T) / (2 * np.pi))
+        qc = np.ascontiguousarray(blat @ q / (2 * np.pi))
+        na = 2
+        pos = np.ascontiguousarray(rng.uniform(0, 1, (na, 3)) @ lat.T)
+        A = rng.standard_normal((3, 3))
+        eps = np.ascontiguousarray(4.0 * np.eye(3) + 0.3 * (A + A.T))
+        if dim == "2":
+            eps[0, 2] = eps[2, 0] = eps[1, 2] = eps[2, 1] = 0.0
+        zb = np.ascontiguousarray(rng.standard_normal((na, 3, 3)))
+        qp = np.ascontiguousarray(rng.standard_normal((na, 3, 3, 3)))
+        zv = np.ascontiguousarray(rng.uniform(1, 6, na))
+        vol = abs(np.linalg.det(lat))
+        outs = []
+        for case in range(4):
+            args = {0: (zv, None, None, None), 1: (None, eps, zb, None),
+                    2: (None, eps, zb, qp), 3: (zv, eps, zb, qp)}[case]
+            out = np.zeros((len(gv), na, 3), np.complex128)
+            L.dVlong_range_kernel(P(qc), P(gv), len(gv),
+                                  *(None if a is None else P(a) for a in args), na, P(pos),
+                                  dim.encode(), vol, lat[2, 2], 30.0, 1.0, 1.3,
+                                  out.ctypes.data_as(dp))
+            outs.append(out)
+        f.update({f"lr{dim}_q": qc, f"lr{dim}_g": gv, f"lr{dim}_pos": pos, f"lr{dim}_eps": eps,
+                  f"lr{dim}_zb": zb, f"lr{dim}_qp": qp, f"lr{dim}_zv": zv,
+                  f"lr{dim}_vol": np.array(vol), f"lr{dim}_zlat": np.array(lat[2, 2]),
+                  f"lr{dim}_out": np.array(outs)})
+    # multiply_eikr
+    dims = np.array([4, 5, 6], np.int64)
+    pot = np.ascontiguousarray(rng.standard_normal((2, 4, 5, 6)) + 1j * rng.standard_normal((2, 4, 5, 6)))
+    q = np.ascontiguousarray(rng.uniform(-1, 1, 3))
+    outs = []
+    for sg in (1, -1):
+        t = pot.copy()
+        L.rg_multiply_eikr(t.ctypes.data_as(dp), P(q), _lp(dims), 2, sg)
+        outs.append(t)
+    f.update(ek_pot=pot, ek_q=q, ek_out=np.array(outs))
+    # rotate_dvscf on a 6^3 cubic grid: C4z with a half translation, and a
+    # time-reversed mirror; nmag 1 and 4 (composite and not)
+    a = 8.0
+    alat = np.ascontiguousarray(a * np.eye(3))
+    blat = np.ascontiguousarray(2 * 3.1415927 * np.linalg.inv(alat).T)
+    dims = np.array([6, 6, 6], np.int64)
+    C4 = np.array([[0.0, -1, 0], [1, 0, 0], [0, 0, 1]])
+    syms = [(C4, np.array([a / 2, 0, 0]), 0), (np.diag([1.0, -1, 1]), np.zeros(3), 1)]
+    qr = np.ascontiguousarray([0.25, 0.5, 0.0])
+    for nmag in (1, 4):
+        din = np.ascontiguousarray(rng.standard_normal((2, nmag, 6, 6, 6))
+                                   + 1j * rng.standard_normal((2, nmag, 6, 6, 6)))
+        outs = []
+        for R, tau, tr in syms:
+            for comp in ((0, 1) if nmag == 4 else (0,)):
+                o = np.zeros_like(din)
+                L.rg_rotate_dvscf(P(qr), din.ctypes.data_as(dp), P(np.ascontiguousarray(R)),
+                                  P(np.ascontiguousarray(tau)), tr, _lp(dims), 2, nmag, 2, P(alat),
+                                  P(blat), comp, o.ctypes.data_as(dp))
+                outs.append(o)
+        f.update({f"rot{nmag}_in": din, f"rot{nmag}_out": np.array(outs)})
+    f.update(rot_q=qr, rot_alat=alat, rot_blat=blat,
+             rot_R=np.array([s[0] for s in syms]), rot_tau=np.array([s[1] for s in syms]),
+             rot_tr=np.array([s[2] for s in syms]))
+    return f
+
+
+OUT_INTERP = os.path.join(HERE, "interp_vectors.npz")
+
+
 def main():
     if not os.path.exists(REF_LIB):
         raise SystemExit(f"{REF_LIB} missing: run `make -C oracle ref` first")
-    f = make(load_ref())
+    L = load_ref()
+    f = make(L)
     np.savez_compressed(OUT, **f)
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {len(f)} arrays)")
+    fi = make_interp(load_interp(L))
+    np.savez_compressed(OUT_INTERP, **fi)
+    print(f"wrote {OUT_INTERP} ({os.path.getsize(OUT_INTERP)} bytes, {len(fi)} arrays)")
 
 
 if __name__ == "__main__":

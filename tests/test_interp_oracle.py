@@ -1,0 +1,259 @@
+"""The oracle's Fourier interpolation of dV_q (oracle/interp_oracle.c) and
+the library's host Wigner-Seitz search, pinned on the CPU:
+
+  * against tests/golden/interp_vectors.npz, written from the reference's own
+    wigner_seitz.c (kd-tree), dVlong_range_kernel.c, multiply_eikr.c and
+    rotate_dvscf.c compiled into oracle/_ref (and live on a second seed when
+    that library is present);
+  * fft_q2R against numpy's FFT;
+  * the long-range G sum placed on the grid against a brute-force DFT;
+  * the whole coarse -> R -> q chain reproduces the coarse potentials;
+  * the dipole branch against the monopole branch at equivalent parameters;
+  * the cartesian/mode basis change as a round trip.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from letzelphc_amd import lelph, synth
+from letzelphc_amd.abi import LrParams, dptr, i64ptr
+from oracle import oracle as O
+
+from .util import rel_err, say
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIXTURE = os.path.join(HERE, "golden", "interp_vectors.npz")
+
+
+def _load():
+    with np.load(FIXTURE, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def _ws_sets(vecs, degen):
+    out, o = [], 0
+    for d in degen:
+        out.append(sorted(map(tuple, vecs[o:o + d])))
+        o += d
+    return out
+
+
+def _check_ws(f, impl):
+    for name in ("fcc", "fccdyn", "hex"):
+        lat, g, pos = f[f"ws_{name}_lat"], f[f"ws_{name}_grid"], f[f"ws_{name}_pos"]
+        pair = bool(f[f"ws_{name}_pair"])
+        v, d = impl(g, lat, 1e-6, pos, pos if pair else None)
+        np.testing.assert_array_equal(d, f[f"ws_{name}_degen"])
+        assert _ws_sets(v, d) == _ws_sets(f[f"ws_{name}_vecs"], f[f"ws_{name}_degen"]), name
+
+
+def check_against(f):
+    _check_ws(f, O.ws_vectors)
+    for dim in ("3", "2"):
+        q, g, pos = f[f"lr{dim}_q"], f[f"lr{dim}_g"], f[f"lr{dim}_pos"]
+        eps, zb, qp, zv = (f[f"lr{dim}_{k}"] for k in ("eps", "zb", "qp", "zv"))
+        for case in range(4):
+            args = {0: (zv, None, None, None), 1: (None, eps, zb, None),
+                    2: (None, eps, zb, qp), 3: (zv, eps, zb, qp)}[case]
+            got = O.lr_kernel(q, g, args[0], args[1], args[2], args[3], pos, dim,
+                              float(f[f"lr{dim}_vol"]), float(f[f"lr{dim}_zlat"]), 30.0, 1.0, 1.3)
+            ref = f[f"lr{dim}_out"][case]
+            assert np.abs(ref).max() > 0
+            assert rel_err(got, ref) < 1e-12, (dim, case)
+    dims = np.array([4, 5, 6], np.int64)
+    for i, sg in enumerate((1, -1)):
+        t = f["ek_pot"].copy()
+        O.lib().orc_multiply_eikr(dptr(t), dptr(f["ek_q"]), i64ptr(dims), 2, sg)
+        assert rel_err(t, f["ek_out"][i]) < 1e-13
+    lat = _rot_lattice(f)
+    for nmag in (1, 4):
+        lat.nmag = nmag
+        din = f[f"rot{nmag}_in"]
+        j = 0
+        for R, tau, tr in zip(f["rot_R"], f["rot_tau"], f["rot_tr"]):
+            for comp in ((0, 1) if nmag == 4 else (0,)):
+                out = np.empty_like(din)
+                O.check(O.lib().orc_rotate_dvscf(dptr(f["rot_q"]), dptr(din),
+                                                 C.byref(O.make_sym(R, tau, tr)), C.byref(lat),
+                                                 comp, dptr(out)))
+                assert rel_err(out, f[f"rot{nmag}_out"][j]) < 1e-13, (nmag, j)
+                j += 1
+
+
+def _rot_lattice(f):
+    from letzelphc_amd.abi import Lattice
+    L = Lattice()
+    L.natom = 0
+    L.nmodes = 2
+    L.nspinor = 2
+    for i in range(3):
+        L.fft_dims[i] = 6
+    for i in range(9):
+        L.alat_vec[i] = float(f["rot_alat"].reshape(9)[i])
+        L.blat_vec[i] = float(f["rot_blat"].reshape(9)[i])
+    return L
+
+
+def test_interp_oracle_vs_reference_fixture():
+    say("case: interpolation helpers vs reference fixture")
+    check_against(_load())
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_LIB), reason="oracle/_ref not built")
+def test_interp_oracle_vs_reference_live():
+    say("case: interpolation helpers vs reference library, second seed")
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "make_ref_vectors", os.path.join(HERE, "golden", "make_ref_vectors.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    L = mod.load_interp(mod.load_ref())
+    check_against(mod.make_interp(L, seed=5))
+
+
+def test_host_wigner_seitz_vs_fixture():
+    """The library's own host search (lelph_wigner_seitz) against the
+    reference's kd-tree results."""
+    say("case: lelph_wigner_seitz vs reference fixture")
+    _check_ws(_load(), lelph.wigner_seitz)
+
+
+def test_fft_q2R_vs_numpy():
+    rng = np.random.default_rng(3)
+    g = np.array([3, 4, 2], np.int64)
+    d = rng.standard_normal((3, 4, 2, 5)) + 1j * rng.standard_normal((3, 4, 2, 5))
+    t = np.ascontiguousarray(d.reshape(24, 5)).copy()
+    O.lib().orc_fft_q2R(dptr(t), i64ptr(g), 5)
+    ref = np.fft.fftn(d, axes=(0, 1, 2)) / 24
+    assert rel_err(t.reshape(3, 4, 2, 5), ref) < 1e-14
+
+
+def _small_system(dimension="3"):
+    if dimension == "2":
+        s = synth.tiny(dimension="2", with_wfc=False)
+    else:
+        s = synth.tiny(with_wfc=False)
+    return s
+
+
+def _lr_params(s, quad=True):
+    rng = np.random.default_rng(11)
+    na = s.natom
+    zb = rng.standard_normal((na, 3, 3))
+    zb -= zb.mean(0)  # acoustic sum rule
+    eps = 6.0 * np.eye(3) + 0.2 * np.diag(rng.uniform(-1, 1, 3))
+    return LrParams(epsilon=eps, Zborn=zb, Qpole=rng.standard_normal((na, 3, 3, 3)) if quad
+                    else None, Zvals=np.array([3.0, 5.0])[:na], only_induced=False,
+                    eta_bare=1.0, eta_induced=1.0, EcutRy=12.0, nmags_add=(1, 1, 0, 0))
+
+
+@pytest.mark.parametrize("dimension", ["3", "2"])
+def test_longrange_grid_vs_bruteforce(dimension):
+    """dV_add_longrange's G box, placement and backward FFT against a direct
+    sum over the box of the kernel's coefficients."""
+    s = _small_system(dimension)
+    lr = _lr_params(s)
+    say(f"case: long range on the grid vs brute force, dimension {dimension}")
+    oi = O.OracleInterp(s, (2, 2, 1), lr=lr)
+    q = np.array([0.31, -0.12, 0.0])
+    nm, nmag = s.nmodes, s.nmag
+    dv0 = np.zeros((nm, nmag) + tuple(s.fft_dims), np.complex128)
+    got = oi.longrange(q, dv0, None, 1)
+    # brute force: every G of the box, e^{2 pi i G.r} on the grid
+    gb = np.zeros(3, np.int64)
+    O.lib().orc_get_fft_box(lr.EcutRy, dptr(np.ascontiguousarray(s.blat)), i64ptr(gb))
+    gb = np.minimum(gb, s.fft_dims)
+    if dimension == "2":
+        gb[2] = s.fft_dims[2]
+    ms = [synth.miller_axis(int(n)) for n in gb]
+    M = np.stack(np.meshgrid(*ms, indexing="ij"), -1).reshape(-1, 3)
+    gv = (M @ s.blat.T) / (2 * synth.RPI)
+    qc = s.blat @ q / (2 * synth.RPI)
+    V = O.lr_kernel(qc, gv, lr.Zvals, lr.epsilon, lr.Zborn, lr.Qpole, s.atomic_pos, dimension,
+                    s.volume, s.alat[2, 2], lr.EcutRy, lr.eta_bare, lr.eta_induced)
+    V = V.reshape(len(M), nm)
+    N = np.array(s.fft_dims)
+    r = [np.arange(n) / n for n in N]
+    ph = np.exp(2j * np.pi * (M[:, 0, None] * r[0][None, :]))[:, :, None, None] * \
+        np.exp(2j * np.pi * (M[:, 1, None] * r[1][None, :]))[:, None, :, None] * \
+        np.exp(2j * np.pi * (M[:, 2, None] * r[2][None, :]))[:, None, None, :]
+    ref = np.einsum("gv,gxyz->vxyz", V, ph)
+    assert np.abs(ref).max() > 0
+    assert rel_err(got[:, 0], ref) < 1e-11
+    # nmags_add (1, 1): the second magnetic component gets the same term
+    if nmag > 1:
+        assert rel_err(got[:, 1], ref) < 1e-11
+
+
+def test_dipole_vs_monopole_branch():
+    """With Z* = z I, eps = e I and eta_bare = eta_ind / e, the dipole term
+    equals the monopole term of a charge -z/e (both are -d/dtau of one
+    screened Coulomb potential)."""
+    rng = np.random.default_rng(4)
+    lat = synth.fcc_cell(10.26)
+    blat = 2 * np.pi * np.linalg.inv(lat).T
+    m = rng.integers(-3, 4, (30, 3)).astype(np.float64)
+    gv = (m @ blat.T) / (2 * np.pi)
+    qc = blat @ rng.uniform(-0.5, 0.5, 3) / (2 * np.pi)
+    pos = rng.uniform(0, 1, (2, 3)) @ lat.T
+    z, e, eta = 1.7, 5.0, 1.3
+    vol = abs(np.linalg.det(lat))
+    dip = O.lr_kernel(qc, gv, None, e * np.eye(3), np.stack([z * np.eye(3)] * 2), None, pos,
+                      "3", vol, lat[2, 2], 40.0, eta / e, eta)
+    mono = O.lr_kernel(qc, gv, np.full(2, -z / e), None, None, None, pos, "3", vol, lat[2, 2],
+                       40.0, eta / e, eta)
+    assert np.abs(dip).max() > 0
+    assert rel_err(dip, mono) < 1e-12
+
+
+def test_change_basis_round_trip():
+    rng = np.random.default_rng(5)
+    nm, nmag, nf = 6, 2, 30
+    u, _ = np.linalg.qr(rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm)))
+    u = np.ascontiguousarray(u)
+    d = rng.standard_normal((nm, nmag, nf)) + 1j * rng.standard_normal((nm, nmag, nf))
+    t = np.ascontiguousarray(d).copy()
+    O.lib().orc_dvscf_change_basis(dptr(t), dptr(u), 1, nm, nmag, nf, b"N")
+    assert rel_err(t, np.einsum("ab,bmf->amf", u, d)) < 1e-13
+    O.lib().orc_dvscf_change_basis(dptr(t), dptr(u), 1, nm, nmag, nf, b"C")
+    assert rel_err(t, d) < 1e-13
+
+
+def coarse_grid(grid):
+    g = np.stack(np.meshgrid(*[np.arange(n) / n for n in grid], indexing="ij"), -1)
+    return g.reshape(-1, 3)
+
+
+@pytest.mark.parametrize("with_lr", [False, True])
+def test_interpolation_exact_on_coarse_grid(with_lr):
+    """q -> R -> q returns every coarse potential in the cartesian basis
+    (the long range, when present, is removed and added back).  The
+    reference's q -> R uses FFTW's exact pi while R -> q and e^{iqr} use
+    ELPH_PI = 3.1415927, so the round trip holds to ~1e-7, not to rounding."""
+    s = synth.tiny(with_wfc=False)
+    grid = (2, 2, 2)
+    qs = coarse_grid(grid)
+    rng = np.random.default_rng(8)
+    masses = np.array([1.5, 2.5])
+    lr = _lr_params(s, quad=False) if with_lr else None
+    oi = O.OracleInterp(s, grid, masses=masses, lr=lr)
+    dvs, eigs = [], []
+    for i, q in enumerate(qs):
+        dv = rng.standard_normal(oi.shape) + 1j * rng.standard_normal(oi.shape)
+        e, _ = np.linalg.qr(rng.standard_normal((s.nmodes,) * 2)
+                            + 1j * rng.standard_normal((s.nmodes,) * 2))
+        # displacement patterns e / sqrt(M), the convention in which the
+        # mass-weighted conjugate transpose is the inverse
+        e = e / np.sqrt(np.repeat(masses, 3))[None, :]
+        dvs.append(dv)
+        eigs.append(e)
+        oi.add_coarse(q, e, dv)
+    oi.finish()
+    say(f"case: interpolation exact on the coarse grid, long range {with_lr}")
+    for i, q in enumerate(qs):
+        got = oi.dvscf(q)
+        rot = eigs[i] * np.repeat(masses, 3)[None, :]
+        ref = np.einsum("va,vm...->am...", rot.conj(), dvs[i])
+        assert rel_err(got, ref) < 2e-6, i
