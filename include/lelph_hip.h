@@ -231,6 +231,26 @@ int lelph_get_kplusq_idxs(int64_t nbz, const double* kpoints_crys,
                           const double qpt_crys[3], int* kplusq_idxs);
 
 /* ------------------------------------------------------------------------
+ * QE dvscf files: complex128 sets (nmodes * nmag) of (Nz, Ny, Nx), x fastest.
+ * ---------------------------------------------------------------------- */
+
+/* read_dvscf_qe (src/io/qe/read_dvscf.c:17-172): reads the file (pinned
+ * staging), transposes to (Nx, Ny, Nz) on the device and rotates patterns
+ * to modes with eigVec @ patterns^H; dvscf_out (nmodes, nmag, Nx, Ny, Nz). */
+int lelph_read_dvscf_qe(lelph_ctx* ctx, const char* dvscf_file, const double* eigVec,
+                        const double* patterns, double* dvscf_out);
+
+/* lelph_set_dvscf with the potential read from a QE dvscf file (kernel
+ * LELPH_KERNEL_DFPT or LELPH_KERNEL_DFPT_LOCAL), without a host copy of
+ * the transposed potential (src/elph/elph_driver.c:226-245). */
+int lelph_set_dvscf_file(lelph_ctx* ctx, const double qpt[3], const double* eigVec,
+                         const char* dvscf_file, const double* patterns, int kernel);
+
+/* write_dvscf_qe (src/io/qe/write_dyn_dvscf_qe.c:99-185): dvscf_in
+ * (nmodes, nmag, Nx, Ny, Nz) written in the file layout above. */
+int lelph_write_dvscf_qe(lelph_ctx* ctx, const char* dvscf_file, const double* dvscf_in);
+
+/* ------------------------------------------------------------------------
  * Fourier interpolation of dV_q onto arbitrary q
  * (src/interpolation/interpolation_driver.c:190-420 coarse side,
  * :790-835 fine side).

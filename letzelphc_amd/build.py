@@ -26,8 +26,31 @@ def _hipcc():
     return h
 
 
+NC4_LIB = os.path.join(HERE, "liblelph_nc4.so")
+HDF5_HOME = os.environ.get("LELPH_HDF5_HOME", "/opt/conda")
+
+
+def build_nc4(verbose: bool = False) -> str | None:
+    """The host-only netCDF-4 writer (csrc/nc4w.c) against HDF5 and its
+    high-level library; returns None when HDF5 is not installed."""
+    inc = os.path.join(HDF5_HOME, "include")
+    lib = os.path.join(HDF5_HOME, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5_hl.h")):
+        return None
+    tmp = NC4_LIB + ".tmp"
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-std=c99", "-Wall", f"-I{inc}",
+           os.path.join(CSRC, "nc4w.c"), "-o", tmp, f"-L{lib}", f"-Wl,-rpath,{lib}",
+           "-lhdf5_hl", "-lhdf5"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, NC4_LIB)
+    return NC4_LIB
+
+
 def build(verbose: bool = False) -> str:
     """Compiles every HIP and C source of the library; returns the .so path."""
+    build_nc4(verbose)
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
     objs = []

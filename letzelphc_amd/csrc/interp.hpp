@@ -454,3 +454,49 @@ __global__ __launch_bounds__(256) void k_rot_gather(const z64* __restrict__ in,
     }
     for (int m = 0; m < nmag; ++m) o[m * nf + p] = tr ? cconj(d[m]) : d[m];
 }
+
+// QE dvscf file layout (nsets, Nz, Ny, Nx) <-> (nsets, Nx, Ny, Nz)
+// (read_dvscf.c:83-101, write_dyn_dvscf_qe.c:145-161): per (set, y) a 2-D
+// transpose of Nz x Nx through a 32 x 32 LDS tile, coalesced on both sides.
+// TO_XYZ: in (z, y, x) -> out (x, y, z); otherwise the inverse.
+template <bool TO_XYZ>
+__global__ __launch_bounds__(256) void k_qe_transpose(const z64* __restrict__ in, int Nx, int Ny,
+                                                      int Nz, z64* __restrict__ out)
+{
+    __shared__ z64 t[32][33];
+    const int tx = (Nx + 31) / 32;
+    const int bx = blockIdx.x % tx, bz = blockIdx.x / tx;
+    const int y = blockIdx.y;
+    const int64_t nf = (int64_t)Nx * Ny * Nz;
+    const z64* src = in + (int64_t)blockIdx.z * nf;
+    z64* dst = out + (int64_t)blockIdx.z * nf;
+    const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5; // 32 x 8
+    // the file side is x-fastest, the grid side z-fastest
+    for (int r = ly; r < 32; r += 8)
+    {
+        if (TO_XYZ)
+        {
+            const int z = bz * 32 + r, x = bx * 32 + lx;
+            if (z < Nz && x < Nx) t[r][lx] = src[((int64_t)z * Ny + y) * Nx + x];
+        }
+        else
+        {
+            const int x = bx * 32 + r, z = bz * 32 + lx;
+            if (z < Nz && x < Nx) t[r][lx] = src[((int64_t)x * Ny + y) * Nz + z];
+        }
+    }
+    __syncthreads();
+    for (int r = ly; r < 32; r += 8)
+    {
+        if (TO_XYZ)
+        {
+            const int x = bx * 32 + r, z = bz * 32 + lx;
+            if (z < Nz && x < Nx) dst[((int64_t)x * Ny + y) * Nz + z] = t[lx][r];
+        }
+        else
+        {
+            const int z = bz * 32 + r, x = bx * 32 + lx;
+            if (z < Nz && x < Nx) dst[((int64_t)z * Ny + y) * Nx + x] = t[lx][r];
+        }
+    }
+}

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -1936,6 +1937,151 @@ int lelph_interp_set_dvscf(lelph_ctx* c, const double qpt[3], const double* eigV
     else
         HIP_TRY(hipMemsetAsync(c->dv64.p, 0, bytes, c->stream));
     return finish_dv(c, qpt);
+}
+
+/* ------------------------------------------------------------------------ */
+/* QE dvscf files (src/io/qe/read_dvscf.c, write_dyn_dvscf_qe.c)             */
+/* ------------------------------------------------------------------------ */
+
+// file sets (Nz, Ny, Nx) -> dst (device, (nsets, Nx, Ny, Nz)); pinned
+// staging of up to 64 MiB per read, transposed on the device
+static int read_qe_file(lelph_ctx* c, const char* path, z64* dst)
+{
+    const lelph_lattice& L = c->lat;
+    const int64_t nf = c->nf, nsets = (int64_t)L.nmodes * L.nmag;
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return LELPH_ERR_ARG;
+    std::fseek(fp, 0, SEEK_END);
+    const long sz = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    if (sz < 0 || (int64_t)sz != nsets * nf * (int64_t)sizeof(z64))
+    {
+        std::fclose(fp);
+        return LELPH_ERR_ARG;
+    }
+    const int64_t per = nf * (int64_t)sizeof(z64);
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nsets, ((int64_t)64 << 20) / per));
+    void* stage = nullptr;
+    if (hipHostMalloc(&stage, (size_t)(per * chunk)) != hipSuccess)
+    {
+        std::fclose(fp);
+        return LELPH_ERR_ALLOC;
+    }
+    int st = c->ip_W.ensure((size_t)(per * chunk));
+    const unsigned tiles = (unsigned)(cdiv(L.fft_dims[0], 32) * cdiv(L.fft_dims[2], 32));
+    for (int64_t s0 = 0; s0 < nsets && st == LELPH_OK; s0 += chunk)
+    {
+        const int64_t n = std::min(chunk, nsets - s0);
+        if (std::fread(stage, (size_t)per, (size_t)n, fp) != (size_t)n)
+        {
+            st = LELPH_ERR_ARG;
+            break;
+        }
+        if (hipMemcpyAsync(c->ip_W.p, stage, (size_t)(per * n), hipMemcpyHostToDevice,
+                           c->stream) != hipSuccess)
+        {
+            st = LELPH_ERR_HIP;
+            break;
+        }
+        hipLaunchKernelGGL(k_qe_transpose<true>, dim3(tiles, (unsigned)L.fft_dims[1], (unsigned)n),
+                           dim3(256), 0, c->stream, c->ip_W.as<z64>(), (int)L.fft_dims[0],
+                           (int)L.fft_dims[1], (int)L.fft_dims[2], dst + s0 * nf);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            st = LELPH_ERR_HIP;
+    }
+    (void)hipHostFree(stage);
+    std::fclose(fp);
+    return st;
+}
+
+// eig @ pats^H (read_dvscf.c:121-127), FP64 on the host (nm x nm), uploaded
+static int upload_eig_pat(lelph_ctx* c, const double* eigVec, const double* patterns)
+{
+    const int nm = c->lat.nmodes;
+    std::vector<z64> ep((size_t)nm * nm);
+    const z64* e = reinterpret_cast<const z64*>(eigVec);
+    const z64* pat = reinterpret_cast<const z64*>(patterns);
+    for (int i = 0; i < nm; ++i)
+        for (int j = 0; j < nm; ++j)
+        {
+            z64 acc = {0, 0};
+            for (int k = 0; k < nm; ++k)
+                acc = cadd(acc, cmul(e[(size_t)i * nm + k], cconj(pat[(size_t)j * nm + k])));
+            ep[(size_t)i * nm + j] = acc;
+        }
+    LELPH_TRY(c->ip_rot.ensure(sizeof(z64) * nm * nm));
+    HIP_TRY(hipMemcpyAsync(c->ip_rot.p, ep.data(), sizeof(z64) * nm * nm, hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LELPH_OK;
+}
+
+int lelph_read_dvscf_qe(lelph_ctx* c, const char* dvscf_file, const double* eigVec,
+                        const double* patterns, double* dvscf_out)
+{
+    if (!c || !dvscf_file || !eigVec || !patterns || !dvscf_out) return LELPH_ERR_ARG;
+    const lelph_lattice& L = c->lat;
+    const int nm = L.nmodes;
+    const int64_t ld = (int64_t)L.nmag * c->nf;
+    const size_t bytes = sizeof(z64) * (size_t)nm * ld;
+    LELPH_TRY(c->ip_tmp.ensure(bytes));
+    LELPH_TRY(c->rot_tmp.ensure(bytes));
+    LELPH_TRY(read_qe_file(c, dvscf_file, c->ip_tmp.as<z64>()));
+    LELPH_TRY(upload_eig_pat(c, eigVec, patterns));
+    hipLaunchKernelGGL(k_mode_rotate, dim3((unsigned)cdiv((int64_t)nm * ld, 256)), dim3(256), 0,
+                       c->stream, c->ip_tmp.as<z64>(), c->ip_rot.as<z64>(), nm, ld,
+                       c->rot_tmp.as<z64>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(dvscf_out, c->rot_tmp.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LELPH_OK;
+}
+
+int lelph_set_dvscf_file(lelph_ctx* c, const double qpt[3], const double* eigVec,
+                         const char* dvscf_file, const double* patterns, int kernel)
+{
+    if (!c || !qpt || !eigVec || !dvscf_file || !patterns) return LELPH_ERR_ARG;
+    if (kernel != LELPH_KERNEL_DFPT && kernel != LELPH_KERNEL_DFPT_LOCAL) return LELPH_ERR_ARG;
+    const lelph_lattice& L = c->lat;
+    const int nm = L.nmodes;
+    const int64_t ld = (int64_t)L.nmag * c->nf;
+    const size_t bytes = sizeof(z64) * (size_t)nm * ld;
+    c->dv_set = false;
+    c->dv32_valid = false;
+    LELPH_TRY(set_eig(c, eigVec));
+    LELPH_TRY(c->dv64.ensure(bytes));
+    LELPH_TRY(c->ip_tmp.ensure(bytes));
+    LELPH_TRY(read_qe_file(c, dvscf_file, c->ip_tmp.as<z64>()));
+    LELPH_TRY(upload_eig_pat(c, eigVec, patterns));
+    hipLaunchKernelGGL(k_mode_rotate, dim3((unsigned)cdiv((int64_t)nm * ld, 256)), dim3(256), 0,
+                       c->stream, c->ip_tmp.as<z64>(), c->ip_rot.as<z64>(), nm, ld,
+                       c->dv64.as<z64>());
+    HIP_TRY(hipGetLastError());
+    return finish_dv(c, qpt);
+}
+
+int lelph_write_dvscf_qe(lelph_ctx* c, const char* dvscf_file, const double* dvscf_in)
+{
+    if (!c || !dvscf_file || !dvscf_in) return LELPH_ERR_ARG;
+    const lelph_lattice& L = c->lat;
+    const int64_t nf = c->nf, nsets = (int64_t)L.nmodes * L.nmag;
+    const size_t bytes = sizeof(z64) * (size_t)(nsets * nf);
+    LELPH_TRY(c->ip_tmp.ensure(bytes));
+    LELPH_TRY(c->rot_tmp.ensure(bytes));
+    HIP_TRY(hipMemcpyAsync(c->ip_tmp.p, dvscf_in, bytes, hipMemcpyHostToDevice, c->stream));
+    const unsigned tiles = (unsigned)(cdiv(L.fft_dims[0], 32) * cdiv(L.fft_dims[2], 32));
+    hipLaunchKernelGGL(k_qe_transpose<false>, dim3(tiles, (unsigned)L.fft_dims[1], (unsigned)nsets),
+                       dim3(256), 0, c->stream, c->ip_tmp.as<z64>(), (int)L.fft_dims[0],
+                       (int)L.fft_dims[1], (int)L.fft_dims[2], c->rot_tmp.as<z64>());
+    HIP_TRY(hipGetLastError());
+    std::vector<z64> host((size_t)(nsets * nf));
+    HIP_TRY(hipMemcpyAsync(host.data(), c->rot_tmp.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    FILE* fp = std::fopen(dvscf_file, "wb");
+    if (!fp) return LELPH_ERR_ARG;
+    const size_t w = std::fwrite(host.data(), sizeof(z64), host.size(), fp);
+    const int cl = std::fclose(fp);
+    return (w == host.size() && cl == 0) ? LELPH_OK : LELPH_ERR_ARG;
 }
 
 int lelph_internal_grid(const lelph_ctx* c, int64_t* nbz, const double** kcrys, const int** kmap)

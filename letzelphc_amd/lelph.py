@@ -34,7 +34,8 @@ EXPORTS = [
     "lelph_electronic_reps", "lelph_elph_q_rotate_gpu", "lelph_dv_longrange",
     "lelph_rotate_dvscf", "lelph_wigner_seitz", "lelph_interp_begin", "lelph_interp_add_coarse",
     "lelph_interp_add_star", "lelph_interp_finish", "lelph_interp_dvscf",
-    "lelph_interp_set_dvscf",
+    "lelph_interp_set_dvscf", "lelph_read_dvscf_qe", "lelph_set_dvscf_file",
+    "lelph_write_dvscf_qe",
 ]
 
 
@@ -102,6 +103,11 @@ def lib():
         L.lelph_interp_finish.argtypes = [C.c_void_p]
         L.lelph_interp_dvscf.argtypes = [C.c_void_p, C.c_int64, c_double_p, c_double_p]
         L.lelph_interp_set_dvscf.argtypes = [C.c_void_p, c_double_p, c_double_p, C.c_int]
+        L.lelph_read_dvscf_qe.argtypes = [C.c_void_p, C.c_char_p, c_double_p, c_double_p,
+                                          c_double_p]
+        L.lelph_set_dvscf_file.argtypes = [C.c_void_p, c_double_p, c_double_p, C.c_char_p,
+                                           c_double_p, C.c_int]
+        L.lelph_write_dvscf_qe.argtypes = [C.c_void_p, C.c_char_p, c_double_p]
         _lib = L
     return _lib
 
@@ -263,6 +269,31 @@ class Context:
                                              int(bool(tim_rev)), dptr(out)),
                "elph_q_rotate_gpu")
         return out
+
+    # ---- QE dvscf files ----
+    def read_dvscf_qe(self, path, eig, patterns) -> np.ndarray:
+        """read_dvscf_qe (src/io/qe/read_dvscf.c:17-172): mode-basis dV."""
+        s = self.sys
+        out = np.empty((s.nmodes, s.nmag) + tuple(s.fft_dims), np.complex128)
+        _check(lib().lelph_read_dvscf_qe(self.h, os.fsencode(path),
+                                         dptr(np.ascontiguousarray(eig, np.complex128)),
+                                         dptr(np.ascontiguousarray(patterns, np.complex128)),
+                                         dptr(out)), "lelph_read_dvscf_qe")
+        return out
+
+    def set_dvscf_file(self, qpt, eig, path, patterns, kernel: str = "dfpt"):
+        q = np.ascontiguousarray(qpt, np.float64)
+        e = np.ascontiguousarray(eig, np.complex128)
+        _check(lib().lelph_set_dvscf_file(self.h, dptr(q), dptr(e), os.fsencode(path),
+                                          dptr(np.ascontiguousarray(patterns, np.complex128)),
+                                          KERNELS[kernel]), "lelph_set_dvscf_file")
+        self.q = q
+        self.eig = e
+
+    def write_dvscf_qe(self, path, dvscf):
+        _check(lib().lelph_write_dvscf_qe(self.h, os.fsencode(path),
+                                          dptr(np.ascontiguousarray(dvscf, np.complex128))),
+               "lelph_write_dvscf_qe")
 
     # ---- Fourier interpolation of dV_q ----
     def dv_longrange(self, qpt, lr: LrParams, dvscf, eig=None, sign: int = 1) -> np.ndarray:

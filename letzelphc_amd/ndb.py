@@ -1,0 +1,227 @@
+"""ndb.elph / ndb.Dmats writers over the netCDF-4 layout of
+include/lelph_nc4.h (liblelph_nc4.so, HDF5 underneath, no libnetcdf).
+
+The variable names, dimension names, shapes, types and chunk sizes are the
+reference's:
+  * elph_driver.c:140-186      POLARIZATION_VECTORS, FREQ, elph_mat (chunked)
+  * elph_driver.c:258-300      eigenvectors / frequencies per q of the star
+  * compute_elph_matq.c:145-217 elph_mat slabs [q, k] (and the star members)
+  * write_basic_data.c:15-215  kpoints, qpoints, qpoints_iBZ, qmap, kmap,
+                               bands, number_of_phonon_symmetries,
+                               time_reversal_phonon, symmetry_matrices,
+                               fractional_translation, kernel, convention,
+                               epsilon, Born_charges, Quadrupole_tensor
+  * compute_dmats.c:20-131      Dmats (chunked)
+Floating-point variables are NC_DOUBLE (the reference built with
+COMPILE_ELPH_DOUBLE, src/elphC.h:23).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import STATUS_NAMES
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblelph_nc4.so")
+
+NC_CHAR, NC_INT, NC_FLOAT, NC_DOUBLE = 2, 4, 5, 6
+NC4_DEFAULT_CHUNK_KB = 2048  # src/io/io.h:11
+_DTYPES = {NC_CHAR: np.uint8, NC_INT: np.int32, NC_FLOAT: np.float32, NC_DOUBLE: np.float64}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "letzelphc_amd.build.build_nc4() (needs HDF5)")
+        L = C.CDLL(LIB_PATH)
+        i64p = C.POINTER(C.c_int64)
+        L.lelph_nc4_create.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.lelph_nc4_def_var.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int, i64p,
+                                        C.POINTER(C.c_char_p), i64p]
+        L.lelph_nc4_put_vara.argtypes = [C.c_void_p, C.c_char_p, i64p, i64p, C.c_void_p]
+        L.lelph_nc4_close.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what}: status {rc} ({STATUS_NAMES.get(rc, '?')})")
+
+
+def _i64(a):
+    a = np.ascontiguousarray(a, np.int64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+class NC4File:
+    """nc_create / def_ncVar / nc_put_vara / nc_close."""
+
+    def __init__(self, path):
+        h = C.c_void_p()
+        _check(lib().lelph_nc4_create(os.fsencode(path), C.byref(h)), "lelph_nc4_create")
+        self.h = h
+        self.types = {}
+
+    def def_var(self, name, xtype, dims, dim_names, chunks=None):
+        d, dp = _i64(dims)
+        names = (C.c_char_p * len(dim_names))(*[n.encode() for n in dim_names])
+        if chunks is not None:
+            c, cp = _i64(chunks)
+        else:
+            c, cp = None, None
+        _check(lib().lelph_nc4_def_var(self.h, name.encode(), xtype, len(d), dp, names, cp),
+               f"def_var {name}")
+        self.types[name] = xtype
+
+    def put(self, name, data, start=None, count=None):
+        a = np.ascontiguousarray(data, _DTYPES[self.types[name]])
+        if start is None:
+            _check(lib().lelph_nc4_put_vara(self.h, name.encode(), None, None,
+                                            a.ctypes.data_as(C.c_void_p)), f"put {name}")
+            return
+        s, sp = _i64(start)
+        c, cp = _i64(count)
+        _check(lib().lelph_nc4_put_vara(self.h, name.encode(), sp, cp,
+                                        a.ctypes.data_as(C.c_void_p)), f"put {name}")
+
+    def close(self):
+        if self.h:
+            _check(lib().lelph_nc4_close(self.h), "lelph_nc4_close")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def elph_chunk_nk(nk, nmodes, nspin, nb):
+    """elph_mat chunk along k (elph_driver.c:163-177)."""
+    c = NC4_DEFAULT_CHUNK_KB * 1024 // (16 * nmodes * nspin * nb * nb)
+    return 1 if c == 0 else min(c, nk)
+
+
+def dmats_chunk_nk(nk, nspin, nb):
+    """Dmats chunk along k (compute_dmats.c:35-47)."""
+    c = NC4_DEFAULT_CHUNK_KB * 1024 // (16 * nspin * nb * nb)
+    return 1 if c == 0 else min(c, nk)
+
+
+def _cplx_view(a):
+    """complex array -> trailing re_im axis of doubles"""
+    a = np.ascontiguousarray(a, np.complex128)
+    return a.view(np.float64).reshape(a.shape + (2,))
+
+
+class ElphWriter:
+    """ndb.elph as elph_driver writes it."""
+
+    def __init__(self, path, nq_bz, nk_bz, nmodes, nspin, nb):
+        self.f = NC4File(path)
+        self.shape = (nq_bz, nk_bz, nmodes, nspin, nb)
+        self.f.def_var("POLARIZATION_VECTORS", NC_DOUBLE, (nq_bz, nmodes, nmodes // 3, 3, 2),
+                       ("nq", "nmodes", "atom", "pol", "re_im"))
+        self.f.def_var("FREQ", NC_DOUBLE, (nq_bz, nmodes), ("nq", "nmodes"))
+        self.f.def_var("elph_mat", NC_DOUBLE, (nq_bz, nk_bz, nmodes, nspin, nb, nb, 2),
+                       ("nq", "nk", "nmodes", "nspin", "initial_band", "final_band_PH_abs",
+                        "re_im"),
+                       (1, elph_chunk_nk(nk_bz, nmodes, nspin, nb), nmodes, nspin, nb, nb, 2))
+
+    def put_phonon(self, qpos, eig, freq):
+        nm = self.shape[2]
+        e = _cplx_view(np.asarray(eig).reshape(nm, nm // 3, 3))
+        self.f.put("POLARIZATION_VECTORS", e, (qpos, 0, 0, 0, 0), (1, nm, nm // 3, 3, 2))
+        self.f.put("FREQ", np.asarray(freq, np.float64), (qpos, 0), (1, nm))
+
+    def put_elph(self, qpos, k0, g):
+        """g (nk_slab, nmodes, nspin, nb, nb) complex at [qpos, k0:k0+nk_slab]"""
+        g = _cplx_view(g)
+        n = g.shape[0]
+        nm, ns, nb = self.shape[2:]
+        self.f.put("elph_mat", g, (qpos, k0, 0, 0, 0, 0, 0), (1, n, nm, ns, nb, nb, 2))
+
+    def write_basic_data(self, kpoints_crys, qpoints_crys, qpoints_ibz, qmap, kmap, bands,
+                         syms, kernel="dfpt", convention="standard", epsilon=None,
+                         Zborn=None, Qpole=None, natom=None):
+        """write_basic_data.c:15-215; syms = [(R, tau, time_rev)]"""
+        f = self.f
+        nk, nq = len(kpoints_crys), len(qpoints_crys)
+        f.def_var("kpoints", NC_DOUBLE, (nk, 3), ("nk", "pol"))
+        f.put("kpoints", kpoints_crys)
+        f.def_var("qpoints", NC_DOUBLE, (nq, 3), ("nq", "pol"))
+        f.put("qpoints", qpoints_crys)
+        f.def_var("qpoints_iBZ", NC_DOUBLE, (len(qpoints_ibz), 3), ("nq_iBZ", "pol"))
+        f.put("qpoints_iBZ", qpoints_ibz)
+        f.def_var("qmap", NC_INT, (nq, 2), ("nq", "dim_two"))
+        f.put("qmap", qmap)
+        f.def_var("kmap", NC_INT, (nk, 2), ("nk", "dim_two"))
+        f.put("kmap", kmap)
+        f.def_var("bands", NC_INT, (2,), ("two_scalars",))
+        f.put("bands", bands)
+        nsym = len(syms)
+        f.def_var("number_of_phonon_symmetries", NC_INT, (1,), ("scalar",))
+        f.put("number_of_phonon_symmetries", [nsym])
+        f.def_var("time_reversal_phonon", NC_INT, (1,), ("scalar",))
+        f.put("time_reversal_phonon", [int(any(s[2] for s in syms))])
+        f.def_var("symmetry_matrices", NC_DOUBLE, (nsym, 3, 3), ("nsym_ph", "pol", "pol"))
+        f.put("symmetry_matrices", np.array([np.asarray(s[0]) for s in syms]))
+        f.def_var("fractional_translation", NC_DOUBLE, (nsym, 3), ("nsym_ph", "pol"))
+        f.put("fractional_translation", np.array([np.asarray(s[1]) for s in syms]))
+        for name, s in (("kernel", kernel), ("convention", convention)):
+            b = np.frombuffer(s.encode() + b"\0", np.uint8)
+            f.def_var(name, NC_CHAR, (len(b),), (f"{name}_str_size",))
+            f.put(name, b)
+        na = natom if natom is not None else self.shape[2] // 3
+        z = np.zeros(27 * na)
+        f.def_var("epsilon", NC_DOUBLE, (3, 3), ("pol", "pol"))
+        f.put("epsilon", z[:9] if epsilon is None else epsilon)
+        f.def_var("Born_charges", NC_DOUBLE, (na, 3, 3), ("natoms", "pol", "pol"))
+        f.put("Born_charges", z[:9 * na] if Zborn is None else Zborn)
+        f.def_var("Quadrupole_tensor", NC_DOUBLE, (na, 3, 3, 3), ("natoms", "pol", "pol", "pol"))
+        f.put("Quadrupole_tensor", z if Qpole is None else Qpole)
+
+    def close(self):
+        self.f.close()
+
+
+def write_dmats(path, D):
+    """ndb.Dmats (compute_dmats.c:20-131): D (nsym_ph, nk, nspin, nb, nb)."""
+    D = np.asarray(D, np.complex128)
+    nsym, nk, ns, nb, _ = D.shape
+    with NC4File(path) as f:
+        f.def_var("Dmats", NC_DOUBLE, (nsym, nk, ns, nb, nb, 2),
+                  ("nsym_ph", "nkpts", "nspin", "Rk_band", "k_band", "re_im"),
+                  (1, dmats_chunk_nk(nk, ns, nb), ns, nb, nb, 2))
+        f.put("Dmats", _cplx_view(D))
+
+
+def write_elph_db(ctx, path, kernel="dfpt", kminusq=False, non_loc=True):
+    """compute_and_write_elphq for every q of the context's system (each q
+    its own star): dV_q set on the device, g for all full-BZ k in one call,
+    slabs, polarisation vectors and the basic data written to `path`
+    (elph_driver.c:200-342, compute_elph_matq.c:135-150)."""
+    s = ctx.sys
+    nq, nk = len(s.qpts), len(s.kpt_bz_crys)
+    w = ElphWriter(path, nq, nk, s.nmodes, s.nspin, s.nbnds)
+    try:
+        for iq, q in enumerate(s.qpts):
+            ctx.set_dvscf(q, s.eigvec(iq), s.dvscf(iq) if kernel.startswith("dfpt") else None,
+                          kernel=kernel)
+            g = ctx.compute_elphq(q, non_loc=non_loc, kminusq=kminusq)
+            w.put_phonon(iq, s.eigvec(iq), np.zeros(s.nmodes))
+            w.put_elph(iq, 0, g)
+        qmap = np.array([[i, 0] for i in range(nq)], np.int32)
+        w.write_basic_data(s.kpt_bz_crys, np.array(s.qpts), np.array(s.qpts), qmap,
+                           np.asarray(s.kmap, np.int32), [1, s.nbnds], s.syms, kernel=kernel,
+                           convention="yambo" if kminusq else "standard", natom=s.natom)
+    finally:
+        w.close()

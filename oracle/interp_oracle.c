@@ -14,6 +14,7 @@
 #include <complex.h>
 #include <math.h>
 #include <stdbool.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -676,4 +677,70 @@ void orc_sorted_qpts_idxs(int64_t nq, const double* q, int64_t* idx)
     qsort(p, nq, sizeof(double*), qcmp);
     for (int64_t i = 0; i < nq; ++i) idx[i] = (p[i] - q) / 3;
     free(p);
+}
+
+/* read_dvscf_qe, src/io/qe/read_dvscf.c:17-172: file sets (Nz, Ny, Nx) ->
+ * (Nx, Ny, Nz), then eig @ pats^H @ dV; out (nmodes, nmag, Nx, Ny, Nz) */
+int orc_read_dvscf_qe(const char* path, const lelph_lattice* lat, const double* eigd,
+                      const double* patd, double* outd)
+{
+    const int64_t* N = lat->fft_dims;
+    const int64_t nf = N[0] * N[1] * N[2], nm = lat->nmodes, nmag = lat->nmag;
+    FILE* fp = fopen(path, "rb");
+    if (!fp) return LELPH_ERR_ARG;
+    cplx* buf = malloc(sizeof(cplx) * nf);
+    cplx* out = (cplx*)outd;
+    for (int64_t s = 0; s < nm * nmag; ++s)
+    {
+        if (fread(buf, sizeof(cplx), nf, fp) != (size_t)nf)
+        {
+            free(buf);
+            fclose(fp);
+            return LELPH_ERR_ARG;
+        }
+        cplx* d = out + s * nf;
+        for (int64_t iy = 0; iy < N[1]; ++iy)
+            for (int64_t ix = 0; ix < N[0]; ++ix)
+                for (int64_t iz = 0; iz < N[2]; ++iz)
+                    d[(ix * N[1] + iy) * N[2] + iz] = buf[(iz * N[1] + iy) * N[0] + ix];
+    }
+    free(buf);
+    fclose(fp);
+    const cplx* eig = (const cplx*)eigd;
+    const cplx* pat = (const cplx*)patd;
+    cplx* ep = malloc(sizeof(cplx) * nm * nm);
+    for (int64_t i = 0; i < nm; ++i)
+        for (int64_t j = 0; j < nm; ++j)
+        {
+            cplx a = 0.0;
+            for (int64_t k = 0; k < nm; ++k) a += eig[i * nm + k] * conj(pat[j * nm + k]);
+            ep[i * nm + j] = a;
+        }
+    orc_dvscf_change_basis(outd, (const double*)ep, 1, nm, nmag, nf, 'N');
+    free(ep);
+    return 0;
+}
+
+/* write_dvscf_qe, src/io/qe/write_dyn_dvscf_qe.c:99-185 */
+int orc_write_dvscf_qe(const char* path, const lelph_lattice* lat, const double* dvd)
+{
+    const int64_t* N = lat->fft_dims;
+    const int64_t nf = N[0] * N[1] * N[2], ns = (int64_t)lat->nmodes * lat->nmag;
+    const cplx* dv = (const cplx*)dvd;
+    FILE* fp = fopen(path, "wb");
+    if (!fp) return LELPH_ERR_ARG;
+    cplx* buf = malloc(sizeof(cplx) * nf);
+    int st = 0;
+    for (int64_t s = 0; s < ns && !st; ++s)
+    {
+        const cplx* d = dv + s * nf;
+        for (int64_t iy = 0; iy < N[1]; ++iy)
+            for (int64_t ix = 0; ix < N[0]; ++ix)
+                for (int64_t iz = 0; iz < N[2]; ++iz)
+                    buf[(iz * N[1] + iy) * N[0] + ix] = d[(ix * N[1] + iy) * N[2] + iz];
+        if (fwrite(buf, sizeof(cplx), nf, fp) != (size_t)nf) st = LELPH_ERR_ARG;
+    }
+    free(buf);
+    if (fclose(fp)) st = LELPH_ERR_ARG;
+    return st;
 }

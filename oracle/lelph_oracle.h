@@ -141,6 +141,10 @@ int orc_fft_R2q_dvscf(const double* dataR, const double* q, const int64_t* qgrid
                       const int64_t* degen, double* dataq);
 int orc_find_qpt_grid(int64_t nq, const double* q, int64_t* grid);
 void orc_sorted_qpts_idxs(int64_t nq, const double* q, int64_t* idx);
+/* QE dvscf files (src/io/qe/read_dvscf.c, write_dyn_dvscf_qe.c) */
+int orc_read_dvscf_qe(const char* path, const lelph_lattice* lat, const double* eig,
+                      const double* pats, double* out);
+int orc_write_dvscf_qe(const char* path, const lelph_lattice* lat, const double* dvscf);
 
 #ifdef __cplusplus
 }

@@ -111,6 +111,9 @@ def lib():
                                         i64p, C.c_int64, i64p, c_double_p]
         L.orc_find_qpt_grid.argtypes = [C.c_int64, c_double_p, i64p]
         L.orc_sorted_qpts_idxs.argtypes = [C.c_int64, c_double_p, i64p]
+        L.orc_read_dvscf_qe.argtypes = [C.c_char_p, C.POINTER(Lattice), c_double_p, c_double_p,
+                                        c_double_p]
+        L.orc_write_dvscf_qe.argtypes = [C.c_char_p, C.POINTER(Lattice), c_double_p]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_inner_threads.argtypes = [C.c_int]
         _lib = L

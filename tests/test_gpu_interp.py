@@ -218,3 +218,39 @@ def test_interp_errors():
             ctx.interp_dvscf(rng.uniform(size=(1, 3)))  # not finished
     finally:
         ctx.close()
+
+
+def test_dvscf_qe_files(tmp_path):
+    """read_dvscf_qe / write_dvscf_qe / set_dvscf_file through the device
+    transpose, against the oracle"""
+    s = synth.tiny(nbnds=4, nmag=2, nspin=2)
+    rng = np.random.default_rng(12)
+    sh = (s.nmodes, s.nmag) + tuple(s.fft_dims)
+    dv = rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
+    f1 = str(tmp_path / "dvscf1")
+    np.ascontiguousarray(dv.transpose(0, 1, 4, 3, 2)).tofile(f1)
+    eig = s.eigvec(0)
+    pats, _ = np.linalg.qr(rng.standard_normal((s.nmodes,) * 2)
+                           + 1j * rng.standard_normal((s.nmodes,) * 2))
+    pats = np.ascontiguousarray(pats)
+    o = O.OracleSystem(s)
+    ref = np.empty(sh, np.complex128)
+    O.check(O.lib().orc_read_dvscf_qe(f1.encode(), o.lat, O.dptr(np.ascontiguousarray(eig)),
+                                      O.dptr(pats), O.dptr(ref)))
+    ctx = lelph.Context(s, "fp64")
+    say("case: QE dvscf files")
+    try:
+        got = ctx.read_dvscf_qe(f1, eig, pats)
+        assert rel_err(got, ref) < TOL64
+        f2 = str(tmp_path / "dvscf2")
+        ctx.write_dvscf_qe(f2, dv)
+        assert open(f1, "rb").read() == open(f2, "rb").read()
+        # the file-fed potential gives the same g as the host-fed one
+        ctx.set_dvscf_file(s.qpts[0], eig, f1, pats, kernel="dfpt")
+        pr = synth.kq_pairs(s, 0)[1]
+        g_file = ctx.gkq(*pr, non_loc=True)
+        ctx.set_dvscf(s.qpts[0], eig, dv, patterns=pats, kernel="dfpt")
+        g_host = ctx.gkq(*pr, non_loc=True)
+        assert rel_err(g_file, g_host) < 1e-13
+    finally:
+        ctx.close()

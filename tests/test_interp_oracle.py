@@ -257,3 +257,32 @@ def test_interpolation_exact_on_coarse_grid(with_lr):
         rot = eigs[i] * np.repeat(masses, 3)[None, :]
         ref = np.einsum("va,vm...->am...", rot.conj(), dvs[i])
         assert rel_err(got, ref) < 2e-6, i
+
+
+def test_dvscf_qe_files(tmp_path):
+    """read_dvscf_qe / write_dvscf_qe of the oracle against numpy's view of
+    the QE layout: sets of (Nz, Ny, Nx) complex128, x fastest."""
+    s = synth.tiny(nmag=2, nspin=2, fft=(4, 5, 6), with_wfc=False)
+    from letzelphc_amd.abi import pack_system
+    pk = pack_system(s)
+    rng = np.random.default_rng(6)
+    sh = (s.nmodes, s.nmag) + tuple(s.fft_dims)
+    dv = rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
+    f1 = str(tmp_path / "dv1")
+    np.ascontiguousarray(dv.transpose(0, 1, 4, 3, 2)).tofile(f1)
+    eye = np.ascontiguousarray(np.eye(s.nmodes, dtype=np.complex128))
+    out = np.empty(sh, np.complex128)
+    O.check(O.lib().orc_read_dvscf_qe(f1.encode(), C.byref(pk.lattice), dptr(eye), dptr(eye),
+                                      dptr(out)))
+    assert np.array_equal(out, dv)
+    f2 = str(tmp_path / "dv2")
+    O.check(O.lib().orc_write_dvscf_qe(f2.encode(), C.byref(pk.lattice), dptr(dv)))
+    assert open(f1, "rb").read() == open(f2, "rb").read()
+    # patterns: eig @ pats^H @ dV
+    e = rng.standard_normal((s.nmodes,) * 2) + 1j * rng.standard_normal((s.nmodes,) * 2)
+    p, _ = np.linalg.qr(rng.standard_normal((s.nmodes,) * 2)
+                        + 1j * rng.standard_normal((s.nmodes,) * 2))
+    e, p = np.ascontiguousarray(e), np.ascontiguousarray(p)
+    O.check(O.lib().orc_read_dvscf_qe(f1.encode(), C.byref(pk.lattice), dptr(e), dptr(p),
+                                      dptr(out)))
+    assert rel_err(out, np.einsum("ab,bm...->am...", e @ p.conj().T, dv)) < 1e-13
