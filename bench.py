@@ -97,6 +97,67 @@ def cpu_baseline(sys_, pairs, dv, cores, modes=(2, 6)):
                        f"{sys_.nmodes} modes: {t_full:.1f} s per pair")}
 
 
+def aux_measurements(s, ctx):
+    """Timings of the rows around the hot path, on their own workloads:
+    the dV_q Fourier interpolation (c0 cell, 24^3, coarse 4x4x4 q grid with a
+    Born-charge long-range term, eps = 12, onto the fine 12x12x12 grid), the
+    electronic D-matrices and the q-star rotation at the headline sizes."""
+    from letzelphc_amd import lelph, synth
+    from letzelphc_amd.abi import LrParams
+    out = {}
+    c0 = synth.config("c0", with_wfc=False)
+    g4 = np.stack(np.meshgrid(*[np.arange(4) / 4] * 3, indexing="ij"), -1).reshape(-1, 3)
+    c0.qpts = [q for q in g4]
+    na = c0.natom
+    zb = np.stack([2.0 * np.eye(3), -2.0 * np.eye(3)])[:na]
+    lr = LrParams(epsilon=12.0 * np.eye(3), Zborn=zb, Zvals=np.full(na, 4.0), eta_bare=1.0,
+                  eta_induced=1.0, EcutRy=30.0, nmags_add=(1, 0, 0, 0))
+    ic = lelph.Context(c0, "fp64", upload=False)
+    try:
+        t0 = time.perf_counter()
+        ic.interp_begin((4, 4, 4), lr=lr)
+        for iq, q in enumerate(c0.qpts):
+            ic.interp_add_coarse(q, c0.eigvec(iq), c0.dvscf(iq))
+        ic.interp_finish()
+        t_coarse = time.perf_counter() - t0
+        fine = np.stack(np.meshgrid(*[np.arange(12) / 12] * 3, indexing="ij"), -1).reshape(-1, 3)
+        t0 = time.perf_counter()
+        ic.interp_dvscf(fine)
+        t_fine = time.perf_counter() - t0
+        tm = ic.interp_timings()
+        cols = c0.nmodes * c0.nmag * c0.nfft
+        nR = 64
+        flops = 8.0 * len(fine) * nR * cols
+        byts = 16.0 * (nR + len(fine)) * cols
+        out["interp_c4"] = {
+            "workload": "c0 cell 24^3, coarse 4x4x4 q (64) with long range (Z* = +-2, "
+                        "eps = 12) -> fine 12x12x12 q (1728), FP64",
+            "coarse_side_s": t_coarse, "fine_wall_s_incl_d2h": t_fine,
+            "device_ms": tm["total_ms"], "r2q_ms": tm["r2q_ms"],
+            "longrange_ms": tm["longrange_ms"], "weights_ms": tm["weights_ms"],
+            "fine_q_per_s_device": len(fine) / (tm["total_ms"] / 1e3),
+            "r2q_fp64_tflops": flops / (tm["r2q_ms"] / 1e3) / 1e12,
+            "r2q_frac_fp64_peak": flops / (tm["r2q_ms"] / 1e3) / 1e12 / 78.6,
+            "r2q_algorithmic_GBps": byts / (tm["r2q_ms"] / 1e3) / 1e9}
+    finally:
+        ic.close()
+    # D-matrices and q-star rotation at the headline sizes
+    t0 = time.perf_counter()
+    R, tau, tr = s.syms[0]
+    reps = 3
+    for _ in range(reps):
+        D = ctx.electronic_reps(R, tau, tr, 0)
+    out["electronic_reps_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    rng = np.random.default_rng(0)
+    nm, ns, nb = s.nmodes, s.nspin, s.nbnds
+    g = rng.standard_normal((nm, ns, nb, nb)) + 1j * rng.standard_normal((nm, ns, nb, nb))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.elph_q_rotate(D, g, D, False)
+    out["elph_q_rotate_ms_incl_transfers"] = (time.perf_counter() - t0) / reps * 1e3
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -108,6 +169,8 @@ def main():
                     help="headline variant: auto, fp32, bf16x3, bf16x6")
     ap.add_argument("--nk", type=int, default=0, help="pairs per step (0: all k)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-aux", action="store_true",
+                    help="skip the interpolation / D-matrix / q-rotation timings")
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (HBM counter calibration)")
@@ -183,6 +246,14 @@ def main():
         if m:
             traffic = int(m.group(1))
 
+    aux = None
+    if not args.no_aux:
+        try:
+            aux = aux_measurements(s, ctx)
+            log(f"aux: {json.dumps(aux)}")
+        except Exception as e:  # never hides the headline
+            log(f"aux measurements failed: {e}")
+
     cpu = None
     if not args.no_cpu:
         cores = args.cpu_cores or min(16, os.cpu_count() or 1)
@@ -220,6 +291,7 @@ def main():
                                         "profiles/k_contract_split_pc_bf16x3_c2.md")
                      if traffic else None},
         "cpu_baseline": cpu,
+        "aux": aux,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

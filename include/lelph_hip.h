@@ -339,6 +339,12 @@ int lelph_interp_dvscf(lelph_ctx* ctx, int64_t nq, const double* qpts, double* d
 int lelph_interp_set_dvscf(lelph_ctx* ctx, const double qpt[3], const double* eigVec,
                            int kernel);
 
+/* Device time of the last lelph_interp_dvscf, summed over its q chunks,
+ * milliseconds: [0] Wigner-Seitz weights, [1] R -> q phase GEMM with
+ * e^{-iqr}, [2] long range added back, [3] total, [4] number of q.
+ * No counterpart in the reference (timing of this library's kernels). */
+int lelph_interp_timings(lelph_ctx* ctx, double* ms, int n);
+
 /* Device time of the last elph_local / nonlocal / gkq / batch call, summed
  * over its pairs (HIP events on the context stream), milliseconds:
  *   [0] sphere scatter + backward FFTs of psi_k and psi_k+q

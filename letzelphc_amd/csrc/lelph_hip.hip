@@ -150,6 +150,9 @@ struct lelph_ctx
     std::vector<double> ip_eps, ip_zb, ip_qp, ip_zv;
     DBuf ip_dvq, ip_dvR, ip_last, ip_tmp, ip_out, ip_ws, ip_off, ip_W, ip_P, ip_qs, ip_rot,
         ip_eig, lr_box, lr_box2, lr_q, lr_zv, lr_zb, lr_qp, rot_idx, rot_sr;
+    // device time of the last lelph_interp_dvscf: weights, R -> q, long range, total, nq
+    hipEvent_t ip_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double ip_ms[5] = {0, 0, 0, 0, 0};
     DBuf nlK, nlKp, Y_K, Y_Kp, wr_K, wr_Kp, beta_K, beta_Kp, bb_K, bb_Kp, bbpart, ebuf;
 
     // per-pair events: [0] start, [1] boxes built, [2] contraction done,
@@ -747,6 +750,8 @@ static void free_ctx(lelph_ctx* c)
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : c->evs)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->ip_ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1874,11 +1879,15 @@ static int ip_r2q(lelph_ctx* c, int64_t nq, const double* qs, z64* out)
     LELPH_TRY(c->ip_W.ensure(sizeof(z64) * nq * nR * na));
     HIP_TRY(hipMemcpyAsync(c->ip_qs.p, qs, sizeof(double) * 3 * nq, hipMemcpyHostToDevice,
                            c->stream));
+    for (auto& e : c->ip_ev)
+        if (!e) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventRecord(c->ip_ev[0], c->stream));
     hipLaunchKernelGGL(k_ws_weights, dim3((unsigned)cdiv(nq * nR * na, 256)), dim3(256), 0,
                        c->stream, c->ip_qs.as<double>(), (int)nq, (int)c->ip_grid[0],
                        (int)c->ip_grid[1], (int)c->ip_grid[2], na, c->ip_off.as<int64_t>(),
                        c->ip_ws.as<int>(), c->ip_W.as<z64>());
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ip_ev[1], c->stream));
     constexpr int MB = 16;
     hipLaunchKernelGGL((k_phase_gemm<MB>),
                        dim3((unsigned)(na * cdiv(cpa, 256)), (unsigned)cdiv(nq, MB)), dim3(256), 0,
@@ -1886,7 +1895,17 @@ static int ip_r2q(lelph_ctx* c, int64_t nq, const double* qs, z64* out)
                        cpa, out, c->ip_qs.as<double>(), (int)L.fft_dims[0], (int)L.fft_dims[1],
                        (int)L.fft_dims[2]);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ip_ev[2], c->stream));
     if (c->ip_has_lr) LELPH_TRY(lr_apply(c, c->ip_lr, qs, nq, nullptr, out, 1.0));
+    HIP_TRY(hipEventRecord(c->ip_ev[3], c->stream));
+    HIP_TRY(hipEventSynchronize(c->ip_ev[3]));
+    float t[3];
+    for (int i = 0; i < 3; ++i) HIP_TRY(hipEventElapsedTime(&t[i], c->ip_ev[i], c->ip_ev[i + 1]));
+    c->ip_ms[0] += t[0];
+    c->ip_ms[1] += t[1];
+    c->ip_ms[2] += t[2];
+    c->ip_ms[3] += t[0] + t[1] + t[2];
+    c->ip_ms[4] += (double)nq;
     return LELPH_OK;
 }
 
@@ -1896,6 +1915,7 @@ int lelph_interp_dvscf(lelph_ctx* c, int64_t nq, const double* qpts, double* dvs
     if (!c->ip_ready) return LELPH_ERR_STATE;
     const lelph_lattice& L = c->lat;
     const size_t per = sizeof(z64) * (size_t)L.nmodes * L.nmag * c->nf;
+    for (double& t : c->ip_ms) t = 0.0;
     // chunks of at most 2 GiB of output and 65535 grid rows
     int64_t chunk = std::max<int64_t>(1, ((int64_t)2 << 30) / (int64_t)per);
     chunk = std::min<int64_t>(chunk, 65535);
@@ -2091,6 +2111,13 @@ int lelph_internal_grid(const lelph_ctx* c, int64_t* nbz, const double** kcrys, 
     *nbz = c->lat.nkpts_BZ;
     *kcrys = c->kpt_bz.data();
     *kmap = c->kmap.data();
+    return LELPH_OK;
+}
+
+int lelph_interp_timings(lelph_ctx* c, double* ms, int n)
+{
+    if (!c || !ms || n < 0) return LELPH_ERR_ARG;
+    for (int i = 0; i < n && i < 5; ++i) ms[i] = c->ip_ms[i];
     return LELPH_OK;
 }
 

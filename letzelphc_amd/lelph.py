@@ -35,7 +35,7 @@ EXPORTS = [
     "lelph_rotate_dvscf", "lelph_wigner_seitz", "lelph_interp_begin", "lelph_interp_add_coarse",
     "lelph_interp_add_star", "lelph_interp_finish", "lelph_interp_dvscf",
     "lelph_interp_set_dvscf", "lelph_read_dvscf_qe", "lelph_set_dvscf_file",
-    "lelph_write_dvscf_qe",
+    "lelph_write_dvscf_qe", "lelph_interp_timings",
 ]
 
 
@@ -108,6 +108,7 @@ def lib():
         L.lelph_set_dvscf_file.argtypes = [C.c_void_p, c_double_p, c_double_p, C.c_char_p,
                                            c_double_p, C.c_int]
         L.lelph_write_dvscf_qe.argtypes = [C.c_void_p, C.c_char_p, c_double_p]
+        L.lelph_interp_timings.argtypes = [C.c_void_p, c_double_p, C.c_int]
         _lib = L
     return _lib
 
@@ -367,6 +368,12 @@ class Context:
         _check(lib().lelph_interp_dvscf(self.h, len(q), dptr(q), dptr(out)),
                "lelph_interp_dvscf")
         return out
+
+    def interp_timings(self) -> dict:
+        t = np.zeros(5)
+        _check(lib().lelph_interp_timings(self.h, dptr(t), 5), "lelph_interp_timings")
+        return {"weights_ms": t[0], "r2q_ms": t[1], "longrange_ms": t[2], "total_ms": t[3],
+                "nq": int(t[4])}
 
     def interp_set_dvscf(self, qpt, eig, kernel: str = "dfpt"):
         q = np.ascontiguousarray(qpt, np.float64)
