@@ -339,6 +339,41 @@ int lelph_interp_dvscf(lelph_ctx* ctx, int64_t nq, const double* qpts, double* d
 int lelph_interp_set_dvscf(lelph_ctx* ctx, const double qpt[3], const double* eigVec,
                            int kernel);
 
+/* ------------------------------------------------------------------------
+ * Fourier interpolation of dynamical matrices
+ * (src/interpolation/interpolation_driver.c:190-530 coarse side, :855-1000
+ * fine side; src/phonon/).  3-D and 2-D long range (dipole, quadrupole),
+ * acoustic sum rule none (0) or simple (1, src/phonon/asr.c:294-318).
+ * ---------------------------------------------------------------------- */
+
+/* Starts it on the context's lattice: coarse grid qgrid, atomic masses
+ * (natom, or NULL), lr (epsilon, Zborn, Qpole used; NULL: no long range),
+ * eta_ph, the cut-off of the long-range G sum (EcutRy, Ry; the reference
+ * uses 30) and the sum rule.  3 natom <= 64. */
+int lelph_dyn_interp_begin(lelph_ctx* ctx, const int64_t qgrid[3], const double* masses,
+                           const lelph_longrange* lr, double eta_ph, double EcutRy, int asr);
+
+/* One coarse q (crystal): frequencies omega (nmodes) and polarisation
+ * vectors (nmodes, 3 natom) as read from the dyn file; pol_vecs_to_dyn
+ * (src/phonon/pol_vecs_to_dyn.c:8-40) and the long range removed
+ * (interpolation_driver.c:446-481). */
+int lelph_dyn_interp_add_coarse(lelph_ctx* ctx, const double qpt[3], const double* omega,
+                                const double* pol_vecs);
+
+/* q -> R, mass factors and the sum rule (interpolation_driver.c:483-532). */
+int lelph_dyn_interp_finish(lelph_ctx* ctx);
+
+/* nq fine q (crystal): fft_R2q_dyn (interpolation_utilities.c:78-163), long
+ * range added back, and (when omega_out or eig_out is given) the Hermitian
+ * part diagonalised on the device (interpolation_driver.c:945-992):
+ * dyn_out (nq, nmodes, nmodes) mass-normalised dynamical matrices,
+ * omega_out (nq, nmodes) sign(w) sqrt|w| ascending, eig_out (nq, nmodes,
+ * 3 natom) eigenvector rows / sqrt(M).  Each may be NULL.  Eigenvectors are
+ * unique up to a phase per mode (a unitary mix within degenerate modes);
+ * this solver does not reproduce LAPACK's choice. */
+int lelph_dyn_interp(lelph_ctx* ctx, int64_t nq, const double* qpts, double* dyn_out,
+                     double* omega_out, double* eig_out);
+
 /* Device time of the last lelph_interp_dvscf, summed over its q chunks,
  * milliseconds: [0] Wigner-Seitz weights, [1] R -> q phase GEMM with
  * e^{-iqr}, [2] long range added back, [3] total, [4] number of q.

@@ -24,6 +24,7 @@
 #include "boxes.hpp"
 #include "dmats.hpp"
 #include "interp.hpp"
+#include "dyn.hpp"
 
 #include <type_traits>
 #include "wfc.hpp"
@@ -150,6 +151,15 @@ struct lelph_ctx
     std::vector<double> ip_eps, ip_zb, ip_qp, ip_zv;
     DBuf ip_dvq, ip_dvR, ip_last, ip_tmp, ip_out, ip_ws, ip_off, ip_W, ip_P, ip_qs, ip_rot,
         ip_eig, lr_box, lr_box2, lr_q, lr_zv, lr_zb, lr_qp, rot_idx, rot_sr;
+    // dynamical-matrix interpolation
+    bool dy_on = false, dy_ready = false;
+    int64_t dy_grid[3] = {0, 0, 0}, dy_nq = 0, dy_Gg[3] = {0, 0, 0};
+    double dy_eta = 1.0;
+    int dy_asr_kind = 0;
+    std::vector<double> dy_mass, dy_eps, dy_zb, dy_qp;
+    std::vector<char> dy_have;
+    DBuf dy_dyn, dy_fc, dy_X1, dy_X2, dy_P1, dy_P2, dy_asr, dy_ws, dy_off, dy_qs, dy_qs2, dy_out,
+        dy_freq, dy_eig, dy_zbd, dy_qpd, dy_massd, dy_P;
     // device time of the last lelph_interp_dvscf: weights, R -> q, long range, total, nq
     hipEvent_t ip_ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double ip_ms[5] = {0, 0, 0, 0, 0};
@@ -745,7 +755,10 @@ static void free_ctx(lelph_ctx* c)
                     &c->qr_dr, &c->qr_g, &c->qr_t, &c->qr_out, &c->ip_dvq, &c->ip_dvR,
                     &c->ip_last, &c->ip_tmp, &c->ip_out, &c->ip_ws, &c->ip_off, &c->ip_W,
                     &c->ip_P, &c->ip_qs, &c->ip_rot, &c->ip_eig, &c->lr_box, &c->lr_box2,
-                    &c->lr_q, &c->lr_zv, &c->lr_zb, &c->lr_qp, &c->rot_idx, &c->rot_sr};
+                    &c->lr_q, &c->lr_zv, &c->lr_zb, &c->lr_qp, &c->rot_idx, &c->rot_sr,
+                    &c->dy_dyn, &c->dy_fc, &c->dy_X1, &c->dy_X2, &c->dy_P1, &c->dy_P2,
+                    &c->dy_asr, &c->dy_ws, &c->dy_off, &c->dy_qs, &c->dy_qs2, &c->dy_out,
+                    &c->dy_freq, &c->dy_eig, &c->dy_zbd, &c->dy_qpd, &c->dy_massd, &c->dy_P};
     for (DBuf* b : bufs) b->release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -2102,6 +2115,282 @@ int lelph_write_dvscf_qe(lelph_ctx* c, const char* dvscf_file, const double* dvs
     const size_t w = std::fwrite(host.data(), sizeof(z64), host.size(), fp);
     const int cl = std::fclose(fp);
     return (w == host.size() && cl == 0) ? LELPH_OK : LELPH_ERR_ARG;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Fourier interpolation of dynamical matrices                               */
+/* ------------------------------------------------------------------------ */
+
+// long range of dyn (nq, nm, nm) on the device += sign * LR (+ ASR term)
+// (add_ph_dyn_long_range, src/phonon/dyn_long_range.c:27-404)
+static int dyn_lr_apply(lelph_ctx* c, const double* qs, int64_t nq, int sign, bool use_mass,
+                        bool with_asr, z64* dyn)
+{
+    const lelph_lattice& L = c->lat;
+    const int na = L.natom, nm = 3 * na;
+    if (c->dy_eps.empty() || (c->dy_zb.empty() && c->dy_qp.empty())) return LELPH_OK;
+    if (L.dimension == '2')
+        for (int64_t i = 0; i < nq; ++i)
+            if (std::fabs(qs[3 * i + 2]) > LELPH_EPS) return LELPH_ERR_ARG;
+    DynLrGeom g{};
+    std::memcpy(g.blat, L.blat_vec, sizeof(g.blat));
+    const double* eps = c->dy_eps.data();
+    std::memcpy(g.ea, eps, sizeof(g.ea));
+    g.eps22 = eps[8];
+    g.zlat = L.alat_vec[8];
+    g.eta = c->dy_eta;
+    g.d2 = L.dimension == '2';
+    double factor = 4.0 * LELPH_RPI * LELPH_E2 / L.volume;
+    if (sign < 0) factor = -factor;
+    if (g.d2)
+    {
+        factor *= g.zlat / 2.0;
+        for (double& x : g.ea) x = 0.0;
+        g.ea[0] = 0.5 * g.zlat * (eps[0] - 1);
+        g.ea[1] = 0.5 * g.zlat * eps[1];
+        g.ea[3] = 0.5 * g.zlat * eps[3];
+        g.ea[4] = 0.5 * g.zlat * (eps[4] - 1);
+    }
+    g.factor = factor;
+    for (int i = 0; i < 3; ++i) g.Gg[i] = (int)c->dy_Gg[i];
+    g.GZ = g.d2 ? 1 : g.Gg[2];
+    g.natom = na;
+    g.has_zb = !c->dy_zb.empty();
+    g.has_qp = !c->dy_qp.empty();
+    g.use_mass = use_mass ? 1 : 0;
+    const int64_t nG = (int64_t)g.Gg[0] * g.Gg[1] * g.GZ;
+    const int64_t per_q = nG * nm * (int64_t)sizeof(z64);
+    int64_t chunk = std::max<int64_t>(1, ((int64_t)1 << 30) / per_q);
+    chunk = std::min<int64_t>({chunk, nq, 65535});
+    LELPH_TRY(c->dy_X1.ensure((size_t)(per_q * chunk)));
+    LELPH_TRY(c->dy_X2.ensure((size_t)(per_q * chunk)));
+    if (g.d2)
+    {
+        LELPH_TRY(c->dy_P1.ensure((size_t)(per_q * chunk)));
+        LELPH_TRY(c->dy_P2.ensure((size_t)(per_q * chunk)));
+    }
+    LELPH_TRY(c->dy_qs.ensure(sizeof(double) * 3 * nq));
+    HIP_TRY(hipMemcpyAsync(c->dy_qs.p, qs, sizeof(double) * 3 * nq, hipMemcpyHostToDevice,
+                           c->stream));
+    for (int64_t q0 = 0; q0 < nq; q0 += chunk)
+    {
+        const int64_t n = std::min(chunk, nq - q0);
+        hipLaunchKernelGGL(k_dyn_lr_vec, dim3((unsigned)cdiv(nG, 256), (unsigned)n), dim3(256), 0,
+                           c->stream, g, c->dy_qs.as<double>() + 3 * q0, c->d_apos,
+                           c->dy_zbd.as<double>(), c->dy_qpd.as<double>(),
+                           c->dy_massd.as<double>(), c->dy_X1.as<z64>(), c->dy_X2.as<z64>(),
+                           g.d2 ? c->dy_P1.as<z64>() : nullptr, g.d2 ? c->dy_P2.as<z64>() : nullptr);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_dyn_lr_acc, dim3((unsigned)n), dim3(256), 0, c->stream,
+                           c->dy_X1.as<z64>(), c->dy_X2.as<z64>(),
+                           g.d2 ? c->dy_P1.as<z64>() : nullptr, g.d2 ? c->dy_P2.as<z64>() : nullptr,
+                           nG, nm, factor, with_asr ? c->dy_asr.as<z64>() : nullptr,
+                           sign < 0 ? 1.0 : -1.0, c->dy_massd.as<double>(), g.use_mass,
+                           dyn + q0 * nm * nm);
+        HIP_TRY(hipGetLastError());
+    }
+    return LELPH_OK;
+}
+
+int lelph_dyn_interp_begin(lelph_ctx* c, const int64_t qgrid[3], const double* masses,
+                           const lelph_longrange* lr, double eta_ph, double EcutRy, int asr)
+{
+    if (!c || !qgrid || eta_ph <= 0 || asr < 0 || asr > 1) return LELPH_ERR_ARG;
+    for (int i = 0; i < 3; ++i)
+        if (qgrid[i] < 1 || qgrid[i] > 1024) return LELPH_ERR_ARG;
+    const lelph_lattice& L = c->lat;
+    const int na = L.natom, nm = 3 * na;
+    if (nm > DYN_MAX_NM) return LELPH_ERR_UNSUPPORTED;
+    if (lr && lr->Qpole && !lr->epsilon) return LELPH_ERR_ARG;
+    c->dy_on = c->dy_ready = false;
+    std::memcpy(c->dy_grid, qgrid, sizeof(c->dy_grid));
+    c->dy_nq = qgrid[0] * qgrid[1] * qgrid[2];
+    c->dy_eta = eta_ph;
+    c->dy_asr_kind = asr;
+    c->dy_mass.assign(masses ? masses : nullptr, masses ? masses + na : nullptr);
+    auto cp = [](std::vector<double>& v, const double* p, size_t n) {
+        v.assign(p ? p : nullptr, p ? p + n : nullptr);
+    };
+    cp(c->dy_eps, lr ? lr->epsilon : nullptr, 9);
+    cp(c->dy_zb, lr ? lr->Zborn : nullptr, (size_t)9 * na);
+    cp(c->dy_qp, lr ? lr->Qpole : nullptr, (size_t)27 * na);
+    lr_fft_box(EcutRy, L.blat_vec, c->dy_Gg); // Ggrid_phonon (interpolation_driver.c:420-421)
+    auto up = [&](DBuf& b, const std::vector<double>& v) -> int {
+        if (v.empty()) return LELPH_OK;
+        LELPH_TRY(b.ensure(sizeof(double) * v.size()));
+        HIP_TRY(hipMemcpy(b.p, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+        return LELPH_OK;
+    };
+    LELPH_TRY(up(c->dy_zbd, c->dy_zb));
+    LELPH_TRY(up(c->dy_qpd, c->dy_qp));
+    LELPH_TRY(up(c->dy_massd, c->dy_mass));
+    // Wigner-Seitz vectors of every (R, atom, atom) (interpolation_driver.c:204-207)
+    const int64_t nR = c->dy_nq;
+    std::vector<int64_t> degen((size_t)nR * na * na);
+    int64_t cap = 16 * nR * na * na, nws = 0;
+    std::vector<int64_t> ws;
+    for (;;)
+    {
+        ws.resize((size_t)3 * cap);
+        const int st = lelph_wigner_seitz(qgrid, L.alat_vec, LELPH_EPS6, c->atomic_pos.data(), na,
+                                          c->atomic_pos.data(), na, cap, ws.data(), degen.data(),
+                                          &nws);
+        if (st == LELPH_OK) break;
+        if (st != LELPH_ERR_RANGE) return st;
+        cap *= 4;
+    }
+    std::vector<int> wsi((size_t)3 * std::max<int64_t>(nws, 1));
+    for (int64_t i = 0; i < 3 * nws; ++i) wsi[i] = (int)ws[i];
+    std::vector<int64_t> off((size_t)nR * na * na + 1, 0);
+    for (int64_t i = 0; i < nR * na * na; ++i) off[i + 1] = off[i] + degen[i];
+    LELPH_TRY(c->dy_ws.ensure(sizeof(int) * wsi.size()));
+    LELPH_TRY(c->dy_off.ensure(sizeof(int64_t) * off.size()));
+    HIP_TRY(hipMemcpy(c->dy_ws.p, wsi.data(), sizeof(int) * wsi.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->dy_off.p, off.data(), sizeof(int64_t) * off.size(),
+                      hipMemcpyHostToDevice));
+    LELPH_TRY(c->dy_dyn.ensure(sizeof(z64) * c->dy_nq * nm * nm));
+    // long-range ASR correction (compute_dyn_lr_asr_correction,
+    // dyn_long_range.c:70-123): the q = 0 term without masses summed over b
+    std::vector<z64> asrv((size_t)9 * na, z64{0, 0});
+    if (!c->dy_eps.empty() && (!c->dy_zb.empty() || !c->dy_qp.empty()))
+    {
+        LELPH_TRY(c->dy_out.ensure(sizeof(z64) * nm * nm));
+        HIP_TRY(hipMemsetAsync(c->dy_out.p, 0, sizeof(z64) * nm * nm, c->stream));
+        const double q0[3] = {0, 0, 0};
+        LELPH_TRY(dyn_lr_apply(c, q0, 1, 1, false, false, c->dy_out.as<z64>()));
+        std::vector<z64> t((size_t)nm * nm);
+        HIP_TRY(hipMemcpyAsync(t.data(), c->dy_out.p, sizeof(z64) * nm * nm,
+                               hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        for (int a = 0; a < na; ++a)
+            for (int b = 0; b < na; ++b)
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        asrv[(size_t)a * 9 + 3 * i + j] =
+                            cadd(asrv[(size_t)a * 9 + 3 * i + j], t[(size_t)(3 * a + i) * nm + 3 * b + j]);
+    }
+    LELPH_TRY(c->dy_asr.ensure(sizeof(z64) * 9 * na));
+    HIP_TRY(hipMemcpy(c->dy_asr.p, asrv.data(), sizeof(z64) * 9 * na, hipMemcpyHostToDevice));
+    c->dy_have.assign((size_t)c->dy_nq, 0);
+    c->dy_on = true;
+    return LELPH_OK;
+}
+
+int lelph_dyn_interp_add_coarse(lelph_ctx* c, const double qpt[3], const double* omega,
+                                const double* pol_vecs)
+{
+    if (!c || !qpt || !omega || !pol_vecs) return LELPH_ERR_ARG;
+    if (!c->dy_on || c->dy_ready) return LELPH_ERR_STATE;
+    const int na = c->lat.natom, nm = 3 * na;
+    int64_t id[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        const double v = (qpt[i] - std::floor(qpt[i])) * c->dy_grid[i];
+        const double r = std::rint(v);
+        if (std::fabs(v - r) > 1e-4) return LELPH_ERR_ARG;
+        id[i] = ((int64_t)r) % c->dy_grid[i];
+    }
+    const int64_t slot = (id[0] * c->dy_grid[1] + id[1]) * c->dy_grid[2] + id[2];
+    if (c->dy_have[slot]) return LELPH_ERR_ARG;
+    // pol_vecs_to_dyn (src/phonon/pol_vecs_to_dyn.c:8-40), nm x nm on the host
+    const z64* pv = reinterpret_cast<const z64*>(pol_vecs);
+    std::vector<z64> e((size_t)nm * nm), d((size_t)nm * nm);
+    for (int v = 0; v < nm; ++v)
+        for (int j = 0; j < nm; ++j)
+            e[(size_t)v * nm + j] =
+                cscale(pv[(size_t)v * nm + j], c->dy_mass.empty() ? 1.0 : std::pow(c->dy_mass[j / 3], 0.5));
+    for (int a = 0; a < nm; ++a)
+        for (int b = 0; b < nm; ++b)
+        {
+            z64 acc = {0, 0};
+            for (int i = 0; i < nm; ++i)
+            {
+                const double f = (omega[i] > 0 ? 1.0 : -1.0) * omega[i] * omega[i];
+                acc = cadd(acc, cmul(e[(size_t)i * nm + a], cscale(cconj(e[(size_t)i * nm + b]), f)));
+            }
+            d[(size_t)a * nm + b] = acc;
+        }
+    z64* slot_p = c->dy_dyn.as<z64>() + slot * nm * nm;
+    HIP_TRY(hipMemcpyAsync(slot_p, d.data(), sizeof(z64) * nm * nm, hipMemcpyHostToDevice,
+                           c->stream));
+    LELPH_TRY(dyn_lr_apply(c, qpt, 1, -1, !c->dy_mass.empty(), true, slot_p));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->dy_have[slot] = 1;
+    return LELPH_OK;
+}
+
+int lelph_dyn_interp_finish(lelph_ctx* c)
+{
+    if (!c) return LELPH_ERR_ARG;
+    if (!c->dy_on || c->dy_ready) return LELPH_ERR_STATE;
+    for (char h : c->dy_have)
+        if (!h) return LELPH_ERR_STATE;
+    const int na = c->lat.natom, nm = 3 * na;
+    const int64_t nq = c->dy_nq, cols = (int64_t)nm * nm;
+    LELPH_TRY(c->dy_P.ensure(sizeof(z64) * nq * nq));
+    LELPH_TRY(c->dy_fc.ensure(sizeof(z64) * nq * cols));
+    hipLaunchKernelGGL(k_q2r_phases, dim3((unsigned)cdiv(nq * nq, 256)), dim3(256), 0, c->stream,
+                       (int)c->dy_grid[0], (int)c->dy_grid[1], (int)c->dy_grid[2],
+                       c->dy_P.as<z64>());
+    HIP_TRY(hipGetLastError());
+    constexpr int MB = 16;
+    hipLaunchKernelGGL((k_phase_gemm<MB>), dim3((unsigned)cdiv(cols, 256), (unsigned)cdiv(nq, MB)),
+                       dim3(256), 0, c->stream, c->dy_P.as<z64>(), (int)nq, (int)nq, 1,
+                       c->dy_dyn.as<z64>(), 1, cols, c->dy_fc.as<z64>(), nullptr, 0, 0, 0);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_fc_asr, dim3((unsigned)cdiv(9 * na, 64)), dim3(64), 0, c->stream,
+                       c->dy_fc.as<z64>(), (int)nq, na, c->dy_massd.as<double>(),
+                       c->dy_mass.empty() ? 0 : 1, c->dy_asr_kind);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->dy_ready = true;
+    return LELPH_OK;
+}
+
+int lelph_dyn_interp(lelph_ctx* c, int64_t nq, const double* qpts, double* dyn_out,
+                     double* omega_out, double* eig_out)
+{
+    if (!c || nq < 0 || (nq && !qpts)) return LELPH_ERR_ARG;
+    if (!c->dy_ready) return LELPH_ERR_STATE;
+    const int na = c->lat.natom, nm = 3 * na;
+    const int64_t nn = (int64_t)nm * nm;
+    const int64_t chunk = std::min<int64_t>(std::max<int64_t>(nq, 1), 65535);
+    LELPH_TRY(c->dy_out.ensure(sizeof(z64) * nn * chunk));
+    LELPH_TRY(c->dy_eig.ensure(sizeof(z64) * nn * chunk));
+    LELPH_TRY(c->dy_freq.ensure(sizeof(double) * nm * chunk));
+    LELPH_TRY(c->dy_qs2.ensure(sizeof(double) * 3 * chunk));
+    for (int64_t q0 = 0; q0 < nq; q0 += chunk)
+    {
+        const int64_t n = std::min(chunk, nq - q0);
+        HIP_TRY(hipMemcpyAsync(c->dy_qs2.p, qpts + 3 * q0, sizeof(double) * 3 * n,
+                               hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_dyn_r2q, dim3((unsigned)cdiv(n * nn, 256)), dim3(256), 0, c->stream,
+                           c->dy_qs2.as<double>(), (int)n, (int)c->dy_grid[0], (int)c->dy_grid[1],
+                           (int)c->dy_grid[2], na, c->dy_off.as<int64_t>(), c->dy_ws.as<int>(),
+                           c->dy_fc.as<z64>(), c->dy_out.as<z64>());
+        HIP_TRY(hipGetLastError());
+        LELPH_TRY(dyn_lr_apply(c, qpts + 3 * q0, n, 1, !c->dy_mass.empty(), true,
+                               c->dy_out.as<z64>()));
+        if (dyn_out)
+            HIP_TRY(hipMemcpyAsync(dyn_out + 2 * q0 * nn, c->dy_out.p, sizeof(z64) * nn * n,
+                                   hipMemcpyDeviceToHost, c->stream));
+        if (omega_out || eig_out)
+        {
+            hipLaunchKernelGGL(k_herm_jacobi, dim3((unsigned)n), dim3(64), 0, c->stream,
+                               c->dy_out.as<z64>(), nm, c->dy_massd.as<double>(),
+                               c->dy_mass.empty() ? 0 : 1, c->dy_freq.as<double>(),
+                               c->dy_eig.as<z64>(), 40);
+            HIP_TRY(hipGetLastError());
+            if (omega_out)
+                HIP_TRY(hipMemcpyAsync(omega_out + q0 * nm, c->dy_freq.p, sizeof(double) * nm * n,
+                                       hipMemcpyDeviceToHost, c->stream));
+            if (eig_out)
+                HIP_TRY(hipMemcpyAsync(eig_out + 2 * q0 * nn, c->dy_eig.p, sizeof(z64) * nn * n,
+                                       hipMemcpyDeviceToHost, c->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return LELPH_OK;
 }
 
 int lelph_internal_grid(const lelph_ctx* c, int64_t* nbz, const double** kcrys, const int** kmap)

@@ -35,7 +35,8 @@ EXPORTS = [
     "lelph_rotate_dvscf", "lelph_wigner_seitz", "lelph_interp_begin", "lelph_interp_add_coarse",
     "lelph_interp_add_star", "lelph_interp_finish", "lelph_interp_dvscf",
     "lelph_interp_set_dvscf", "lelph_read_dvscf_qe", "lelph_set_dvscf_file",
-    "lelph_write_dvscf_qe", "lelph_interp_timings",
+    "lelph_write_dvscf_qe", "lelph_interp_timings", "lelph_dyn_interp_begin",
+    "lelph_dyn_interp_add_coarse", "lelph_dyn_interp_finish", "lelph_dyn_interp",
 ]
 
 
@@ -109,6 +110,12 @@ def lib():
                                            c_double_p, C.c_int]
         L.lelph_write_dvscf_qe.argtypes = [C.c_void_p, C.c_char_p, c_double_p]
         L.lelph_interp_timings.argtypes = [C.c_void_p, c_double_p, C.c_int]
+        L.lelph_dyn_interp_begin.argtypes = [C.c_void_p, i64p, c_double_p, C.POINTER(LongRange),
+                                             C.c_double, C.c_double, C.c_int]
+        L.lelph_dyn_interp_add_coarse.argtypes = [C.c_void_p, c_double_p, c_double_p, c_double_p]
+        L.lelph_dyn_interp_finish.argtypes = [C.c_void_p]
+        L.lelph_dyn_interp.argtypes = [C.c_void_p, C.c_int64, c_double_p, c_double_p, c_double_p,
+                                       c_double_p]
         _lib = L
     return _lib
 
@@ -368,6 +375,40 @@ class Context:
         _check(lib().lelph_interp_dvscf(self.h, len(q), dptr(q), dptr(out)),
                "lelph_interp_dvscf")
         return out
+
+    # ---- Fourier interpolation of dynamical matrices ----
+    def dyn_interp_begin(self, qgrid, masses=None, lr: LrParams | None = None, eta_ph=1.0,
+                         ecut=30.0, asr: str = "simple"):
+        g = np.ascontiguousarray(qgrid, np.int64)
+        self._dy_mass = None if masses is None else np.ascontiguousarray(masses, np.float64)
+        self._dy_plr = None if lr is None else lr.pack()
+        self._dy_lr = lr
+        _check(lib().lelph_dyn_interp_begin(self.h, i64ptr(g),
+                                            None if self._dy_mass is None else dptr(self._dy_mass),
+                                            None if self._dy_plr is None else C.byref(self._dy_plr),
+                                            float(eta_ph), float(ecut),
+                                            {"none": 0, "simple": 1}[asr]),
+               "lelph_dyn_interp_begin")
+
+    def dyn_interp_add_coarse(self, qpt, omega, pol_vecs):
+        _check(lib().lelph_dyn_interp_add_coarse(
+            self.h, dptr(np.ascontiguousarray(qpt, np.float64)),
+            dptr(np.ascontiguousarray(omega, np.float64)),
+            dptr(np.ascontiguousarray(pol_vecs, np.complex128))), "lelph_dyn_interp_add_coarse")
+
+    def dyn_interp_finish(self):
+        _check(lib().lelph_dyn_interp_finish(self.h), "lelph_dyn_interp_finish")
+
+    def dyn_interp(self, qpts):
+        """(dyn (nq,nm,nm), omega (nq,nm), eig (nq,nm,nm)) at the fine q."""
+        q = np.ascontiguousarray(np.atleast_2d(qpts), np.float64)
+        nm = self.sys.nmodes
+        dyn = np.empty((len(q), nm, nm), np.complex128)
+        om = np.empty((len(q), nm))
+        eig = np.empty((len(q), nm, nm), np.complex128)
+        _check(lib().lelph_dyn_interp(self.h, len(q), dptr(q), dptr(dyn), dptr(om), dptr(eig)),
+               "lelph_dyn_interp")
+        return dyn, om, eig
 
     def interp_timings(self) -> dict:
         t = np.zeros(5)

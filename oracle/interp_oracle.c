@@ -744,3 +744,269 @@ int orc_write_dvscf_qe(const char* path, const lelph_lattice* lat, const double*
     if (fclose(fp)) st = LELPH_ERR_ARG;
     return st;
 }
+
+/* ------------------------------------------------------------------------ */
+/* dynamical matrices (src/phonon/, interpolation_utilities.c:78-163)      */
+/* ------------------------------------------------------------------------ */
+
+/* pol_vecs_to_dyn, src/phonon/pol_vecs_to_dyn.c:8-40 (with
+ * mass_normalize_pol_vecs power 0.5, mass_normalize_phonons.c:11-34) */
+void orc_pol_vecs_to_dyn(const double* omega, int64_t natom, const double* masses,
+                         double* pold)
+{
+    const int64_t nm = 3 * natom;
+    cplx* pol = (cplx*)pold;
+    if (masses)
+        for (int64_t v = 0; v < nm; ++v)
+            for (int64_t j = 0; j < nm; ++j) pol[v * nm + j] *= pow(masses[j / 3], 0.5);
+    cplx* e = malloc(sizeof(cplx) * nm * nm);
+    memcpy(e, pol, sizeof(cplx) * nm * nm);
+    for (int64_t a = 0; a < nm; ++a)
+        for (int64_t b = 0; b < nm; ++b)
+        {
+            cplx acc = 0.0;
+            for (int64_t i = 0; i < nm; ++i)
+            {
+                const double f = (omega[i] > 0 ? 1.0 : -1.0) * omega[i] * omega[i];
+                acc += e[i * nm + a] * (f * conj(e[i * nm + b]));
+            }
+            pol[a * nm + b] = acc;
+        }
+    free(e);
+}
+
+/* add_ph_dyn_long_range_internal, src/phonon/dyn_long_range.c:125-404 */
+static void dyn_lr_internal(const double* qpt, const lelph_lattice* lat, const double* eps,
+                            const double* Zb, const double* Qp, const int64_t* Gg, int sign,
+                            const double* masses, double eta, cplx* dyn)
+{
+    if (!eps || (!Zb && !Qp)) return;
+    const int64_t na = lat->natom, nm = 3 * na;
+    double factor = 4.0 * RPI * RE2 / lat->volume;
+    if (sign < 0) factor = -factor;
+    double ea[9];
+    memcpy(ea, eps, sizeof(ea));
+    const double zlat = lat->alat_vec[8];
+    const bool d2 = lat->dimension == '2';
+    if (d2)
+    {
+        factor = factor * zlat / 2.0;
+        for (int i = 0; i < 9; ++i) ea[i] = 0.0;
+        ea[0] = 0.5 * zlat * (eps[0] - 1);
+        ea[1] = 0.5 * zlat * eps[1];
+        ea[3] = 0.5 * zlat * eps[3];
+        ea[4] = 0.5 * zlat * (eps[4] - 1);
+    }
+    const int64_t GZ = d2 ? 1 : Gg[2];
+    const int64_t nG = Gg[0] * Gg[1] * GZ;
+    cplx* X1 = calloc(nG * nm, sizeof(cplx));
+    cplx* X2 = calloc(nG * nm, sizeof(cplx));
+    cplx* P1 = d2 ? calloc(nG * nm, sizeof(cplx)) : NULL;
+    cplx* P2 = d2 ? calloc(nG * nm, sizeof(cplx)) : NULL;
+    for (int64_t ig = 0; ig < nG; ++ig)
+    {
+        const int64_t gx = ig / (Gg[1] * GZ), gy = (ig % (Gg[1] * GZ)) / GZ,
+                      gz = (ig % (Gg[1] * GZ)) % GZ;
+        double t[3] = {qpt[0] + miller(gx, Gg[0]), qpt[1] + miller(gy, Gg[1]),
+                       qpt[2] + miller(gz, GZ)};
+        double qG[3];
+        mv3(lat->blat_vec, t, false, qG);
+        const double qn = sqrt(dot3(qG, qG));
+        if (qn < REPS) continue;
+        double tb[3];
+        mv3(ea, qG, false, tb);
+        double qeq = dot3(tb, qG);
+        double qeqp = 1.0;
+        double decay = exp(-fabs(qeq) * 0.125 / eta);
+        if (d2)
+        {
+            const double Leff = eta * zlat * 1.001 * (1.0 - 1.0 / eps[8]);
+            decay = 1 - tanh(qn * 0.5 * Leff);
+            if (fabs(decay) < REPS6) continue;
+            qeq *= decay;
+            qeq += qn;
+            qeqp = 1.0;
+            qeqp = fabs(qeqp) < REPS ? 0.0 : -qn / qeqp;
+            decay = sqrt(decay);
+        }
+        for (int64_t ia = 0; ia < na; ++ia)
+        {
+            const double* Z = Zb ? Zb + 9 * ia : NULL;
+            const double* Q = Qp ? Qp + 27 * ia : NULL;
+            double zq[3] = {0, 0, 0}, qb[3] = {0, 0, 0};
+            if (Z) mv3(Z, qG, true, zq);
+            if (Q)
+            {
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        for (int k = 0; k < 3; ++k) qb[k] += qG[i] * qG[j] * Q[k + 3 * j + 9 * i];
+                if (d2)
+                    for (int i = 0; i < 3; ++i) qb[i] -= qn * qn * Q[i + 24];
+            }
+            cplx ph = cexp(-I * dot3(qG, lat->atomic_pos + 3 * ia));
+            ph /= qeq;
+            ph *= decay;
+            if (masses) ph /= sqrt(masses[ia]);
+            for (int i = 0; i < 3; ++i)
+            {
+                X1[ig * nm + 3 * ia + i] = (zq[i] - I * 0.5 * qb[i]) * ph;
+                X2[ig * nm + 3 * ia + i] = X1[ig * nm + 3 * ia + i] * qeq;
+            }
+            if (d2)
+            {
+                ph *= qeq;
+                for (int i = 0; i < 3; ++i)
+                {
+                    if (Z)
+                    {
+                        P1[ig * nm + 3 * ia + i] += Z[i + 6] * ph * qeqp / eps[8];
+                        P2[ig * nm + 3 * ia + i] += Z[i + 6] * ph / eps[8];
+                    }
+                    if (Q)
+                        for (int x = 0; x < 3; ++x)
+                        {
+                            P1[ig * nm + 3 * ia + i] -= I * qG[x] * Q[i + 18 + 3 * x] * ph * qeqp;
+                            P2[ig * nm + 3 * ia + i] -= qG[x] * I * Q[i + 18 + 3 * x] * ph;
+                        }
+                }
+            }
+        }
+    }
+    /* dyn += factor X2^H X1 (and the out-of-plane pair in 2-D) */
+    for (int64_t a = 0; a < nm; ++a)
+        for (int64_t b = 0; b < nm; ++b)
+        {
+            cplx acc = 0.0, accp = 0.0;
+            for (int64_t ig = 0; ig < nG; ++ig)
+            {
+                acc += conj(X2[ig * nm + a]) * X1[ig * nm + b];
+                if (d2) accp += conj(P2[ig * nm + a]) * P1[ig * nm + b];
+            }
+            dyn[a * nm + b] += factor * acc;
+            if (d2) dyn[a * nm + b] += factor * accp;
+        }
+    free(X1);
+    free(X2);
+    free(P1);
+    free(P2);
+}
+
+/* compute_dyn_lr_asr_correction, dyn_long_range.c:70-123: out (natom,3,3) */
+void orc_dyn_lr_asr(const lelph_lattice* lat, const double* eps, const double* Zb,
+                    const double* Qp, const int64_t* Gg, double eta, double* outd)
+{
+    const int64_t na = lat->natom, nm = 3 * na;
+    cplx* out = (cplx*)outd;
+    for (int64_t i = 0; i < 9 * na; ++i) out[i] = 0.0;
+    if (!eps || (!Zb && !Qp)) return;
+    cplx* t = calloc(nm * nm, sizeof(cplx));
+    const double q0[3] = {0, 0, 0};
+    dyn_lr_internal(q0, lat, eps, Zb, Qp, Gg, 1, NULL, eta, t);
+    for (int64_t ia = 0; ia < na; ++ia)
+        for (int64_t ja = 0; ja < na; ++ja)
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) out[ia * 9 + 3 * i + j] += t[(ia * 3 + i) * nm + ja * 3 + j];
+    free(t);
+}
+
+/* add_ph_dyn_long_range, dyn_long_range.c:27-68 */
+int orc_add_ph_dyn_long_range(const double* qpt, const lelph_lattice* lat, const double* eps,
+                              const double* Zb, const double* Qp, const int64_t* Gg, int sign,
+                              const double* masses, const double* asrd, double eta,
+                              double* dynd)
+{
+    if (lat->dimension == '2' && fabs(qpt[2]) > REPS && eps && (Zb || Qp)) return LELPH_ERR_ARG;
+    cplx* dyn = (cplx*)dynd;
+    dyn_lr_internal(qpt, lat, eps, Zb, Qp, Gg, sign, masses, eta, dyn);
+    if (!asrd) return 0;
+    const cplx* asr = (const cplx*)asrd;
+    const double f = sign < 0 ? 1.0 : -1.0;
+    const int64_t na = lat->natom, nm = 3 * na;
+    for (int64_t ia = 0; ia < na; ++ia)
+    {
+        const double m = masses ? 1.0 / masses[ia] : 1.0;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                dyn[(ia * 3 + i) * nm + ia * 3 + j] += f * asr[ia * 9 + 3 * i + j] * m;
+    }
+    return 0;
+}
+
+/* fft_R2q_dyn, interpolation_utilities.c:78-163: dataR (R, nm, nm) */
+int orc_fft_R2q_dyn(const double* dataRd, const double* q, const int64_t* qgrid, int64_t natom,
+                    const int64_t* ws, int64_t nws, const int64_t* degen, double* dataqd)
+{
+    const cplx* dR = (const cplx*)dataRd;
+    cplx* dq = (cplx*)dataqd;
+    const int64_t nR = qgrid[0] * qgrid[1] * qgrid[2], nm = 3 * natom;
+    for (int64_t i = 0; i < nm * nm; ++i) dq[i] = 0.0;
+    const int64_t yz = qgrid[1] * qgrid[2];
+    int64_t iw = 0;
+    for (int64_t i = 0; i < nR; ++i)
+    {
+        const double R[3] = {(double)miller(i / yz, qgrid[0]),
+                             (double)miller((i % yz) / qgrid[2], qgrid[1]),
+                             (double)miller((i % yz) % qgrid[2], qgrid[2])};
+        const cplx eR = cexp(I * 2 * RPI * dot3(q, R));
+        for (int64_t ia = 0; ia < natom; ++ia)
+            for (int64_t ja = 0; ja < natom; ++ja)
+            {
+                const int64_t dg = degen[i * natom * natom + ia * natom + ja];
+                cplx eT = 0.0;
+                for (int64_t t = 0; t < dg; ++t)
+                {
+                    if (iw >= nws) return LELPH_ERR_RANGE;
+                    eT += cexp(I * 2 * RPI * (q[0] * ws[3 * iw] + q[1] * ws[3 * iw + 1] +
+                                              q[2] * ws[3 * iw + 2]));
+                    ++iw;
+                }
+                eT *= eR;
+                eT *= 1.0 / dg;
+                for (int x = 0; x < 3; ++x)
+                    for (int y = 0; y < 3; ++y)
+                    {
+                        const int64_t im = x + 3 * ia, jm = y + 3 * ja;
+                        dq[im * nm + jm] += eT * dR[i * nm * nm + im * nm + jm];
+                    }
+            }
+    }
+    return 0;
+}
+
+/* mass_normalize_force_constants, mass_normalize_phonons.c:36-71 */
+void orc_mass_normalize_fc(const double* masses, int64_t nsets, int64_t natom, double power,
+                           double* frcd)
+{
+    if (!masses) return;
+    cplx* frc = (cplx*)frcd;
+    const int64_t nm = 3 * natom;
+    for (int64_t ia = 0; ia < natom; ++ia)
+    {
+        const double Ma = pow(masses[ia], power);
+        for (int64_t ib = 0; ib < natom; ++ib)
+        {
+            const double Mb = pow(masses[ib], power);
+            for (int64_t s = 0; s < nsets; ++s)
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        frc[s * nm * nm + ia * 9 * natom + 3 * ib + j + i * nm] *= Ma * Mb;
+        }
+    }
+}
+
+/* apply_acoustic_sum_rule_fc, ASR_SIMPLE branch, src/phonon/asr.c:294-318 */
+void orc_asr_simple_fc(const int64_t* qgrid, int64_t natom, double* frcd)
+{
+    cplx* frc = (cplx*)frcd;
+    const int64_t ng = qgrid[0] * qgrid[1] * qgrid[2], nm = 3 * natom;
+    for (int64_t ia = 0; ia < natom; ++ia)
+        for (int64_t al = 0; al < 3; ++al)
+            for (int64_t be = 0; be < 3; ++be)
+            {
+                cplx sum = 0.0;
+                const int64_t base = ia * 9 * natom + al * nm + be;
+                for (int64_t g = 0; g < ng; ++g)
+                    for (int64_t ib = 0; ib < natom; ++ib) sum += frc[g * nm * nm + base + ib * 3];
+                frc[base + ia * 3] -= sum;
+            }
+}

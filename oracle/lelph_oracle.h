@@ -141,6 +141,18 @@ int orc_fft_R2q_dvscf(const double* dataR, const double* q, const int64_t* qgrid
                       const int64_t* degen, double* dataq);
 int orc_find_qpt_grid(int64_t nq, const double* q, int64_t* grid);
 void orc_sorted_qpts_idxs(int64_t nq, const double* q, int64_t* idx);
+/* dynamical matrices (src/phonon/, interpolation_utilities.c:78-163) */
+void orc_pol_vecs_to_dyn(const double* omega, int64_t natom, const double* masses, double* pol);
+void orc_dyn_lr_asr(const lelph_lattice* lat, const double* eps, const double* Zb,
+                    const double* Qp, const int64_t* Gg, double eta, double* out);
+int orc_add_ph_dyn_long_range(const double* qpt, const lelph_lattice* lat, const double* eps,
+                              const double* Zb, const double* Qp, const int64_t* Gg, int sign,
+                              const double* masses, const double* asr, double eta, double* dyn);
+int orc_fft_R2q_dyn(const double* dataR, const double* q, const int64_t* qgrid, int64_t natom,
+                    const int64_t* ws, int64_t nws, const int64_t* degen, double* dataq);
+void orc_mass_normalize_fc(const double* masses, int64_t nsets, int64_t natom, double power,
+                           double* frc);
+void orc_asr_simple_fc(const int64_t* qgrid, int64_t natom, double* frc);
 /* QE dvscf files (src/io/qe/read_dvscf.c, write_dyn_dvscf_qe.c) */
 int orc_read_dvscf_qe(const char* path, const lelph_lattice* lat, const double* eig,
                       const double* pats, double* out);

@@ -111,6 +111,17 @@ def lib():
                                         i64p, C.c_int64, i64p, c_double_p]
         L.orc_find_qpt_grid.argtypes = [C.c_int64, c_double_p, i64p]
         L.orc_sorted_qpts_idxs.argtypes = [C.c_int64, c_double_p, i64p]
+        L.orc_pol_vecs_to_dyn.argtypes = [c_double_p, C.c_int64, c_double_p, c_double_p]
+        L.orc_dyn_lr_asr.argtypes = [C.POINTER(Lattice), c_double_p, c_double_p, c_double_p, i64p,
+                                     C.c_double, c_double_p]
+        L.orc_add_ph_dyn_long_range.argtypes = [c_double_p, C.POINTER(Lattice), c_double_p,
+                                                c_double_p, c_double_p, i64p, C.c_int,
+                                                c_double_p, c_double_p, C.c_double, c_double_p]
+        L.orc_fft_R2q_dyn.argtypes = [c_double_p, c_double_p, i64p, C.c_int64, i64p, C.c_int64,
+                                      i64p, c_double_p]
+        L.orc_mass_normalize_fc.argtypes = [c_double_p, C.c_int64, C.c_int64, C.c_double,
+                                            c_double_p]
+        L.orc_asr_simple_fc.argtypes = [i64p, C.c_int64, c_double_p]
         L.orc_read_dvscf_qe.argtypes = [C.c_char_p, C.POINTER(Lattice), c_double_p, c_double_p,
                                         c_double_p]
         L.orc_write_dvscf_qe.argtypes = [C.c_char_p, C.POINTER(Lattice), c_double_p]
@@ -381,3 +392,92 @@ class OracleInterp:
         if self.lr is not None:
             out = self.longrange(q, out, None, 1)
         return out
+
+
+def _opt(a):
+    return None if a is None else dptr(np.ascontiguousarray(a, np.float64))
+
+
+class OracleDynInterp:
+    """The oracle's dynamical-matrix interpolation, in the order of
+    interpolation_driver.c:190-530 (coarse) and :855-992 (fine), with
+    numpy's LAPACK eigh standing in for the reference's zheev."""
+
+    def __init__(self, sys, qgrid, masses=None, lr: LrParams | None = None, eta_ph=1.0,
+                 ecut=30.0, asr="simple"):
+        self.sys = sys
+        self.packed = pack_system(sys)
+        self.grid = np.ascontiguousarray(qgrid, np.int64)
+        self.nq = int(np.prod(self.grid))
+        self.masses = None if masses is None else np.ascontiguousarray(masses, np.float64)
+        self.lr = lr
+        self.eps = None if lr is None else lr.epsilon
+        self.zb = None if lr is None else lr.Zborn
+        self.qp = None if lr is None else lr.Qpole
+        self.eta = float(eta_ph)
+        self.asr = asr
+        nm = sys.nmodes
+        self.Gg = np.zeros(3, np.int64)
+        lib().orc_get_fft_box(float(ecut), dptr(np.ascontiguousarray(sys.blat)), i64ptr(self.Gg))
+        self.ws, self.degen = ws_vectors(self.grid, sys.alat, 1e-6, sys.atomic_pos, sys.atomic_pos)
+        self.asr_lr = np.zeros(9 * sys.natom, np.complex128)
+        lib().orc_dyn_lr_asr(C.byref(self.packed.lattice), _opt(self.eps), _opt(self.zb),
+                             _opt(self.qp), i64ptr(self.Gg), self.eta, dptr(self.asr_lr))
+        self.dyns = np.zeros((self.nq, nm, nm), np.complex128)
+        self.have = np.zeros(self.nq, bool)
+
+    def slot(self, q):
+        idx = [int(np.rint((q[i] - np.floor(q[i])) * self.grid[i])) % int(self.grid[i])
+               for i in range(3)]
+        return (idx[0] * self.grid[1] + idx[1]) * self.grid[2] + idx[2]
+
+    def lr_term(self, q, dyn, sign, masses=True, asr=True):
+        d = np.ascontiguousarray(dyn, np.complex128).copy()
+        check(lib().orc_add_ph_dyn_long_range(
+            dptr(np.ascontiguousarray(q, np.float64)), C.byref(self.packed.lattice),
+            _opt(self.eps), _opt(self.zb), _opt(self.qp), i64ptr(self.Gg), int(sign),
+            _opt(self.masses) if masses else None, dptr(self.asr_lr) if asr else None,
+            self.eta, dptr(d)))
+        return d
+
+    def add_coarse(self, q, omega, pol):
+        d = np.ascontiguousarray(pol, np.complex128).copy()
+        lib().orc_pol_vecs_to_dyn(dptr(np.ascontiguousarray(omega, np.float64)), self.sys.natom,
+                                  _opt(self.masses), dptr(d))
+        d = self.lr_term(q, d, -1)
+        i = self.slot(q)
+        assert not self.have[i]
+        self.dyns[i] = d
+        self.have[i] = True
+
+    def finish(self):
+        assert self.have.all()
+        nm = self.sys.nmodes
+        fc = np.ascontiguousarray(self.dyns.reshape(self.nq, nm * nm)).copy()
+        lib().orc_fft_q2R(dptr(fc), i64ptr(self.grid), nm * nm)
+        m = _opt(self.masses)
+        lib().orc_mass_normalize_fc(m, self.nq, self.sys.natom, 0.5, dptr(fc))
+        if self.asr == "simple":
+            lib().orc_asr_simple_fc(i64ptr(self.grid), self.sys.natom, dptr(fc))
+        lib().orc_mass_normalize_fc(m, self.nq, self.sys.natom, -0.5, dptr(fc))
+        self.fc = fc.reshape(self.nq, nm, nm)
+
+    def dyn(self, q):
+        nm = self.sys.nmodes
+        out = np.zeros((nm, nm), np.complex128)
+        check(lib().orc_fft_R2q_dyn(dptr(np.ascontiguousarray(self.fc)),
+                                    dptr(np.ascontiguousarray(q, np.float64)), i64ptr(self.grid),
+                                    self.sys.natom, i64ptr(self.ws), len(self.ws),
+                                    i64ptr(self.degen), dptr(out)))
+        return self.lr_term(q, out, 1)
+
+    def modes(self, dyn):
+        """symmetrised lower triangle, eigh, sign(w) sqrt|w|, rows / sqrt(M)"""
+        lo = np.tril(0.5 * (dyn + dyn.conj().T))
+        h = lo + np.tril(lo, -1).conj().T
+        w, v = np.linalg.eigh(h)
+        f = np.sign(w) * np.sqrt(np.abs(w))
+        e = v.T.copy()
+        if self.masses is not None:
+            e = e / np.sqrt(np.repeat(self.masses, 3))[None, :]
+        return f, e

@@ -254,3 +254,53 @@ def test_dvscf_qe_files(tmp_path):
         assert rel_err(g_file, g_host) < 1e-13
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("case", ["3d", "2d"])
+def test_dyn_interpolation(case):
+    """dynamical matrices interpolated on the device against the oracle
+    (long range with Born charges and quadrupoles, simple sum rule), and the
+    device eigensolver against LAPACK's eigenvalues and a reconstruction of
+    the Hermitian part from its eigenvectors"""
+    rng = np.random.default_rng(17)
+    if case == "3d":
+        s = synth.config("c0", with_wfc=False)
+        grid = (3, 3, 3)
+        fine = rng.uniform(-0.6, 0.6, (9, 3))
+        fine[0] = 0.0
+    else:
+        s = synth.tiny(dimension="2", with_wfc=False)
+        grid = (3, 3, 1)
+        fine = np.c_[rng.uniform(-0.5, 0.5, (6, 2)), np.zeros(6)]
+    nm = s.nmodes
+    masses = masses_for(s)
+    lr = lr_params(s, quad=True)
+    od = O.OracleDynInterp(s, grid, masses=masses, lr=lr, eta_ph=1.0, ecut=30.0, asr="simple")
+    ctx = lelph.Context(s, "fp64", upload=False)
+    say(f"case: dynamical-matrix interpolation {case}")
+    try:
+        ctx.dyn_interp_begin(grid, masses=masses, lr=lr, eta_ph=1.0, ecut=30.0, asr="simple")
+        for q in coarse_grid(grid):
+            e, _ = np.linalg.qr(rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm)))
+            om = np.sort(rng.uniform(0.05, 1.0, nm))
+            pol = e / np.sqrt(np.repeat(masses, 3))[None, :]
+            ctx.dyn_interp_add_coarse(q, om, pol)
+            od.add_coarse(q, om, pol)
+        ctx.dyn_interp_finish()
+        od.finish()
+        dyn, om, eig = ctx.dyn_interp(fine)
+        sq = np.sqrt(np.repeat(masses, 3))
+        for i, q in enumerate(fine):
+            ref = od.dyn(q)
+            assert rel_err(dyn[i], ref) < TOL64, (case, i)
+            f_ref, _ = od.modes(ref)
+            w = np.sign(om[i]) * om[i] ** 2
+            w_ref = np.sign(f_ref) * f_ref ** 2
+            assert np.abs(w - w_ref).max() < 1e-10 * np.abs(w_ref).max(), (case, i)
+            V = (eig[i] * sq[None, :]).T  # columns: unit eigenvectors
+            assert np.abs(V.conj().T @ V - np.eye(nm)).max() < 1e-10
+            lo = np.tril(0.5 * (ref + ref.conj().T))
+            h = lo + np.tril(lo, -1).conj().T
+            assert rel_err(V @ np.diag(w) @ V.conj().T, h) < 1e-10, (case, i)
+    finally:
+        ctx.close()

@@ -286,3 +286,109 @@ def test_dvscf_qe_files(tmp_path):
     O.check(O.lib().orc_read_dvscf_qe(f1.encode(), C.byref(pk.lattice), dptr(e), dptr(p),
                                       dptr(out)))
     assert rel_err(out, np.einsum("ab,bm...->am...", e @ p.conj().T, dv)) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# dynamical matrices
+# ---------------------------------------------------------------------------
+
+def _dyn_lr(s, lr):
+    return O.OracleDynInterp(s, (1, 1, 1), masses=None, lr=lr, eta_ph=1.0, ecut=30.0,
+                             asr="none")
+
+
+def test_pol_vecs_to_dyn_eigensystem():
+    """pol_vecs_to_dyn builds the matrix whose eigenpairs are sign(w) w^2 and
+    the mass-weighted polarisation vectors."""
+    rng = np.random.default_rng(2)
+    na, nm = 2, 6
+    masses = np.array([1.7, 3.1])
+    e, _ = np.linalg.qr(rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm)))
+    om = np.array([-0.01, 0.02, 0.5, 0.7, 1.1, 1.3])
+    pol = np.ascontiguousarray(e / np.sqrt(np.repeat(masses, 3))[None, :])
+    d = pol.copy()
+    O.lib().orc_pol_vecs_to_dyn(dptr(om), na, dptr(masses), dptr(d))
+    f = np.sign(om) * om ** 2
+    assert rel_err(np.linalg.eigvalsh(d), np.sort(f)) < 1e-13
+    for i in range(nm):
+        assert np.abs(d @ e[i] - f[i] * e[i]).max() < 1e-12 * np.abs(f).max()
+
+
+@pytest.mark.parametrize("dimension", ["3", "2"])
+def test_dyn_longrange_invariants(dimension):
+    """The long-range dynamical matrix is Hermitian, satisfies D(-q) =
+    conj D(q), is unchanged by a common shift of the atoms, and with
+    charge-neutral Born charges its q = 0 row sums are the ASR correction
+    the reference subtracts."""
+    s = _small_system(dimension)
+    lr = _lr_params(s)
+    od = _dyn_lr(s, lr)
+    q = np.array([0.23, -0.11, 0.0 if dimension == "2" else 0.17])
+    nm = s.nmodes
+    z = np.zeros((nm, nm), np.complex128)
+    D = od.lr_term(q, z, 1, masses=False, asr=False)
+    assert np.abs(D).max() > 0
+    assert rel_err(D, D.conj().T) < 1e-12
+    assert rel_err(od.lr_term(-q, z, 1, masses=False, asr=False), D.conj()) < 1e-12
+    s2 = _small_system(dimension)
+    s2.atomic_pos = s2.atomic_pos + np.array([0.37, -0.21, 0.0 if dimension == "2" else 0.5])
+    D2 = _dyn_lr(s2, lr).lr_term(q, z, 1, masses=False, asr=False)
+    assert rel_err(D2, D) < 1e-11
+    D0 = od.lr_term(np.zeros(3), z, 1, masses=False, asr=False)
+    rows = D0.reshape(s.natom, 3, s.natom, 3).sum(2)
+    assert np.abs(rows.reshape(-1) - od.asr_lr).max() <= 1e-12 * max(np.abs(D0).max(), 1e-300)
+    Dc = od.lr_term(np.zeros(3), z, 1, masses=False, asr=True)
+    assert np.abs(Dc.reshape(s.natom, 3, s.natom, 3).sum(2)).max() <= 1e-10 * np.abs(D0).max()
+
+
+def test_dyn_longrange_nonanalytic_limit():
+    """Near q = 0 only the G = 0 term depends on the direction of q:
+    D(d1 dq) - D(d2 dq) -> (4 pi e^2 / Omega) [Z^T d (Z^T d)^T / (d eps d)]
+    evaluated for d1 minus d2 (Gonze's non-analytic term)."""
+    s = synth.tiny(with_wfc=False)
+    lr = _lr_params(s, quad=False)
+    od = _dyn_lr(s, lr)
+    nm = s.nmodes
+    z = np.zeros((nm, nm), np.complex128)
+    dq = 2e-5  # |q| above the reference's q + G = 0 threshold (1e-5, cartesian)
+    dirs = [np.array([1.0, 0, 0]), np.array([0.3, -0.5, 0.8])]
+    Ds, NA = [], []
+    for d in dirs:
+        dcrys = np.linalg.solve(s.blat, d / np.linalg.norm(d))  # crystal of a unit cart vector
+        Ds.append(od.lr_term(dq * dcrys, z, 1, masses=False, asr=False))
+        qc = s.blat @ (dq * dcrys)
+        zq = np.einsum("aij,i->aj", lr.Zborn, qc).reshape(-1)
+        NA.append(4 * synth.RPI * 2.0 / s.volume * np.outer(zq, zq) / (qc @ lr.epsilon @ qc))
+    assert rel_err(Ds[0] - Ds[1], NA[0] - NA[1]) < 2e-4  # O(|q|) from the analytic rest
+
+
+@pytest.mark.parametrize("asr", ["none", "simple"])
+def test_dyn_interpolation_coarse_and_asr(asr):
+    """coarse -> R -> q returns each coarse dynamical matrix (asr none, to
+    the reference's pi mismatch), and the simple sum rule zeroes the
+    R-summed mass-unweighted force constants of every atom row."""
+    s = synth.tiny(with_wfc=False)
+    grid = (2, 2, 2)
+    masses = np.array([1.5, 2.5])
+    lr = _lr_params(s, quad=False)
+    od = O.OracleDynInterp(s, grid, masses=masses, lr=lr, eta_ph=1.0, asr=asr)
+    rng = np.random.default_rng(3)
+    dyns = []
+    for q in coarse_grid(grid):
+        e, _ = np.linalg.qr(rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6)))
+        om = rng.uniform(0.1, 1.0, 6)
+        pol = e / np.sqrt(np.repeat(masses, 3))[None, :]
+        od.add_coarse(q, om, pol)
+        d = pol.copy()
+        O.lib().orc_pol_vecs_to_dyn(dptr(om), 2, dptr(masses), dptr(d))
+        dyns.append(d)
+    od.finish()
+    say(f"case: dyn interpolation, asr {asr}")
+    if asr == "none":
+        for i, q in enumerate(coarse_grid(grid)):
+            assert rel_err(od.dyn(q), dyns[i]) < 2e-6, i
+    else:
+        M = np.sqrt(np.outer(np.repeat(masses, 3), np.repeat(masses, 3)))
+        fc = od.fc * M[None]
+        s_ = fc.reshape(len(fc), 2, 3, 2, 3).sum(axis=(0, 3))
+        assert np.abs(s_).max() < 1e-12 * np.abs(fc).max()
