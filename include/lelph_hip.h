@@ -374,6 +374,28 @@ int lelph_dyn_interp_finish(lelph_ctx* ctx);
 int lelph_dyn_interp(lelph_ctx* ctx, int64_t nq, const double* qpts, double* dyn_out,
                      double* omega_out, double* eig_out);
 
+/* ------------------------------------------------------------------------
+ * QE phonon files (ph_save)
+ * ---------------------------------------------------------------------- */
+
+/* read_dyn_qe (src/io/qe/read_dyn.c:38-293, old text format, first
+ * matrix of the file): qpt (3) as written, omega (nmodes) sign(w) sqrt|w|
+ * ascending, pol_vecs (nmodes, 3 natom) eigenvector rows / sqrt(M), amass
+ * (natom, or NULL) the masses of the file.  The Hermitian part is
+ * diagonalised on the device (eigenvector phases are this solver's own). */
+int lelph_read_dyn_qe(lelph_ctx* ctx, const char* dyn_file, double* qpt, double* omega,
+                      double* pol_vecs, double* amass);
+
+/* read_pattern_qe (src/io/qe/read_patterns.c:18-125), host only: pat_vecs
+ * (nmodes, nmodes) complex from patterns.N.xml (identity for the
+ * DVSCF_IS_IN_CARTESIAN_BASIS placeholder). */
+int lelph_read_patterns_qe(const char* pat_file, int natom, double* pat_vecs);
+
+/* write_dyn_qe (src/io/qe/write_dyn_dvscf_qe.c:18-97), host only: one q,
+ * dyn (nmodes, nmodes) mass normalised, written times sqrt(Ma Mb). */
+int lelph_write_dyn_qe(const char* dyn_file, int natom, const double qpt[3], const double* dyn,
+                       const double* masses);
+
 /* Device time of the last lelph_interp_dvscf, summed over its q chunks,
  * milliseconds: [0] Wigner-Seitz weights, [1] R -> q phase GEMM with
  * e^{-iqr}, [2] long range added back, [3] total, [4] number of q.

@@ -159,7 +159,7 @@ struct lelph_ctx
     std::vector<double> dy_mass, dy_eps, dy_zb, dy_qp;
     std::vector<char> dy_have;
     DBuf dy_dyn, dy_fc, dy_X1, dy_X2, dy_P1, dy_P2, dy_asr, dy_ws, dy_off, dy_qs, dy_qs2, dy_out,
-        dy_freq, dy_eig, dy_zbd, dy_qpd, dy_massd, dy_P;
+        dy_freq, dy_eig, dy_zbd, dy_qpd, dy_massd, dy_P, dy_rmass;
     // device time of the last lelph_interp_dvscf: weights, R -> q, long range, total, nq
     hipEvent_t ip_ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double ip_ms[5] = {0, 0, 0, 0, 0};
@@ -758,7 +758,8 @@ static void free_ctx(lelph_ctx* c)
                     &c->lr_q, &c->lr_zv, &c->lr_zb, &c->lr_qp, &c->rot_idx, &c->rot_sr,
                     &c->dy_dyn, &c->dy_fc, &c->dy_X1, &c->dy_X2, &c->dy_P1, &c->dy_P2,
                     &c->dy_asr, &c->dy_ws, &c->dy_off, &c->dy_qs, &c->dy_qs2, &c->dy_out,
-                    &c->dy_freq, &c->dy_eig, &c->dy_zbd, &c->dy_qpd, &c->dy_massd, &c->dy_P};
+                    &c->dy_freq, &c->dy_eig, &c->dy_zbd, &c->dy_qpd, &c->dy_massd, &c->dy_P,
+                    &c->dy_rmass};
     for (DBuf* b : bufs) b->release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -2390,6 +2391,38 @@ int lelph_dyn_interp(lelph_ctx* c, int64_t nq, const double* qpts, double* dyn_o
         }
         HIP_TRY(hipStreamSynchronize(c->stream));
     }
+    return LELPH_OK;
+}
+
+int lelph_qe_parse_dyn(const char* path, int natom, double* qpt, double* masses, double* dynd);
+
+int lelph_read_dyn_qe(lelph_ctx* c, const char* dyn_file, double* qpt, double* omega,
+                      double* pol_vecs, double* amass)
+{
+    if (!c || !dyn_file || !qpt || !omega || !pol_vecs) return LELPH_ERR_ARG;
+    const int na = c->lat.natom, nm = 3 * na;
+    if (nm > DYN_MAX_NM) return LELPH_ERR_UNSUPPORTED;
+    std::vector<double> m((size_t)na), d((size_t)2 * nm * nm);
+    LELPH_TRY(lelph_qe_parse_dyn(dyn_file, na, qpt, m.data(), d.data()));
+    LELPH_TRY(c->dy_out.ensure(sizeof(z64) * nm * nm));
+    LELPH_TRY(c->dy_eig.ensure(sizeof(z64) * nm * nm));
+    LELPH_TRY(c->dy_freq.ensure(sizeof(double) * nm));
+    LELPH_TRY(c->dy_rmass.ensure(sizeof(double) * na));
+    HIP_TRY(hipMemcpyAsync(c->dy_out.p, d.data(), sizeof(z64) * nm * nm, hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->dy_rmass.p, m.data(), sizeof(double) * na, hipMemcpyHostToDevice,
+                           c->stream));
+    // zheev of the Hermitian part (read_dyn.c:236-271) on the device
+    hipLaunchKernelGGL(k_herm_jacobi, dim3(1), dim3(64), 0, c->stream, c->dy_out.as<z64>(), nm,
+                       c->dy_rmass.as<double>(), 1, c->dy_freq.as<double>(), c->dy_eig.as<z64>(),
+                       40);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(omega, c->dy_freq.p, sizeof(double) * nm, hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(pol_vecs, c->dy_eig.p, sizeof(z64) * nm * nm, hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (amass) std::memcpy(amass, m.data(), sizeof(double) * na);
     return LELPH_OK;
 }
 

@@ -37,6 +37,7 @@ EXPORTS = [
     "lelph_interp_set_dvscf", "lelph_read_dvscf_qe", "lelph_set_dvscf_file",
     "lelph_write_dvscf_qe", "lelph_interp_timings", "lelph_dyn_interp_begin",
     "lelph_dyn_interp_add_coarse", "lelph_dyn_interp_finish", "lelph_dyn_interp",
+    "lelph_read_dyn_qe", "lelph_read_patterns_qe", "lelph_write_dyn_qe",
 ]
 
 
@@ -114,6 +115,11 @@ def lib():
                                              C.c_double, C.c_double, C.c_int]
         L.lelph_dyn_interp_add_coarse.argtypes = [C.c_void_p, c_double_p, c_double_p, c_double_p]
         L.lelph_dyn_interp_finish.argtypes = [C.c_void_p]
+        L.lelph_read_dyn_qe.argtypes = [C.c_void_p, C.c_char_p, c_double_p, c_double_p,
+                                        c_double_p, c_double_p]
+        L.lelph_read_patterns_qe.argtypes = [C.c_char_p, C.c_int, c_double_p]
+        L.lelph_write_dyn_qe.argtypes = [C.c_char_p, C.c_int, c_double_p, c_double_p,
+                                         c_double_p]
         L.lelph_dyn_interp.argtypes = [C.c_void_p, C.c_int64, c_double_p, c_double_p, c_double_p,
                                        c_double_p]
         _lib = L
@@ -410,6 +416,17 @@ class Context:
                "lelph_dyn_interp")
         return dyn, om, eig
 
+    def read_dyn_qe(self, path):
+        """read_dyn_qe (src/io/qe/read_dyn.c:38-293): (q, omega, pol_vecs, masses)."""
+        na, nm = self.sys.natom, self.sys.nmodes
+        q = np.zeros(3)
+        om = np.zeros(nm)
+        pv = np.zeros((nm, nm), np.complex128)
+        am = np.zeros(na)
+        _check(lib().lelph_read_dyn_qe(self.h, os.fsencode(path), dptr(q), dptr(om), dptr(pv),
+                                       dptr(am)), "lelph_read_dyn_qe")
+        return q, om, pv, am
+
     def interp_timings(self) -> dict:
         t = np.zeros(5)
         _check(lib().lelph_interp_timings(self.h, dptr(t), 5), "lelph_interp_timings")
@@ -475,3 +492,21 @@ def wigner_seitz(grid, lat, eps, rvec_m=None, rvec_n=None):
             continue
         _check(rc, "lelph_wigner_seitz")
         return vecs[: n.value].copy(), degen
+
+
+def read_patterns_qe(path, natom) -> np.ndarray:
+    """read_pattern_qe (src/io/qe/read_patterns.c:18-125), host code."""
+    nm = 3 * natom
+    out = np.zeros((nm, nm), np.complex128)
+    _check(lib().lelph_read_patterns_qe(os.fsencode(path), int(natom), dptr(out)),
+           "lelph_read_patterns_qe")
+    return out
+
+
+def write_dyn_qe(path, qpt, dyn, masses):
+    """write_dyn_qe (src/io/qe/write_dyn_dvscf_qe.c:18-97), host code."""
+    m = np.ascontiguousarray(masses, np.float64)
+    _check(lib().lelph_write_dyn_qe(os.fsencode(path), len(m),
+                                    dptr(np.ascontiguousarray(qpt, np.float64)),
+                                    dptr(np.ascontiguousarray(dyn, np.complex128)), dptr(m)),
+           "lelph_write_dyn_qe")
