@@ -139,6 +139,25 @@ def aux_measurements(s, ctx):
             "r2q_fp64_tflops": flops / (tm["r2q_ms"] / 1e3) / 1e12,
             "r2q_frac_fp64_peak": flops / (tm["r2q_ms"] / 1e3) / 1e12 / 78.6,
             "r2q_algorithmic_GBps": byts / (tm["r2q_ms"] / 1e3) / 1e9}
+        # dynamical matrices of the same cell: coarse 4x4x4 -> fine 12^3 with
+        # the dipole long range and the simple sum rule, eigenpairs on the device
+        rng = np.random.default_rng(5)
+        masses = np.linspace(12.0, 16.0, na)
+        t0 = time.perf_counter()
+        ic.dyn_interp_begin((4, 4, 4), masses=masses, lr=lr, eta_ph=1.0, ecut=30.0,
+                            asr="simple")
+        for q in c0.qpts:
+            e, _ = np.linalg.qr(rng.standard_normal((c0.nmodes,) * 2)
+                                + 1j * rng.standard_normal((c0.nmodes,) * 2))
+            ic.dyn_interp_add_coarse(q, np.sort(rng.uniform(0.001, 0.01, c0.nmodes)),
+                                     e / np.sqrt(np.repeat(masses, 3))[None, :])
+        ic.dyn_interp_finish()
+        t_dc = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ic.dyn_interp(fine)
+        out["dyn_interp_c4"] = {"coarse_side_s": t_dc,
+                                "fine_wall_s": time.perf_counter() - t0,
+                                "nq_fine": len(fine)}
     finally:
         ic.close()
     # D-matrices and q-star rotation at the headline sizes
