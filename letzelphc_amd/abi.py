@@ -151,7 +151,7 @@ def pack_system(sys) -> Packed:
     lat.nmodes = sys.nmodes
     lat.nmag = sys.nmag
     lat.dimension = sys.dimension.encode()
-    lat.is_soc_present = 0
+    lat.is_soc_present = int(bool(getattr(sys.pseudo, "soc", False)))
     for i in range(3):
         lat.fft_dims[i] = int(sys.fft_dims[i])
     for i in range(9):
@@ -182,7 +182,15 @@ def pack_system(sys) -> Packed:
     pp.nltimesj = ps.nltimesj
     pp.PP_table = dptr(hold(ps.PP_table, np.float64))
     pp.Fsign = dptr(hold(ps.Fsign, np.float64))
-    pp.fCoeff = None
+    fco = getattr(ps, "fcoeff", None)
+    if fco is not None:
+        arr = (c_double_p * len(fco))()
+        for i, f in enumerate(fco):
+            arr[i] = None if f is None else dptr(hold(f, np.complex128))
+        keep.append(arr)
+        pp.fCoeff = C.cast(arr, C.POINTER(c_double_p))
+    else:
+        pp.fCoeff = None
     locs = (LocalPseudo * ps.ntype)()
     for t, (v, r, rab, z) in enumerate(ps.loc):
         locs[t].Vloc_atomic = dptr(hold(v, np.float64))

@@ -80,6 +80,8 @@ class PseudoData:
     loc_A: float             # V_loc(r) = -e2 Z erf(r)/r + loc_A exp(-r^2/loc_b)
     loc_b: float
     qmax_abs: float
+    soc: bool = False        # fully relativistic projectors (j = l +- 1/2)
+    fcoeff: list = None      # [lidx * ntype + itype] -> (2l+1, 2l+1, 2, 2) or None
 
 
 @dataclass
@@ -208,26 +210,93 @@ def _radial_grid(Z: float, n: int = 1001, xmin: float = -7.0, dx: float = 0.0125
     return r, r * dx
 
 
+ELPH_SQRT2 = 1.4142136  # src/common/constants.h:12, enters the f coefficients
+
+
+def _cmat(m1, m2):
+    """real -> complex spherical-harmonic transformation element
+    (Cmat, src/nonloc/fcoeff.c:203-248)"""
+    a1, a2 = abs(m1), abs(m2)
+    if a1 != a2:
+        return 0.0
+    if m1 == 0 and m2 == 0:
+        return 1.0
+    if m1 == a1 and m2 == a2:
+        return (-1.0) ** a1 / ELPH_SQRT2
+    if m1 == -a1 and m2 == a2:
+        return 1.0 / ELPH_SQRT2
+    if m1 == a1 and m2 == -a2:
+        return 1j * (-1.0) ** a1 / ELPH_SQRT2
+    return -1j / ELPH_SQRT2
+
+
+def _cg(jp1, down, l, twomj):
+    """Clebsch-Gordan alpha (CGCoeff, src/nonloc/fcoeff.c:250-283)"""
+    mj = twomj / 2.0
+    if jp1:
+        m = mj - 0.5
+        return math.sqrt((l - m) / (2 * l + 1.0)) if down else math.sqrt((l + m + 1) / (2 * l + 1.0))
+    m = mj + 0.5
+    return -math.sqrt((l + m) / (2 * l + 1.0)) if down else math.sqrt((l - m + 1) / (2 * l + 1.0))
+
+
+def _umat(jp1, down, l, twomj, m2):
+    """U (Umat, src/nonloc/fcoeff.c:166-201)"""
+    if jp1:
+        m = (twomj - 1) // 2
+        return _cmat(m + 1 if down else m, m2)
+    m = (twomj + 1) // 2
+    return _cmat(m if down else m - 1, m2)
+
+
+def soc_fcoeff(l: int, twoj: int) -> np.ndarray:
+    """f^{l,j}_{m m' s s'} = sum_mj alpha U conj(alpha U), (2l+1, 2l+1, 2, 2)
+    (alloc_and_Compute_f_Coeff, src/nonloc/fcoeff.c:24-143)"""
+    jp1 = twoj == 2 * l + 1
+    if not jp1 and twoj != 2 * l - 1:
+        raise ValueError("inconsistent j")
+    f = np.zeros((2 * l + 1, 2 * l + 1, 2, 2), np.complex128)
+    for i1 in range(2 * l + 1):
+        for i2 in range(2 * l + 1):
+            for s1 in range(2):
+                for s2 in range(2):
+                    acc = 0.0
+                    for mj in range(-twoj, twoj + 1, 2):
+                        acc += (_cg(jp1, s1, l, mj) * _cg(jp1, s2, l, mj) *
+                                _umat(jp1, s1, l, mj, i1 - l) *
+                                np.conj(_umat(jp1, s2, l, mj, i2 - l)))
+                    f[i1, i2, s1, s2] = acc
+    return f
+
+
 def _make_pseudo(ntype, Zvals, qmax_abs, lmax=1, Dl=(1.0, -0.5), kb_alpha=1.5,
-                 loc_A=-2.0, loc_b=1.2):
+                 loc_A=-2.0, loc_b=1.2, soc=False):
     from scipy.special import erf
-    nlj = lmax + 1
+    # scalar: one channel per l; fully relativistic: s 1/2, then p 1/2 and
+    # p 3/2 with the same projector and D (so that sum_j f^{l,j} = 1 leaves
+    # the non-local part of the scalar pseudopotential)
+    chans = [(l, 0) for l in range(lmax + 1)] if not soc else \
+        [(0, 1)] + [(l, tj) for l in range(1, lmax + 1) for tj in (2 * l - 1, 2 * l + 1)]
+    nlj = len(chans)
     PP = np.zeros((nlj, ntype, 3))
     Fs = np.zeros((nlj, ntype))
     D = np.zeros((nlj, ntype))
-    for li in range(nlj):
+    for li, (l, tj) in enumerate(chans):
         for t in range(ntype):
-            PP[li, t] = (li + 1, 0, 1)
-            D[li, t] = Dl[li]
-            Fs[li, t] = np.sign(Dl[li])
+            PP[li, t] = (l + 1, tj, 1)
+            D[li, t] = Dl[l]
+            Fs[li, t] = np.sign(Dl[l])
     loc = []
     for t in range(ntype):
         Z = Zvals[t]
         r, rab = _radial_grid(Z if Z > 1 else 1.0)
         v = -E2 * Z * erf(r) / r + loc_A * np.exp(-r * r / loc_b)
         loc.append((v, r, rab, float(Z)))
+    fco = None
+    if soc:
+        fco = [soc_fcoeff(l, tj) for (l, tj) in chans for _ in range(ntype)]
     return PseudoData(ntype, lmax, nlj, PP, Fs, D, kb_alpha, loc, loc_A, loc_b,
-                      qmax_abs)
+                      qmax_abs, soc, fco)
 
 
 def kb_radial_ft(l: int, q: np.ndarray, a: float) -> np.ndarray:
@@ -311,7 +380,7 @@ def _rot_z(th):
 
 def make_system(name, seed, alat, crys_pos, atom_type, Zvals, fft_dims, kgrid,
                 ecut, nbnds, syms, timerev, qpts, nspin=1, nspinor=1, nmag=1,
-                dimension="3", with_wfc=True):
+                dimension="3", with_wfc=True, soc=False):
     alat = np.asarray(alat, np.float64)
     blat = reciprocal_vecs(alat)
     vol = float(np.linalg.det(alat))
@@ -321,7 +390,9 @@ def make_system(name, seed, alat, crys_pos, atom_type, Zvals, fft_dims, kgrid,
     kibz, kmap = _reduce_kgrid(kbz, syms, alat, blat)
     kcart = kibz @ (blat / (2 * RPI)).T
     qmax = max([float(np.max(np.abs(q))) for q in qpts] + [0.0])
-    ps = _make_pseudo(int(max(atom_type)) + 1, Zvals, qmax)
+    if soc and nspinor != 2:
+        raise ValueError("fully relativistic projectors need nspinor = 2")
+    ps = _make_pseudo(int(max(atom_type)) + 1, Zvals, qmax, soc=soc)
     s = System(name, seed, alat, blat, vol, tuple(int(x) for x in fft_dims),
                natom, pos, np.asarray(atom_type, np.int32), nspin, nspinor,
                nbnds, 3 * natom, nmag, dimension, syms, timerev, kibz,
@@ -385,7 +456,7 @@ def config(cfg: str, nbnds=None, kgrid=None, with_wfc=True, **kw) -> System:
 
 def tiny(nspin=1, nspinor=1, nmag=1, nbnds=4, fft=(12, 12, 12), kgrid=(2, 2, 2),
          syms="trev", ecut=6.0, seed=7, q=(0.0, 0.0, 0.5), with_wfc=True,
-         dimension="3") -> System:
+         dimension="3", soc=False) -> System:
     """A small fcc cell for branch coverage (spin, spinors, symmetries);
     dimension "2": a hexagonal slab cell (a = 5, c = 20 bohr, 12x12x48)
     with the 2-D Coulomb cut-off."""
@@ -397,7 +468,7 @@ def tiny(nspin=1, nspinor=1, nmag=1, nbnds=4, fft=(12, 12, 12), kgrid=(2, 2, 2),
                            [[1 / 3, 2 / 3, 0.5], [2 / 3, 1 / 3, 0.56]], [0, 1], [3.0, 5.0],
                            (12, 12, 48), (kgrid[0], kgrid[1], 1), ecut, nbnds, sl, 1,
                            [(0.5, 0.0, 0.0)], nspin=nspin, nspinor=nspinor, nmag=nmag,
-                           dimension="2", with_wfc=with_wfc)
+                           dimension="2", with_wfc=with_wfc, soc=soc)
     if syms == "id":
         sl, tr = [(I3, np.zeros(3), 0)], 0
     elif syms == "trev":
@@ -411,12 +482,12 @@ def tiny(nspin=1, nspinor=1, nmag=1, nbnds=4, fft=(12, 12, 12), kgrid=(2, 2, 2),
         tr = 1
         return make_system("tiny", seed, a * I3, [[0, 0, 0], [0.5, 0.5, 0.3]], [0, 1],
                            [3.0, 5.0], fft, kgrid, ecut, nbnds, sl, tr, [q],
-                           nspin=nspin, nspinor=nspinor, nmag=nmag, with_wfc=with_wfc)
+                           nspin=nspin, nspinor=nspinor, nmag=nmag, with_wfc=with_wfc, soc=soc)
     else:
         raise ValueError(syms)
     return make_system("tiny", seed, fcc_cell(a), [[0, 0, 0], [0.25, 0.25, 0.25]], [0, 1],
                        [3.0, 5.0], fft, kgrid, ecut, nbnds, sl, tr, [q], nspin=nspin,
-                       nspinor=nspinor, nmag=nmag, with_wfc=with_wfc)
+                       nspinor=nspinor, nmag=nmag, with_wfc=with_wfc, soc=soc)
 
 
 def kq_pairs(sys: System, iq: int, kminusq: bool = False):

@@ -84,3 +84,26 @@ void rg_rotate_dvscf(const double* q, const double* in, const double* R, const d
     rotate_dvscf(q, (const ELPH_cmplx*)in, &s, &L, composite != 0, (ELPH_cmplx*)out,
                  MPI_COMM_SELF);
 }
+
+/* alloc_and_Compute_f_Coeff (src/nonloc/fcoeff.c:24) for one channel (l, 2j)
+ * of one atom type; out (2l+1, 2l+1, 2, 2) complex */
+#include "nonloc/fcoeff.h"
+int rg_fcoeff(int l, int twoj, double* out)
+{
+    struct Lattice L;
+    memset(&L, 0, sizeof(L));
+    L.nspinor = 2;
+    L.is_soc_present = true;
+    struct Pseudo P;
+    memset(&P, 0, sizeof(P));
+    double pp[3] = {l + 1.0, (double)twoj, 1.0};
+    P.PP_table = pp;
+    P.nltimesj = 1;
+    P.ntype = 1;
+    alloc_and_Compute_f_Coeff(&L, &P);
+    if (!P.fCoeff || !P.fCoeff[0]) return 1;
+    const int n = (2 * l + 1) * (2 * l + 1) * 4;
+    memcpy(out, P.fCoeff[0], sizeof(ELPH_cmplx) * n);
+    free_f_Coeff(&P);
+    return 0;
+}

@@ -19,6 +19,7 @@ and tests/golden/interp_vectors.npz:
   dVlong_range_kernel                src/dvloc/dVlong_range_kernel.c:31-458
   multiply_eikr                      src/dvloc/multiply_eikr.c:14-64
   rotate_dvscf                       src/symmetries/rotate_dvscf.c:22-368
+  alloc_and_Compute_f_Coeff          src/nonloc/fcoeff.c:24-143
 
 Run:  python tests/golden/make_ref_vectors.py
 The fixture is data (seeded inputs, the reference's outputs); the tests
@@ -228,6 +229,8 @@ def load_interp(L):
     L.rg_multiply_eikr.argtypes = [dp, dp, C.POINTER(ll), ll, C.c_int]
     L.rg_rotate_dvscf.argtypes = [dp, dp, dp, dp, C.c_int, C.POINTER(ll), ll, ll, C.c_int, dp,
                                   dp, C.c_int, dp]
+    L.rg_fcoeff.argtypes = [C.c_int, C.c_int, dp]
+    L.rg_fcoeff.restype = C.c_int
     if L.rg_init() != 0:
         raise RuntimeError("MPI initialisation of the reference library failed")
     return L
@@ -341,6 +344,12 @@ def make_interp(L, seed=77):
                                   P(blat), comp, o.ctypes.data_as(dp))
                 outs.append(o)
         f.update({f"rot{nmag}_in": din, f"rot{nmag}_out": np.array(outs)})
+    # spin-orbit f coefficients of every channel up to l = 3
+    for l in range(4):
+        for tj in ((1,) if l == 0 else (2 * l - 1, 2 * l + 1)):
+            fo = np.zeros((2 * l + 1, 2 * l + 1, 2, 2), np.complex128)
+            assert L.rg_fcoeff(l, tj, fo.ctypes.data_as(dp)) == 0
+            f[f"fc_l{l}_j{tj}"] = fo
     f.update(rot_q=qr, rot_alat=alat, rot_blat=blat,
              rot_R=np.array([s[0] for s in syms]), rot_tau=np.array([s[1] for s in syms]),
              rot_tr=np.array([s[2] for s in syms]))

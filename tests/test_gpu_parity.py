@@ -115,7 +115,7 @@ def test_compute_elphq_reduced(c0, c0ctx, prec, tol):
 
 
 @pytest.mark.parametrize("kind", ["spin2", "spinor_nmag1", "spinor_nmag4", "cubic_syms",
-                                  "bare", "dfpt_local", "slab_2d"])
+                                  "bare", "dfpt_local", "slab_2d", "soc", "soc_nmag4"])
 def test_tiny_variants(kind):
     kw = dict(nbnds=4)
     kernel = "dfpt"
@@ -131,6 +131,10 @@ def test_tiny_variants(kind):
         kernel = kind
     elif kind == "slab_2d":
         kw.update(dimension="2")
+    elif kind == "soc":
+        kw.update(nspinor=2, nmag=1, soc=True)
+    elif kind == "soc_nmag4":
+        kw.update(nspinor=2, nmag=4, soc=True, syms="cubic", q=(0.0, 0.5, 0.0))
     s = synth.tiny(**kw)
     o = O.OracleSystem(s)
     say(f"case: tiny {kind}")

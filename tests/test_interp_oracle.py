@@ -392,3 +392,41 @@ def test_dyn_interpolation_coarse_and_asr(asr):
         fc = od.fc * M[None]
         s_ = fc.reshape(len(fc), 2, 3, 2, 3).sum(axis=(0, 3))
         assert np.abs(s_).max() < 1e-12 * np.abs(fc).max()
+
+
+# ---------------------------------------------------------------------------
+# spin-orbit f coefficients and the fully relativistic non-local part
+# ---------------------------------------------------------------------------
+
+def test_soc_fcoeff_vs_reference_and_completeness():
+    """f^{l,j} (the synthetic inputs' restatement of fcoeff.c) against the
+    reference's compiled alloc_and_Compute_f_Coeff, and sum_j f^{l,j} = 1
+    (to the reference's truncated sqrt(2) = 1.4142136)"""
+    f = _load()
+    for l in range(4):
+        tjs = (1,) if l == 0 else (2 * l - 1, 2 * l + 1)
+        tot = 0
+        for tj in tjs:
+            got = synth.soc_fcoeff(l, tj)
+            assert np.abs(got - f[f"fc_l{l}_j{tj}"]).max() < 1e-15, (l, tj)
+            tot = tot + got
+        I = np.einsum("ab,st->abst", np.eye(2 * l + 1), np.eye(2))
+        assert np.abs(tot - I).max() < 1e-7, l
+
+
+def test_soc_nonlocal_reduces_to_scalar():
+    """With both j = l +- 1/2 channels carrying the same projector and D,
+    the fully relativistic non-local term of the oracle equals the scalar
+    spinor one (sum_j f^{l,j} = 1), so g agrees to the sqrt(2) truncation."""
+    say("case: SOC non-local with identical j channels vs scalar spinors")
+    a = synth.tiny(nspinor=2, soc=True, nbnds=4)
+    b = synth.tiny(nspinor=2, soc=False, nbnds=4)
+    assert np.array_equal(a.wfcs[0].wfc, b.wfcs[0].wfc)
+    oa, ob = O.OracleSystem(a), O.OracleSystem(b)
+    dv = oa.dv_total(0)
+    for pr in synth.kq_pairs(a, 0)[:2]:
+        z = np.zeros((a.nmodes, a.nspin, a.nbnds, a.nbnds), np.complex128)
+        ga = oa.add_nonlocal(0, *pr, z)
+        gb = ob.add_nonlocal(0, *pr, z)
+        assert np.abs(gb).max() > 0
+        assert rel_err(ga, gb) < 1e-6, pr
