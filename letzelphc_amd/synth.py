@@ -454,6 +454,58 @@ def config(cfg: str, nbnds=None, kgrid=None, with_wfc=True, **kw) -> System:
     raise ValueError(cfg)
 
 
+# c3: eight uneven cells (FFT grid, lattice constant or hex (a, c), atoms,
+# bands, target plane waves); 16 (k, k+q) pairs each
+C3_CELLS = [
+    ((24, 24, 24), "fcc", 10.26, 2, 8, None),
+    ((32, 32, 32), "sc", 9.0, 2, 16, 500),
+    ((36, 36, 36), "sc", 10.5, 4, 16, 900),
+    ((45, 45, 108), "hex", (5.9, 14.2), 3, 24, 1500),
+    ((48, 48, 48), "sc", 13.0, 6, 32, 3000),
+    ((60, 60, 60), "sc", 15.5, 8, 32, 5000),
+    ((64, 64, 64), "sc", 16.5, 10, 48, 8000),
+    ((72, 72, 72), "sc", 19.0, 12, 64, 12000),
+]
+
+
+def config_c3(with_wfc=True, cells=None):
+    """The batching workload: eight cells with FFT grids 24^3 ... 72^3 and
+    45x45x108, 180 ... 12,000 plane waves, 8 ... 64 bands, 6 ... 36 modes,
+    16 (k, k+q) pairs each (identity symmetry, so all 16 k are iBZ).  The
+    cut-off of each cell gives its target plane-wave count, capped so that
+    4 G_max fits the grid (no aliasing of the products)."""
+    out = []
+    I3 = np.eye(3)
+    for ic, (dims, kind, a, nat, nb, npw) in enumerate(C3_CELLS):
+        if cells is not None and ic not in cells:
+            continue
+        rng = rng_for(3000 + ic, 9)
+        if kind == "fcc":
+            out.append(config("c0", nbnds=nb, kgrid=(2, 2, 4), with_wfc=with_wfc))
+            continue
+        if kind == "hex":
+            alat = hex_cell(*a)
+            kgrid, q = (4, 4, 1), (0.25, 0.0, 0.0)
+            dim = "2"
+        else:
+            alat = a * I3
+            kgrid, q = (2, 2, 4), (0.0, 0.0, 0.25)
+            dim = "3"
+        pos = rng.uniform(0.05, 0.95, (nat, 3))
+        if kind == "hex":
+            pos[:, 2] = 0.5 + 0.1 * (pos[:, 2] - 0.5)
+        typ = [i % 2 for i in range(nat)]
+        blat = reciprocal_vecs(np.asarray(alat, np.float64))
+        bn = np.linalg.norm(blat, axis=0)
+        cap = min(((dims[i] - 1) / 4.0 * bn[i]) ** 2 for i in range(3))
+        vol = abs(np.linalg.det(alat))
+        ecut = min(0.95 * cap, (npw * 6 * math.pi ** 2 / vol) ** (2.0 / 3.0))
+        out.append(make_system(f"c3_{ic}", 3000 + ic, alat, pos, typ, [4.0, 6.0], dims, kgrid,
+                               ecut, nb, [(I3, np.zeros(3), 0)], 0, [q], dimension=dim,
+                               with_wfc=with_wfc))
+    return out
+
+
 def tiny(nspin=1, nspinor=1, nmag=1, nbnds=4, fft=(12, 12, 12), kgrid=(2, 2, 2),
          syms="trev", ecut=6.0, seed=7, q=(0.0, 0.0, 0.5), with_wfc=True,
          dimension="3", soc=False) -> System:
