@@ -165,6 +165,13 @@ struct lelph_ctx
     double ip_ms[5] = {0, 0, 0, 0, 0};
     DBuf nlK, nlKp, Y_K, Y_Kp, wr_K, wr_Kp, beta_K, beta_Kp, bb_K, bb_Kp, bbpart, ebuf;
 
+    // psi box reuse along a batch: the k+q box of the previous pair is the k
+    // box of the next when their (iBZ k, symmetry) agree, up to the umklapp
+    // shift of the previous k+q side
+    bool box_reuse = false, box_prev_valid = false;
+    int box_prev_ik = -1, box_prev_sym = -1, box_prev_prec = -1;
+    int box_prev_g0[3] = {0, 0, 0};
+
     // per-pair events: [0] start, [1] boxes built, [2] contraction done,
     // [3] local done, [4] non-local done
     std::vector<hipEvent_t> evs;
@@ -400,35 +407,74 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
     const bool pruned = std::is_same<T, float>::value && fft_has_c(Nx) && fft_has_c(Ny) &&
                         fft_has_c(Nz) && Nx <= 1024 && plane_lds_bytes(Ny, Nz) <= 160 * 1024 &&
                         nbox <= 65535;
-    if (!pruned)
+    // the k side of this pair is the k+q side of the previous one: swap the
+    // boxes and undo that side's umklapp shift instead of transforming again
+    const int prec_tag = std::is_same<T, float>::value ? 1 : 0;
+    const bool reuse = c->box_reuse && c->box_prev_valid && c->box_prev_ik == pr.ik &&
+                       c->box_prev_sym == pr.ksym && c->box_prev_prec == prec_tag;
+    int g0[3];
+    for (int i = 0; i < 3; ++i) g0[i] = (int)std::lrint(gkq.G0c[i]);
+    c->box_prev_valid = false;
+    if (reuse)
     {
-        HIP_TRY(hipMemsetAsync(c->box_k.p, 0, bb, c->stream));
-        HIP_TRY(hipMemsetAsync(c->box_kq.p, 0, bb, c->stream));
+        std::swap(c->box_k.p, c->box_kq.p);
+        std::swap(c->box_k.n, c->box_kq.n);
+        const int* pg = c->box_prev_g0;
+        if (pg[0] || pg[1] || pg[2])
+        {
+            hipLaunchKernelGGL(k_box_shift<T>, dim3((unsigned)cdiv(c->nf, 256)), dim3(256), 0,
+                               c->stream, c->box_k.as<cpx<T>>(), nbox, Nx, Ny, Nz, pg[0], pg[1],
+                               pg[2], -1);
+            HIP_TRY(hipGetLastError());
+        }
+        if (!pruned) HIP_TRY(hipMemsetAsync(c->box_kq.p, 0, bb, c->stream));
     }
-    hipLaunchKernelGGL(k_gindex, dim3((unsigned)cdiv(wk.npw, 256)), dim3(256), 0, c->stream,
-                       wk.gvec, wk.npw, gk, Nx, Ny, Nz, c->idx_k.as<int>(), c->ph_k.as<z64>(),
-                       c->d_err);
+    else
+    {
+        if (!pruned)
+        {
+            HIP_TRY(hipMemsetAsync(c->box_k.p, 0, bb, c->stream));
+            HIP_TRY(hipMemsetAsync(c->box_kq.p, 0, bb, c->stream));
+        }
+        hipLaunchKernelGGL(k_gindex, dim3((unsigned)cdiv(wk.npw, 256)), dim3(256), 0, c->stream,
+                           wk.gvec, wk.npw, gk, Nx, Ny, Nz, c->idx_k.as<int>(),
+                           c->ph_k.as<z64>(), c->d_err);
+    }
     hipLaunchKernelGGL(k_gindex, dim3((unsigned)cdiv(wkq.npw, 256)), dim3(256), 0, c->stream,
                        wkq.gvec, wkq.npw, gkq, Nx, Ny, Nz, c->idx_kq.as<int>(),
                        c->ph_kq.as<z64>(), c->d_err);
+    bool done = false;
     if constexpr (std::is_same<T, float>::value)
         if (pruned)
         {
-            LELPH_TRY(pruned_side(c, wk, gk, c->idx_k.as<int>(), c->ph_k.as<z64>(), 0,
-                                  c->box_k.as<c32>()));
+            if (!reuse)
+                LELPH_TRY(pruned_side(c, wk, gk, c->idx_k.as<int>(), c->ph_k.as<z64>(), 0,
+                                      c->box_k.as<c32>()));
             LELPH_TRY(pruned_side(c, wkq, gkq, c->idx_kq.as<int>(), c->ph_kq.as<z64>(), 1,
                                   c->box_kq.as<c32>()));
-            return LELPH_OK;
+            done = true;
         }
-    hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)cdiv(nsb * wk.npw, 256)), dim3(256), 0,
-                       c->stream, wk.wfc, nsb, L.nspinor, wk.npw, c->idx_k.as<int>(),
-                       c->ph_k.as<z64>(), gk, c->nf, c->box_k.as<cpx<T>>());
-    hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)cdiv(nsb * wkq.npw, 256)), dim3(256), 0,
-                       c->stream, wkq.wfc, nsb, L.nspinor, wkq.npw, c->idx_kq.as<int>(),
-                       c->ph_kq.as<z64>(), gkq, c->nf, c->box_kq.as<cpx<T>>());
-    HIP_TRY(hipGetLastError());
-    LELPH_TRY(fft3d_batch<T>(c->ax, c->box_k.as<cpx<T>>(), nbox, +1, c->stream));
-    LELPH_TRY(fft3d_batch<T>(c->ax, c->box_kq.as<cpx<T>>(), nbox, +1, c->stream));
+    if (!done)
+    {
+        if (!reuse)
+            hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)cdiv(nsb * wk.npw, 256)), dim3(256),
+                               0, c->stream, wk.wfc, nsb, L.nspinor, wk.npw, c->idx_k.as<int>(),
+                               c->ph_k.as<z64>(), gk, c->nf, c->box_k.as<cpx<T>>());
+        hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)cdiv(nsb * wkq.npw, 256)), dim3(256), 0,
+                           c->stream, wkq.wfc, nsb, L.nspinor, wkq.npw, c->idx_kq.as<int>(),
+                           c->ph_kq.as<z64>(), gkq, c->nf, c->box_kq.as<cpx<T>>());
+        HIP_TRY(hipGetLastError());
+        if (!reuse) LELPH_TRY(fft3d_batch<T>(c->ax, c->box_k.as<cpx<T>>(), nbox, +1, c->stream));
+        LELPH_TRY(fft3d_batch<T>(c->ax, c->box_kq.as<cpx<T>>(), nbox, +1, c->stream));
+    }
+    if (c->box_reuse)
+    {
+        c->box_prev_valid = true;
+        c->box_prev_ik = pr.ikq;
+        c->box_prev_sym = pr.kqsym;
+        c->box_prev_prec = prec_tag;
+        std::memcpy(c->box_prev_g0, g0, sizeof(g0));
+    }
     return LELPH_OK;
 }
 
@@ -1322,9 +1368,57 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
             return LELPH_ERR_ARG;
         if (p.ksym < 0 || p.kqsym < 0 || p.ksym >= c->lat.nsym || p.kqsym >= c->lat.nsym)
             return LELPH_ERR_ARG;
-        LELPH_TRY(gkq_device(c, qpt, p, 1, non_loc, c->gbatch.as<z64>() + i * gsz,
-                             c->evs.data() + 5 * i));
     }
+    // order the pairs along k -> k+q chains: a pair follows the one whose
+    // k+q side (iBZ k, symmetry) is its own k side, so every second psi box
+    // is reused (build_boxes); results keep the caller's order
+    std::vector<int64_t> order;
+    order.reserve((size_t)npairs);
+    {
+        std::vector<int64_t> next((size_t)npairs, -1);
+        std::vector<char> has_prev((size_t)npairs, 0), used((size_t)npairs, 0);
+        std::vector<std::pair<int64_t, int64_t>> keys((size_t)npairs);
+        for (int64_t i = 0; i < npairs; ++i)
+            keys[i] = {(int64_t)pairs[i].ik * c->lat.nsym + pairs[i].ksym, i};
+        std::sort(keys.begin(), keys.end());
+        for (int64_t i = 0; i < npairs; ++i)
+        {
+            const int64_t want = (int64_t)pairs[i].ikq * c->lat.nsym + pairs[i].kqsym;
+            auto it = std::lower_bound(keys.begin(), keys.end(), std::make_pair(want, (int64_t)-1));
+            if (it != keys.end() && it->first == want && it->second != i && !has_prev[it->second])
+            {
+                next[i] = it->second;
+                has_prev[it->second] = 1;
+            }
+        }
+        for (int pass = 0; pass < 2; ++pass)
+            for (int64_t i = 0; i < npairs; ++i)
+            {
+                // chain heads first, then what is left of cycles
+                if (used[i] || (pass == 0 && has_prev[i])) continue;
+                for (int64_t j = i; j >= 0 && !used[j]; j = next[j])
+                {
+                    used[j] = 1;
+                    order.push_back(j);
+                }
+            }
+    }
+    c->box_reuse = true;
+    c->box_prev_valid = false;
+    for (int64_t n = 0; n < npairs; ++n)
+    {
+        const int64_t i = order[n];
+        const int st = gkq_device(c, qpt, pairs[i], 1, non_loc, c->gbatch.as<z64>() + i * gsz,
+                                  c->evs.data() + 5 * i);
+        if (st != LELPH_OK)
+        {
+            c->box_reuse = false;
+            c->box_prev_valid = false;
+            return st;
+        }
+    }
+    c->box_reuse = false;
+    c->box_prev_valid = false;
     HIP_TRY(hipMemcpyAsync(g_out, c->gbatch.p, sizeof(z64) * npairs * gsz, hipMemcpyDeviceToHost,
                            c->stream));
     LELPH_TRY(check_err(c));

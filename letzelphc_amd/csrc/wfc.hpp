@@ -104,3 +104,32 @@ __global__ void k_scatter(const z64* __restrict__ wfc, int64_t nsets, int nspino
         out[(2 * s + 1) * nf + id] = {(T)d2.x, (T)d2.y};
     }
 }
+
+// box[b][p] *= e^{sgn 2 pi i G0 . r_p} for every box b: the backward
+// transform of coefficients shifted by the integer G0 (shift theorem), so a
+// psi(r) box built for one umklapp vector serves another.  Exact integer
+// reduction of G0 . r before the FP64 sincospi.
+template <typename T>
+__global__ __launch_bounds__(256) void k_box_shift(cpx<T>* __restrict__ box, int64_t nbox, int Nx,
+                                                   int Ny, int Nz, int g0, int g1, int g2, int sgn)
+{
+    const int64_t nf = (int64_t)Nx * Ny * Nz;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nf) return;
+    const int ix = (int)(p / ((int64_t)Ny * Nz)), iy = (int)((p / Nz) % Ny), iz = (int)(p % Nz);
+    auto fr = [](int g, int i, int n) {
+        int64_t m = ((int64_t)g * i) % n;
+        if (m < 0) m += n;
+        return (double)m / n;
+    };
+    double f = fr(g0, ix, Nx) + fr(g1, iy, Ny) + fr(g2, iz, Nz);
+    f -= floor(f);
+    double s, c;
+    sincospi(2.0 * f * sgn, &s, &c);
+    const cpx<T> e = {(T)c, (T)s};
+    for (int64_t b = 0; b < nbox; ++b)
+    {
+        cpx<T>& v = box[b * nf + p];
+        v = cmul(v, e);
+    }
+}

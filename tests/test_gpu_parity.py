@@ -267,3 +267,39 @@ def test_elph_q_rotate_gpu(nb):
             assert rel_err(got, ref) < TOL64, (nb, tr)
     finally:
         ctx.close()
+
+
+def _chain_links(pairs):
+    """pairs whose k side (iBZ k, symmetry) is another pair's k+q side: the
+    psi box the batch takes over instead of transforming again"""
+    ks = {(ik, ksym) for (_, ik, _, ksym) in pairs}
+    return sum((ikq, kqsym) in ks and (ikq, kqsym) != (ik, ksym)
+               for (ikq, ik, kqsym, ksym) in pairs)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", TOL32), ("bf16x3", 5e-5)])
+@pytest.mark.parametrize("kminusq", [False, True])
+def test_batch_box_reuse(c0, c0ctx, prec, tol, kminusq):
+    """A batch walks k -> k+q chains and takes each pair's k box from the
+    previous pair's k+q box, shifted back by its umklapp vector; pair by pair
+    (every box transformed) gives the same g.  The pairs go in reversed
+    order so the batch has to rebuild the chains itself."""
+    s, o, dv = c0
+    pairs = synth.kq_pairs(s, 0, kminusq=kminusq)[::-1]
+    links = _chain_links(pairs)
+    say(f"case: box reuse {prec} kminusq={kminusq} pairs={len(pairs)} links={links}")
+    assert links >= len(pairs) // 2
+    c0ctx.set_precision(prec)
+    try:
+        gb = c0ctx.gkq_batch(pairs, non_loc=True, qpt=s.qpts[0])
+        one = np.stack([c0ctx.gkq(*p, non_loc=True, qpt=s.qpts[0]) for p in pairs])
+    finally:
+        c0ctx.set_precision("fp64")
+    err = rel_err(gb, one)
+    say(f"  batch vs pair by pair: {err:.3e}")
+    assert err < tol
+    if prec == "fp64":
+        ref = o.compute_elphq(0, dv, non_loc=True, kminusq=kminusq, k_first=0, k_count=4)
+        n = len(pairs)
+        assert rel_err(gb[::-1][:4], ref) < TOL64
+        assert n == len(s.kpt_bz_crys)
