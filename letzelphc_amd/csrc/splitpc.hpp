@@ -89,13 +89,14 @@ __device__ inline void pc_produce(const PcRaw& r, lds_char* LA, int rb, int lane
             aim[p][k] = pi[p];
         }
     }
+    const int slot = rb * 1024 + lane * 16;
 #pragma unroll
     for (int p = 0; p < NP; ++p)
     {
         *reinterpret_cast<__attribute__((address_space(3))) bf16x8*>(
-            LA + ((2 * p) * 4 + rb) * 1024 + lane * 16) = are[p];
+            LA + (2 * p) * 4096 + slot) = are[p];
         *reinterpret_cast<__attribute__((address_space(3))) bf16x8*>(
-            LA + ((2 * p + 1) * 4 + rb) * 1024 + lane * 16) = aim[p];
+            LA + (2 * p + 1) * 4096 + slot) = aim[p];
     }
 }
 
@@ -181,6 +182,114 @@ __device__ __forceinline__ void pc_split_step(int i, int n, const PcRaw2& rp, Pc
     if (i < n && i + 1 < n) pc_produce2<NP>(rp, L + ((i + 1) & 1) * A_BYTES, pw, lane);
     pc_load_raw2(rl, ap, dp, i + 3 < n ? i + 3 : n - 1);
     if (i < n) lds_barrier();
+}
+
+// one chunk of the consumer: 8 column blocks of 16, 12 MFMAs each
+template <int NP>
+__device__ __forceinline__ void pc_chunk16(const lds_char* LA, const lds_char* LB, int w4,
+                                           int lane, flt4v* accr, flt4v* acci)
+{
+    bf16x8 are[NP], aim[NP], nre[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+    {
+        are[p] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8*>(
+            LA + ((2 * p) * 4 + w4) * 1024 + lane * 16);
+        aim[p] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8*>(
+            LA + ((2 * p + 1) * 4 + w4) * 1024 + lane * 16);
+        nre[p] = neg_bf16x8(are[p]);
+    }
+    // B fragment of column block cb: plane, kq = lane/16, column cb*16 + lane%16
+    const lds_char* bq = LB + ((lane >> 4) * PC_TC + (lane & 15)) * 16;
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb)
+    {
+        bf16x8 br[NP], bi[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+        {
+            br[p] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8*>(
+                bq + (2 * p) * 8192 + cb * 256);
+            bi[p] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8*>(
+                bq + (2 * p + 1) * 8192 + cb * 256);
+        }
+#pragma unroll
+        for (int p1 = 0; p1 < NP; ++p1)
+#pragma unroll
+            for (int p2 = 0; p2 < NP; ++p2)
+            {
+                if (p1 + p2 > 1) continue; // the 3 products of the 2-piece split
+                // C += a conj(b): re = ar br + ai bi, im = ai br - ar bi
+                accr[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(are[p1], br[p2], accr[cb], 0, 0, 0);
+                acci[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aim[p1], br[p2], acci[cb], 0, 0, 0);
+                accr[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aim[p1], bi[p2], accr[cb], 0, 0, 0);
+                acci[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nre[p1], bi[p2], acci[cb], 0, 0, 0);
+            }
+    }
+}
+
+__device__ __forceinline__ void pc_flush_block(flt4v& ar, flt4v& ai, double* dr, double* di)
+{
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+    {
+        dr[e] += (double)ar[e];
+        di[e] += (double)ai[e];
+        ar[e] = 0.f;
+        ai[e] = 0.f;
+    }
+}
+
+// consumer: rows n0 + 16 w4 + .., columns m0 .. m0 + 127 (8 blocks of 16);
+// FP32 chains of PC_FLUSH chunks flushed into FP64
+template <int NP>
+__device__ __forceinline__ void pc_consume16(const lds_char* L, int n, int w4, int lane,
+                                             const SplitArgs& a, z64* __restrict__ part,
+                                             int split, int v, int is, int n0, int m0)
+{
+    constexpr int A_BYTES = pc_a_bytes<NP>();
+    constexpr int B_BYTES = pc_b_bytes<NP>();
+    flt4v accr[8], acci[8];
+    double dr[8][4], di[8][4];
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            accr[cb][e] = 0.f;
+            acci[cb][e] = 0.f;
+            dr[cb][e] = 0.0;
+            di[cb][e] = 0.0;
+        }
+    lds_barrier();
+    int u = 0, u3 = 0, nfl = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        pc_chunk16<NP>(L + u * A_BYTES, L + 2 * A_BYTES + u3 * B_BYTES, w4, lane, accr, acci);
+        if (++nfl == PC_FLUSH)
+        {
+            nfl = 0;
+#pragma unroll
+            for (int cb = 0; cb < 8; ++cb) pc_flush_block(accr[cb], acci[cb], dr[cb], di[cb]);
+        }
+        lds_barrier();
+        u ^= 1;
+        u3 = (u3 == 2) ? 0 : u3 + 1;
+    }
+    const int row = n0 + 16 * w4;
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const int rr = row + 4 * (lane >> 4) + e;
+            const int cc = m0 + cb * 16 + (lane & 15);
+            if (rr < a.nb && cc < a.nb)
+            {
+                const int64_t o = ((((int64_t)split * a.nm + v) * a.nspin + is) * a.nb + rr) * a.nb + cc;
+                part[o] = {dr[cb][e] + (double)accr[cb][e], di[cb][e] + (double)acci[cb][e]};
+            }
+        }
 }
 
 template <int NP>
@@ -276,92 +385,5 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
         return;
     }
 
-    // consumer: rows n0 + 16 w4 + .., columns m0 .. m0 + 127 (8 blocks)
-    flt4v accr[8], acci[8];
-    double dr[8][4], di[8][4];
-#pragma unroll
-    for (int cb = 0; cb < 8; ++cb)
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-        {
-            accr[cb][e] = 0.f;
-            acci[cb][e] = 0.f;
-            dr[cb][e] = 0.0;
-            di[cb][e] = 0.0;
-        }
-    lds_barrier();
-    int u = 0, u3 = 0, nfl = 0;
-    for (int i = 0; i < n; ++i)
-    {
-        const lds_char* LA = L + u * A_BYTES;
-        const lds_char* LB = L + 2 * A_BYTES + u3 * B_BYTES;
-        bf16x8 are[NP], aim[NP], nre[NP];
-#pragma unroll
-        for (int p = 0; p < NP; ++p)
-        {
-            are[p] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8*>(
-                LA + ((2 * p) * 4 + w4) * 1024 + lane * 16);
-            aim[p] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8*>(
-                LA + ((2 * p + 1) * 4 + w4) * 1024 + lane * 16);
-            nre[p] = neg_bf16x8(are[p]);
-        }
-        // B fragment of column block cb: plane, kq = lane/16, column cb*16 + lane%16
-        const lds_char* bq = LB + ((lane >> 4) * PC_TC + (lane & 15)) * 16;
-#pragma unroll
-        for (int cb = 0; cb < 8; ++cb)
-        {
-            bf16x8 br[NP], bi[NP];
-#pragma unroll
-            for (int p = 0; p < NP; ++p)
-            {
-                br[p] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8*>(
-                    bq + (2 * p) * 8192 + cb * 256);
-                bi[p] = *reinterpret_cast<const __attribute__((address_space(3))) bf16x8*>(
-                    bq + (2 * p + 1) * 8192 + cb * 256);
-            }
-#pragma unroll
-            for (int p1 = 0; p1 < NP; ++p1)
-#pragma unroll
-                for (int p2 = 0; p2 < NP; ++p2)
-                {
-                    if (p1 + p2 > 1) continue; // the 3 products of the 2-piece split
-                    // C += a conj(b): re = ar br + ai bi, im = ai br - ar bi
-                    accr[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(are[p1], br[p2], accr[cb], 0, 0, 0);
-                    acci[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aim[p1], br[p2], acci[cb], 0, 0, 0);
-                    accr[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aim[p1], bi[p2], accr[cb], 0, 0, 0);
-                    acci[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nre[p1], bi[p2], acci[cb], 0, 0, 0);
-                }
-        }
-        if (++nfl == PC_FLUSH)
-        {
-            nfl = 0;
-#pragma unroll
-            for (int cb = 0; cb < 8; ++cb)
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                {
-                    dr[cb][e] += (double)accr[cb][e];
-                    di[cb][e] += (double)acci[cb][e];
-                    accr[cb][e] = 0.f;
-                    acci[cb][e] = 0.f;
-                }
-        }
-        lds_barrier();
-        u ^= 1;
-        u3 = (u3 == 2) ? 0 : u3 + 1;
-    }
-    const int row = n0 + 16 * w4;
-#pragma unroll
-    for (int cb = 0; cb < 8; ++cb)
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-        {
-            const int rr = row + 4 * (lane >> 4) + e;
-            const int cc = m0 + cb * 16 + (lane & 15);
-            if (rr < a.nb && cc < a.nb)
-            {
-                const int64_t o = ((((int64_t)split * a.nm + v) * a.nspin + is) * a.nb + rr) * a.nb + cc;
-                part[o] = {dr[cb][e] + (double)accr[cb][e], di[cb][e] + (double)acci[cb][e]};
-            }
-        }
+    pc_consume16<NP>(L, n, w4, lane, a, part, split, v, is, n0, m0);
 }
