@@ -391,6 +391,53 @@ int lelph_read_dyn_qe(lelph_ctx* ctx, const char* dyn_file, double* qpt, double*
  * DVSCF_IS_IN_CARTESIAN_BASIS placeholder). */
 int lelph_read_patterns_qe(const char* pat_file, int natom, double* pat_vecs);
 
+/* data-file-schema.xml contents (parse_qexml, src/io/qe/read_qexml.c:20-643) */
+typedef struct lelph_qexml_info
+{
+    int64_t natom, ntype;
+    int64_t nsym;        /* crystal symmetries listed (before time-reversal doubling) */
+    int64_t fft_dims[3]; /* basis_set/fft_grid nr1 nr2 nr3 */
+    double lat_vec[9];   /* a[:, i] = a_i, bohr */
+    double alat[3];      /* atomic_structure alat, three times */
+    char dimension;      /* '2' when assume_isolated starts with "2D", else '3' */
+    int is_soc_present;  /* magnetization/spinorbit */
+    int nmag;            /* 1, 2 (lsda) or 4 (noncolin + do_magnetization) */
+    int nspinor;         /* 2 when noncolin */
+    int ph_mag_symm;     /* symmetries carry time_reversal="true" (magnetic group) */
+    int ph_tim_rev;      /* time reversal usable for phonons */
+    char pseudo_dir[1024];
+} lelph_qexml_info;
+
+/* parse_qexml (src/io/qe/read_qexml.c:20-643), host only.  Fills info;
+ * with non-NULL arrays also atom_type (natom, 0-based species index),
+ * atomic_pos (natom, 3; cartesian bohr as written), pseudo_files (ntype x
+ * 256 chars), sym_mats (nsym, 3, 3; crystal rotations as written),
+ * sym_tau (nsym, 3; crystal fractional translations, QE's x' = Sx - v
+ * sign) and trevs (nsym).  For a magnetic group the operations without
+ * time reversal come first (read_qexml.c:556-613).  Call once with NULL
+ * arrays to learn natom, ntype and nsym. */
+int lelph_parse_qexml(const char* xml_file, lelph_qexml_info* info, int* atom_type,
+                      double* atomic_pos, char* pseudo_files, double* sym_mats, double* sym_tau,
+                      int* trevs);
+
+/* parse_upf (src/io/qe/parse_upf.c:28-336), host only: UPF-1 or UPF-2,
+ * norm-conserving (LELPH_ERR_UNSUPPORTED otherwise).  ngrid and zval
+ * always; r, rab, vloc (ngrid each, Ry) when not NULL. */
+int lelph_parse_upf(const char* upf_file, int64_t* ngrid, double* zval, double* r, double* rab,
+                    double* vloc);
+
+/* read_ph_tensors_qe + read_quadrupole_fmt (src/io/qe/read_tensors.c),
+ * host only: epsilon (3, 3), zborn (natom, 3, 3), qpole (natom, 3, 3, 3);
+ * present bit 0 / 1 / 2 says which were found.  Absent files are not an
+ * error; either path may be NULL. */
+int lelph_read_ph_tensors_qe(const char* tensors_xml, const char* quadrupole_fmt, int natom,
+                             double* epsilon, double* zborn, double* qpole, int* present);
+
+/* read_qpts_qe (src/io/qe/read_dyn.c:551-609), host only: q grid product
+ * nq_bz, iBZ count nq_ibz and, when qpts is not NULL, the iBZ q points
+ * (nq_ibz, 3) of dyn0 (cartesian, 2 pi / alat). */
+int lelph_read_qpts_qe(const char* dyn0_file, int64_t* nq_ibz, int64_t* nq_bz, double* qpts);
+
 /* write_dyn_qe (src/io/qe/write_dyn_dvscf_qe.c:18-97), host only: one q,
  * dyn (nmodes, nmodes) mass normalised, written times sqrt(Ma Mb). */
 int lelph_write_dyn_qe(const char* dyn_file, int natom, const double qpt[3], const double* dyn,

@@ -11,6 +11,8 @@
  * (netCDF4-python, yambopy) sees the reference's variable names, dimension
  * names, shapes, types and chunking.
  *
+ * It also reads netCDF-4 files (the Yambo SAVE databases) the same way.
+ *
  * Plain C; every call returns LELPH_OK (0) or a lelph_status code.
  */
 #ifndef LELPH_NC4_H
@@ -50,8 +52,24 @@ int lelph_nc4_def_var(lelph_nc4* nc, const char* name, int xtype, int rank,
 int lelph_nc4_put_vara(lelph_nc4* nc, const char* name, const int64_t* start,
                        const int64_t* count, const void* data);
 
-/* nc_close */
+/* nc_close (of a file created or opened) */
 int lelph_nc4_close(lelph_nc4* nc);
+
+/* nc_open(path, NC_NOWRITE) of a netCDF-4 file, as the Yambo SAVE reader
+ * opens ns.db1, ndb.kindx, ns.kb_pp_pwscf* and ns.wf_fragments_*
+ * (src/io/get_save_data.c:108-118,164-170,454-460,566-573,701-707). */
+int lelph_nc4_open(const char* path, lelph_nc4** out);
+
+/* nc_inq_varid + nc_inq_vartype + dimension lengths: xtype (lelph_nc_type,
+ * -1 for other types), rank and dims (up to 32) of variable name. */
+int lelph_nc4_inq_var(lelph_nc4* nc, const char* name, int* xtype, int* rank, int64_t* dims);
+
+/* nc_get_vara / nc_get_var (quick_read, quick_read_sub,
+ * src/io/get_save_data.c:729-770): the hyperslab start/count (NULL: the
+ * whole variable) converted to mem_type (LELPH_NC_DOUBLE, LELPH_NC_FLOAT
+ * or LELPH_NC_INT), C order. */
+int lelph_nc4_get_vara(lelph_nc4* nc, const char* name, const int64_t* start,
+                       const int64_t* count, int mem_type, void* out);
 
 #ifdef __cplusplus
 }

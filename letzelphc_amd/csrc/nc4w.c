@@ -276,3 +276,103 @@ int lelph_nc4_close(lelph_nc4* nc)
     free(nc);
     return st;
 }
+
+/* ------------------------------------------------------------------------
+ * Reading (nc_open / nc_inq_var / nc_get_vara of a netCDF-4 file): a
+ * variable is the HDF5 dataset of its name; values convert to the asked
+ * memory type as netCDF converts them.
+ * ---------------------------------------------------------------------- */
+
+int lelph_nc4_open(const char* path, lelph_nc4** out)
+{
+    if (!path || !out) return LELPH_ERR_ARG;
+    *out = NULL;
+    H5Eset_auto2(H5E_DEFAULT, NULL, NULL);
+    hid_t f = H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return LELPH_ERR_ARG;
+    lelph_nc4* nc = calloc(1, sizeof(*nc));
+    if (!nc)
+    {
+        H5Fclose(f);
+        return LELPH_ERR_ALLOC;
+    }
+    nc->file = f;
+    *out = nc;
+    return LELPH_OK;
+}
+
+int lelph_nc4_inq_var(lelph_nc4* nc, const char* name, int* xtype, int* rank, int64_t* dims)
+{
+    if (!nc || !name) return LELPH_ERR_ARG;
+    if (H5Lexists(nc->file, name, H5P_DEFAULT) <= 0) return LELPH_ERR_ARG;
+    hid_t ds = H5Dopen2(nc->file, name, H5P_DEFAULT);
+    if (ds < 0) return LELPH_ERR_ARG;
+    hid_t t = H5Dget_type(ds);
+    int xt = -1;
+    const size_t sz = H5Tget_size(t);
+    switch (H5Tget_class(t))
+    {
+        case H5T_FLOAT: xt = sz == 8 ? LELPH_NC_DOUBLE : LELPH_NC_FLOAT; break;
+        case H5T_INTEGER: xt = sz == 4 ? LELPH_NC_INT : -1; break;
+        case H5T_STRING: xt = LELPH_NC_CHAR; break;
+        default: break;
+    }
+    H5Tclose(t);
+    hid_t sp = H5Dget_space(ds);
+    const int r = H5Sget_simple_extent_ndims(sp);
+    hsize_t d[32];
+    int st = LELPH_OK;
+    if (r < 0 || r > 32)
+        st = LELPH_ERR_ARG;
+    else
+    {
+        H5Sget_simple_extent_dims(sp, d, NULL);
+        if (rank) *rank = r;
+        if (dims)
+            for (int i = 0; i < r; ++i) dims[i] = (int64_t)d[i];
+        if (xtype) *xtype = xt;
+    }
+    H5Sclose(sp);
+    H5Dclose(ds);
+    return st;
+}
+
+int lelph_nc4_get_vara(lelph_nc4* nc, const char* name, const int64_t* start,
+                       const int64_t* count, int mtype, void* out)
+{
+    if (!nc || !name || !out || (!start) != (!count)) return LELPH_ERR_ARG;
+    if (mtype != LELPH_NC_DOUBLE && mtype != LELPH_NC_INT && mtype != LELPH_NC_FLOAT)
+        return LELPH_ERR_ARG;
+    if (H5Lexists(nc->file, name, H5P_DEFAULT) <= 0) return LELPH_ERR_ARG;
+    hid_t ds = H5Dopen2(nc->file, name, H5P_DEFAULT);
+    if (ds < 0) return LELPH_ERR_ARG;
+    hid_t fs = H5Dget_space(ds);
+    const int r = H5Sget_simple_extent_ndims(fs);
+    int st = LELPH_ERR_ARG;
+    hid_t ms = -1;
+    if (r < 0 || r > 32) goto done;
+    hsize_t d[32], st_[32], ct[32];
+    H5Sget_simple_extent_dims(fs, d, NULL);
+    for (int i = 0; i < r; ++i)
+    {
+        st_[i] = start ? (hsize_t)start[i] : 0;
+        ct[i] = count ? (hsize_t)count[i] : d[i];
+        if (start && (start[i] < 0 || count[i] < 1 || st_[i] + ct[i] > d[i])) goto done;
+    }
+    if (r > 0)
+    {
+        H5Sselect_hyperslab(fs, H5S_SELECT_SET, st_, NULL, ct, NULL);
+        ms = H5Screate_simple(r, ct, NULL);
+    }
+    else
+        ms = H5Screate(H5S_SCALAR);
+    hid_t mt = mem_type(mtype);
+    const herr_t e = H5Dread(ds, mt, ms, fs, H5P_DEFAULT, out);
+    H5Tclose(mt);
+    st = e < 0 ? LELPH_ERR_ARG : LELPH_OK;
+done:
+    if (ms >= 0) H5Sclose(ms);
+    H5Sclose(fs);
+    H5Dclose(ds);
+    return st;
+}

@@ -20,7 +20,7 @@ import os
 import numpy as np
 
 from .abi import (KERNELS, PRECISIONS, Lattice, LongRange, LrParams, Pair, Pseudo, Symmetry,
-                  c_double_p, dptr, i64ptr, pack_system)
+                  c_double_p, c_int_p, dptr, i64ptr, pack_system)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblelph_hip.so")
@@ -38,6 +38,7 @@ EXPORTS = [
     "lelph_write_dvscf_qe", "lelph_interp_timings", "lelph_dyn_interp_begin",
     "lelph_dyn_interp_add_coarse", "lelph_dyn_interp_finish", "lelph_dyn_interp",
     "lelph_read_dyn_qe", "lelph_read_patterns_qe", "lelph_write_dyn_qe",
+    "lelph_parse_qexml", "lelph_parse_upf", "lelph_read_ph_tensors_qe", "lelph_read_qpts_qe",
 ]
 
 
@@ -118,6 +119,14 @@ def lib():
         L.lelph_read_dyn_qe.argtypes = [C.c_void_p, C.c_char_p, c_double_p, c_double_p,
                                         c_double_p, c_double_p]
         L.lelph_read_patterns_qe.argtypes = [C.c_char_p, C.c_int, c_double_p]
+        L.lelph_parse_qexml.argtypes = [C.c_char_p, C.POINTER(QexmlInfo), c_int_p, c_double_p,
+                                        C.c_char_p, c_double_p, c_double_p, c_int_p]
+        i64p = C.POINTER(C.c_int64)
+        L.lelph_parse_upf.argtypes = [C.c_char_p, i64p, c_double_p, c_double_p, c_double_p,
+                                      c_double_p]
+        L.lelph_read_ph_tensors_qe.argtypes = [C.c_char_p, C.c_char_p, C.c_int, c_double_p,
+                                               c_double_p, c_double_p, c_int_p]
+        L.lelph_read_qpts_qe.argtypes = [C.c_char_p, i64p, i64p, c_double_p]
         L.lelph_write_dyn_qe.argtypes = [C.c_char_p, C.c_int, c_double_p, c_double_p,
                                          c_double_p]
         L.lelph_dyn_interp.argtypes = [C.c_void_p, C.c_int64, c_double_p, c_double_p, c_double_p,
@@ -510,3 +519,81 @@ def write_dyn_qe(path, qpt, dyn, masses):
                                     dptr(np.ascontiguousarray(qpt, np.float64)),
                                     dptr(np.ascontiguousarray(dyn, np.complex128)), dptr(m)),
            "lelph_write_dyn_qe")
+
+
+class QexmlInfo(C.Structure):
+    """lelph_qexml_info (include/lelph_hip.h)"""
+    _fields_ = [("natom", C.c_int64), ("ntype", C.c_int64), ("nsym", C.c_int64),
+                ("fft_dims", C.c_int64 * 3), ("lat_vec", C.c_double * 9),
+                ("alat", C.c_double * 3), ("dimension", C.c_char),
+                ("is_soc_present", C.c_int), ("nmag", C.c_int), ("nspinor", C.c_int),
+                ("ph_mag_symm", C.c_int), ("ph_tim_rev", C.c_int),
+                ("pseudo_dir", C.c_char * 1024)]
+
+
+def _path(p):
+    return None if p is None else os.fsencode(p)
+
+
+def parse_qexml(path) -> dict:
+    """parse_qexml (src/io/qe/read_qexml.c:20-643), host code: the fields of
+    data-file-schema.xml the reference reads, as a dict."""
+    info = QexmlInfo()
+    _check(lib().lelph_parse_qexml(_path(path), C.byref(info), None, None, None, None, None,
+                                   None), "lelph_parse_qexml")
+    na, nt, ns = info.natom, info.ntype, info.nsym
+    atype = np.zeros(na, np.int32)
+    pos = np.zeros((na, 3))
+    files = C.create_string_buffer(256 * nt)
+    R = np.zeros((ns, 3, 3))
+    tau = np.zeros((ns, 3))
+    trev = np.zeros(ns, np.int32)
+    _check(lib().lelph_parse_qexml(_path(path), C.byref(info), atype.ctypes.data_as(c_int_p),
+                                   dptr(pos), files, dptr(R), dptr(tau),
+                                   trev.ctypes.data_as(c_int_p)), "lelph_parse_qexml")
+    raw = files.raw
+    names = [raw[256 * t:256 * (t + 1)].split(b"\0", 1)[0].decode().strip() for t in range(nt)]
+    return {"natom": na, "ntype": nt, "nsym": ns, "fft_dims": tuple(info.fft_dims),
+            "lat_vec": np.array(info.lat_vec).reshape(3, 3), "alat": np.array(info.alat),
+            "dimension": info.dimension.decode(), "is_soc_present": bool(info.is_soc_present),
+            "nmag": info.nmag, "nspinor": info.nspinor, "ph_mag_symm": bool(info.ph_mag_symm),
+            "ph_tim_rev": bool(info.ph_tim_rev), "pseudo_dir": info.pseudo_dir.decode(),
+            "atom_type": atype, "atomic_pos": pos, "pseudo_files": names,
+            "sym_mats": R, "sym_tau": tau, "trevs": trev.astype(bool)}
+
+
+def parse_upf(path):
+    """parse_upf (src/io/qe/parse_upf.c:28-336), host code: (Vloc, r, rab,
+    Zval) of a norm-conserving UPF-1 or UPF-2 file."""
+    n = C.c_int64()
+    z = C.c_double()
+    _check(lib().lelph_parse_upf(_path(path), C.byref(n), C.byref(z), None, None, None),
+           "lelph_parse_upf")
+    r, rab, v = np.zeros(n.value), np.zeros(n.value), np.zeros(n.value)
+    _check(lib().lelph_parse_upf(_path(path), C.byref(n), C.byref(z), dptr(r), dptr(rab),
+                                 dptr(v)), "lelph_parse_upf")
+    return v, r, rab, z.value
+
+
+def read_ph_tensors_qe(tensors_xml, quadrupole_fmt, natom):
+    """read_ph_tensors_qe + read_quadrupole_fmt (src/io/qe/read_tensors.c),
+    host code: (epsilon, Zborn, Qpole), None where absent."""
+    eps, zb, qp = np.zeros((3, 3)), np.zeros((natom, 3, 3)), np.zeros((natom, 3, 3, 3))
+    pres = C.c_int()
+    _check(lib().lelph_read_ph_tensors_qe(_path(tensors_xml), _path(quadrupole_fmt), int(natom),
+                                          dptr(eps), dptr(zb), dptr(qp), C.byref(pres)),
+           "lelph_read_ph_tensors_qe")
+    p = pres.value
+    return (eps if p & 1 else None, zb if p & 2 else None, qp if p & 4 else None)
+
+
+def read_qpts_qe(dyn0):
+    """read_qpts_qe (src/io/qe/read_dyn.c:551-609), host code: (iBZ q points
+    (nq, 3) cartesian in 2 pi / alat, number of q in the full grid)."""
+    ni, nb = C.c_int64(), C.c_int64()
+    _check(lib().lelph_read_qpts_qe(_path(dyn0), C.byref(ni), C.byref(nb), None),
+           "lelph_read_qpts_qe")
+    q = np.zeros((ni.value, 3))
+    _check(lib().lelph_read_qpts_qe(_path(dyn0), C.byref(ni), C.byref(nb), dptr(q)),
+           "lelph_read_qpts_qe")
+    return q, nb.value

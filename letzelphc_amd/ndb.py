@@ -47,6 +47,11 @@ def lib():
                                         C.POINTER(C.c_char_p), i64p]
         L.lelph_nc4_put_vara.argtypes = [C.c_void_p, C.c_char_p, i64p, i64p, C.c_void_p]
         L.lelph_nc4_close.argtypes = [C.c_void_p]
+        L.lelph_nc4_open.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.lelph_nc4_inq_var.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int), i64p]
+        L.lelph_nc4_get_vara.argtypes = [C.c_void_p, C.c_char_p, i64p, i64p, C.c_int,
+                                         C.c_void_p]
         _lib = L
     return _lib
 
@@ -62,13 +67,44 @@ def _i64(a):
 
 
 class NC4File:
-    """nc_create / def_ncVar / nc_put_vara / nc_close."""
+    """nc_create / def_ncVar / nc_put_vara / nc_close; with mode "r"
+    nc_open / nc_inq_var / nc_get_vara of an existing netCDF-4 file."""
 
-    def __init__(self, path):
+    def __init__(self, path, mode="w"):
         h = C.c_void_p()
-        _check(lib().lelph_nc4_create(os.fsencode(path), C.byref(h)), "lelph_nc4_create")
+        if mode == "r":
+            _check(lib().lelph_nc4_open(os.fsencode(path), C.byref(h)),
+                   f"lelph_nc4_open {path}")
+        else:
+            _check(lib().lelph_nc4_create(os.fsencode(path), C.byref(h)), "lelph_nc4_create")
         self.h = h
         self.types = {}
+
+    def inq(self, name):
+        """(xtype, shape) of variable name"""
+        xt, rk = C.c_int(), C.c_int()
+        d = np.zeros(32, np.int64)
+        _check(lib().lelph_nc4_inq_var(self.h, name.encode(), C.byref(xt), C.byref(rk),
+                                       d.ctypes.data_as(C.POINTER(C.c_int64))), f"inq {name}")
+        return xt.value, tuple(int(x) for x in d[:rk.value])
+
+    def get(self, name, start=None, count=None, dtype=np.float64):
+        """nc_get_var (start None) or nc_get_vara, converted to dtype
+        (float64, float32 or int32)"""
+        mt = {np.dtype(np.float64): NC_DOUBLE, np.dtype(np.float32): NC_FLOAT,
+              np.dtype(np.int32): NC_INT}[np.dtype(dtype)]
+        _, shape = self.inq(name)
+        if start is None:
+            out = np.empty(shape, dtype)
+            _check(lib().lelph_nc4_get_vara(self.h, name.encode(), None, None, mt,
+                                            out.ctypes.data_as(C.c_void_p)), f"get {name}")
+            return out
+        s, sp = _i64(start)
+        c, cp = _i64(count)
+        out = np.empty(tuple(int(x) for x in c), dtype)
+        _check(lib().lelph_nc4_get_vara(self.h, name.encode(), sp, cp, mt,
+                                        out.ctypes.data_as(C.c_void_p)), f"get {name}")
+        return out
 
     def def_var(self, name, xtype, dims, dim_names, chunks=None):
         d, dp = _i64(dims)

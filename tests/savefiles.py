@@ -1,0 +1,292 @@
+"""Writers of Yambo SAVE and QE ph_save directories from a synthetic
+System, in the layouts the reference's readers take
+(src/io/get_save_data.c, src/io/qe/read_qexml.c, parse_upf.c,
+read_tensors.c, read_dyn.c:551-609).  Test infrastructure: the readers
+under test (letzelphc_amd/save.py and the host C parsers) must give back
+the System these files were written from."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from letzelphc_amd import ndb, synth
+
+
+def _fmt(a, per_line=4):
+    a = np.asarray(a, np.float64).reshape(-1)
+    lines = []
+    for i in range(0, len(a), per_line):
+        lines.append(" ".join(f"{v:.17e}" for v in a[i:i + per_line]))
+    return "\n".join(lines)
+
+
+def write_upf2(path, vloc, r, rab, zval, pseudo_type="NC"):
+    n = len(r)
+    with open(path, "w") as f:
+        f.write('<UPF version="2.0.1">\n')
+        f.write("  <PP_INFO>\n    synthetic pseudopotential\n  </PP_INFO>\n")
+        f.write(f'  <PP_HEADER generated="test" element="X" pseudo_type="{pseudo_type}" '
+                f'relativistic="scalar" is_ultrasoft="false" z_valence="{zval:.17e}" '
+                f'mesh_size="{n}" number_of_proj="0"/>\n')
+        f.write("  <PP_MESH>\n")
+        f.write(f'    <PP_R type="real" size="{n}" columns="4">\n{_fmt(r)}\n    </PP_R>\n')
+        f.write(f'    <PP_RAB type="real" size="{n}" columns="4">\n{_fmt(rab)}\n    </PP_RAB>\n')
+        f.write("  </PP_MESH>\n")
+        f.write(f'  <PP_LOCAL type="real" size="{n}" columns="4">\n{_fmt(vloc)}\n  </PP_LOCAL>\n')
+        f.write("</UPF>\n")
+
+
+def write_upf1(path, vloc, r, rab, zval):
+    n = len(r)
+    with open(path, "w") as f:
+        f.write("<PP_INFO>\n  synthetic pseudopotential\n</PP_INFO>\n")
+        f.write("<PP_HEADER>\n")
+        f.write("   0                   Version Number\n")
+        f.write("  X                    Element\n")
+        f.write("   NC                  Norm - Conserving pseudopotential\n")
+        f.write("    F                  Nonlinear Core Correction\n")
+        f.write(" SLA  PZ   NOGX NOGC   PZ  Exchange-Correlation functional\n")
+        f.write(f"   {zval:.11f}        Z valence\n")
+        f.write("   0.00000000000      Total energy\n")
+        f.write("   0.0000000    0.0000000 Suggested cutoff for wfc and rho\n")
+        f.write("    0                  Max angular momentum component\n")
+        f.write(f"  {n}                  Number of points in mesh\n")
+        f.write("    0    0             Number of Wavefunctions, Number of Projectors\n")
+        f.write("</PP_HEADER>\n")
+        f.write(f"<PP_MESH>\n  <PP_R>\n{_fmt(r)}\n  </PP_R>\n  <PP_RAB>\n{_fmt(rab)}\n"
+                "  </PP_RAB>\n</PP_MESH>\n")
+        f.write(f"<PP_LOCAL>\n{_fmt(vloc)}\n</PP_LOCAL>\n")
+
+
+def _crystal_syms(s):
+    """crystal rotations / translations of the operations without time
+    reversal, in QE's form: S_crys = A^-1 R A, tau_QE = -A^-1 v"""
+    A = s.alat
+    Ai = np.linalg.inv(A)
+    out = []
+    for (R, v, tr) in s.syms:
+        if tr:
+            continue
+        S = np.rint(Ai @ np.asarray(R) @ A)
+        out.append((S, -Ai @ np.asarray(v)))
+    return out
+
+
+def write_qexml(path, s, pseudo_files, names, noinv=False, isolated=None, mag=None,
+                extra_syms=None):
+    """data-file-schema.xml with what parse_qexml reads; mag: None, "lsda",
+    "noncolin" or "noncolin_mag"; extra_syms: [(S_crys, tau_QE, time_rev)]
+    written instead of the System's own"""
+    syms = extra_syms
+    if syms is None:
+        syms = [(S, t, None) for (S, t) in _crystal_syms(s)]
+    t = lambda b: "true" if b else "false"
+    lsda = mag == "lsda"
+    nonc = mag in ("noncolin", "noncolin_mag")
+    dmag = mag == "noncolin_mag"
+    A = s.alat
+    L = ['<?xml version="1.0" encoding="UTF-8"?>',
+         '<qes:espresso xmlns:qes="http://www.quantum-espresso.org/ns/qes/qes-1.0">',
+         "  <!-- <output> inside a comment must not be taken -->",
+         "  <input>",
+         "    <control_variables>",
+         "      <prefix>test</prefix>",
+         "      <pseudo_dir>./pseudo/</pseudo_dir>",
+         "    </control_variables>",
+         "    <symmetry_flags>",
+         f"      <nosym>false</nosym><noinv>{t(noinv)}</noinv>",
+         "    </symmetry_flags>",
+         "  </input>",
+         "  <output>"]
+    if isolated:
+        L += ["    <boundary_conditions>",
+              f"      <assume_isolated>{isolated}</assume_isolated>",
+              "    </boundary_conditions>"]
+    L.append(f'    <atomic_species ntyp="{len(names)}">')
+    for n, pf in zip(names, pseudo_files):
+        L += [f'      <species name="{n}">', "        <mass>1.0</mass>",
+              f"        <pseudo_file>{pf}</pseudo_file>", "      </species>"]
+    L.append("    </atomic_species>")
+    L.append(f'    <atomic_structure nat="{s.natom}" alat="{A[0, 0]:.17e}">')
+    L.append("      <atomic_positions>")
+    for a in range(s.natom):
+        p = np.asarray(s.atomic_pos[a])
+        L.append(f'        <atom name="{names[s.atom_type[a]]}" index="{a + 1}">'
+                 f"{p[0]:.17e} {p[1]:.17e} {p[2]:.17e}</atom>")
+    L.append("      </atomic_positions>")
+    L.append("      <cell>")
+    for i in range(3):
+        L.append(f"        <a{i + 1}>{A[0, i]:.17e} {A[1, i]:.17e} {A[2, i]:.17e}</a{i + 1}>")
+    L.append("      </cell>")
+    L.append("    </atomic_structure>")
+    L.append("    <symmetries>")
+    L.append(f"      <nsym>{len(syms)}</nsym>")
+    L.append(f"      <nrot>{len(syms) + 1}</nrot>")
+    for (S, tau, trv) in syms:
+        info = '<info name="crystal_symmetry"' + (
+            f' time_reversal="{t(trv)}"' if trv is not None else "") + ">crystal_symmetry</info>"
+        L += ["      <symmetry>", f"        {info}",
+              '        <rotation rank="2" dims="3 3">' + " ".join(f"{v:.1f}" for v in
+                                                                np.asarray(S).reshape(-1))
+              + "</rotation>",
+              "        <fractional_translation>" + " ".join(f"{v:.17e}" for v in tau)
+              + "</fractional_translation>", "      </symmetry>"]
+    # a lattice symmetry beyond nsym, which the reader must not take
+    L += ["      <symmetry>", '        <info name="lattice_symmetry">lattice_symmetry</info>',
+          '        <rotation rank="2" dims="3 3">0 1 0 1 0 0 0 0 1</rotation>', "      </symmetry>"]
+    L.append("    </symmetries>")
+    L.append("    <basis_set>")
+    L.append(f'      <fft_grid nr1="{s.fft_dims[0]}" nr2="{s.fft_dims[1]}" '
+             f'nr3="{s.fft_dims[2]}"></fft_grid>')
+    L.append("    </basis_set>")
+    L += ["    <magnetization>", f"      <lsda>{t(lsda)}</lsda>",
+          f"      <noncolin>{t(nonc)}</noncolin>",
+          f"      <spinorbit>{t(getattr(s.pseudo, 'soc', False))}</spinorbit>",
+          f"      <do_magnetization>{t(dmag)}</do_magnetization>", "    </magnetization>"]
+    L += ["  </output>", "</qes:espresso>"]
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
+def write_tensors(path, eps, zborn):
+    """tensors.xml: both stored transposed"""
+    with open(path, "w") as f:
+        f.write('<?xml version="1.0"?>\n<Root>\n  <EF_TENSORS>\n')
+        f.write("    <DONE_ELECTRIC_FIELD>true</DONE_ELECTRIC_FIELD>\n")
+        f.write(f"    <DONE_EFFECTIVE_CHARGE_EU>{'true' if zborn is not None else 'false'}"
+                "</DONE_EFFECTIVE_CHARGE_EU>\n")
+        f.write('    <DIELECTRIC_CONSTANT type="real" size="9">\n'
+                f"{_fmt(np.asarray(eps).T, 3)}\n    </DIELECTRIC_CONSTANT>\n")
+        if zborn is not None:
+            zt = np.stack([np.asarray(z).T for z in zborn])
+            f.write(f'    <EFFECTIVE_CHARGES_EU type="real" size="{zt.size}">\n'
+                    f"{_fmt(zt, 3)}\n    </EFFECTIVE_CHARGES_EU>\n")
+        f.write("  </EF_TENSORS>\n</Root>\n")
+
+
+def write_quadrupole(path, Q):
+    """quadrupole.fmt: atom, direction, Qxx Qyy Qzz Qyz Qxz Qxy of Q[a][x][y][dir]"""
+    with open(path, "w") as f:
+        f.write("   atom   dir        Qxx        Qyy        Qzz        Qyz        Qxz        Qxy\n")
+        for a in range(Q.shape[0]):
+            for d in range(3):
+                q = Q[a, :, :, d]
+                f.write(f"{a + 1:6d} {d + 1:5d} {q[0, 0]:.17e} {q[1, 1]:.17e} {q[2, 2]:.17e} "
+                        f"{q[1, 2]:.17e} {q[0, 2]:.17e} {q[0, 1]:.17e}\n")
+
+
+def write_dyn0(path, s, qgrid, qpts_crys):
+    """dyn0: grid, count, iBZ q in cartesian 2 pi / alat"""
+    B = synth.reciprocal_vecs(s.alat) / (2 * synth.RPI)
+    alat = s.alat[0, 0]
+    with open(path, "w") as f:
+        f.write(f"{qgrid[0]:4d}{qgrid[1]:4d}{qgrid[2]:4d}\n{len(qpts_crys)}\n")
+        for q in qpts_crys:
+            qc = (B @ np.asarray(q)) * alat
+            f.write(f"{qc[0]:.17e} {qc[1]:.17e} {qc[2]:.17e}\n")
+
+
+def write_save(save_dir, s, dtype=np.float64, extra_g=5):
+    """Yambo SAVE of the System: ndb.kindx, ns.db1, ns.kb_pp_pwscf and the
+    per-k fragments (get_save_data.c:108-614); G vectors stored once in a
+    union list with extra_g unused entries, per-k 1-based index lists
+    padded to the longest"""
+    os.makedirs(save_dir, exist_ok=True)
+    nc_f = ndb.NC_DOUBLE if dtype == np.float64 else ndb.NC_FLOAT
+    j = lambda n: os.path.join(save_dir, n)
+    nibz = len(s.wfcs)
+    lat_param = np.array([s.alat[0, 0]] * 3)
+    # union of integer G (crystal) over k, plus a few never used
+    allg = {}
+    for w in s.wfcs:
+        for g in map(tuple, np.asarray(w.gint)):
+            allg.setdefault(g, len(allg))
+    for i in range(extra_g):
+        allg.setdefault((99 + i, 0, 0), len(allg))
+    gl = np.array(list(allg.keys()), np.float64)
+    gcart = gl @ np.linalg.inv(s.alat)  # G cart without 2 pi = (B/2pi) G_crys
+    ng_total = len(gl)
+    ng_shell = max(w.npw for w in s.wfcs) + 2
+    gidx = np.zeros((nibz, ng_shell))
+    for ik, w in enumerate(s.wfcs):
+        gidx[ik, :w.npw] = [allg[tuple(g)] + 1 for g in np.asarray(w.gint)]
+    nsym = len(s.syms)
+    nbtot = s.nbnds
+    with ndb.NC4File(j("ndb.kindx")) as f:
+        f.def_var("HEAD_VERSION", nc_f, (3,), ("D_3",))
+        f.put("HEAD_VERSION", [5, 1, 0])
+        f.def_var("nXkbz", ndb.NC_INT, (1,), ("D_1",))
+        f.put("nXkbz", [len(s.kpt_bz_crys)])
+    with ndb.NC4File(j("ns.db1")) as f:
+        dims = np.zeros(18)
+        dims[5], dims[6], dims[7], dims[8] = nbtot, nibz, ng_total, ng_shell
+        dims[9], dims[10], dims[11], dims[12] = s.timerev, nsym, s.nspinor, s.nspin
+        f.def_var("DIMENSIONS", nc_f, (18,), ("D_18",))
+        f.put("DIMENSIONS", dims)
+        f.def_var("LATTICE_PARAMETER", nc_f, (3,), ("D_3",))
+        f.put("LATTICE_PARAMETER", lat_param)
+        f.def_var("LATTICE_VECTORS", nc_f, (3, 3), ("D_3", "D_3"))
+        f.put("LATTICE_VECTORS", s.alat)
+        f.def_var("SYMMETRY", nc_f, (nsym, 3, 3), ("D_nsym", "D_3", "D_3"))
+        f.put("SYMMETRY", np.stack([np.asarray(R).T for (R, _, _) in s.syms]))
+        f.def_var("K-POINTS", nc_f, (3, nibz), ("D_3", "D_nk"))
+        f.put("K-POINTS", (np.asarray(s.kpt_ibz_cart) * lat_param[None, :]).T)
+        ntype = s.pseudo.ntype
+        f.def_var("number_of_atom_species", nc_f, (1,), ("D_1",))
+        f.put("number_of_atom_species", [ntype])
+        nat_t = np.array([np.sum(np.asarray(s.atom_type) == t) for t in range(ntype)])
+        f.def_var("N_ATOMS", nc_f, (ntype,), ("D_ntype",))
+        f.put("N_ATOMS", nat_t)
+        mx = int(nat_t.max())
+        amap = np.zeros((ntype, mx))
+        for t in range(ntype):
+            ids = np.nonzero(np.asarray(s.atom_type) == t)[0]
+            amap[t, :len(ids)] = ids + 1
+        f.def_var("ATOM_MAP", nc_f, (ntype, mx), ("D_ntype", "D_nspec"))
+        f.put("ATOM_MAP", amap)
+        f.def_var("WFC_NG", nc_f, (nibz,), ("D_nk",))
+        f.put("WFC_NG", [w.npw for w in s.wfcs])
+        f.def_var("G-VECTORS", nc_f, (3, ng_total), ("D_3", "D_ng"))
+        f.put("G-VECTORS", (gcart * lat_param[None, :]).T)
+        f.def_var("WFC_GRID", nc_f, (nibz, ng_shell), ("D_nk", "D_ngshell"))
+        f.put("WFC_GRID", gidx)
+    ps = s.pseudo
+    nlj = ps.nltimesj
+    with ndb.NC4File(j("ns.kb_pp_pwscf")) as f:
+        f.def_var("PARS", nc_f, (4,), ("D_4",))
+        f.put("PARS", [0, 0, nlj, ps.lmax])
+        f.def_var("PP_TABLE", nc_f, (nlj, ntype, 3), ("D_nlj", "D_ntype", "D_3"))
+        f.put("PP_TABLE", ps.PP_table)
+        f.def_var("PP_KBS", nc_f, (nlj, ntype), ("D_nlj", "D_ntype"))
+        f.put("PP_KBS", ps.Fsign)
+    for ik, w in enumerate(s.wfcs):
+        with ndb.NC4File(j(f"ns.kb_pp_pwscf_fragment_{ik + 1}")) as f:
+            f.def_var(f"PP_KB_K{ik + 1}", nc_f, (nlj, ntype, w.npw), ("D_nlj", "D_ntype", "D_ng"))
+            f.put(f"PP_KB_K{ik + 1}", w.Fk)
+        for sp in range(s.nspin):
+            with ndb.NC4File(j(f"ns.wf_fragments_{sp * nibz + ik + 1}_1")) as f:
+                nm = f"WF_COMPONENTS_@_SP_POL{sp + 1}_K{ik + 1}_BAND_GRP_1"
+                ngw = w.npw + 3  # the stored G dimension may exceed this k's count
+                raw = np.zeros((nbtot, s.nspinor, ngw, 2))
+                raw[:, :, :w.npw, 0] = np.asarray(w.wfc[sp]).real
+                raw[:, :, :w.npw, 1] = np.asarray(w.wfc[sp]).imag
+                f.def_var(nm, nc_f, raw.shape, ("D_nb", "D_nspinor", "D_ngw", "D_2"))
+                f.put(nm, raw)
+
+
+def write_ph_save(ph_dir, s, qgrid, qpts_crys, upf="2", tensors=None, quadrupole=None,
+                  **xml_kw):
+    """ph_save with dyn0, data-file-schema.xml, one UPF per type and
+    optionally tensors.xml / quadrupole.fmt"""
+    os.makedirs(ph_dir, exist_ok=True)
+    names = [f"X{t}" for t in range(s.pseudo.ntype)]
+    files = [f"x{t}.upf" for t in range(s.pseudo.ntype)]
+    for t, (v, r, rab, z) in enumerate(s.pseudo.loc):
+        (write_upf2 if upf == "2" else write_upf1)(os.path.join(ph_dir, files[t]), v, r, rab, z)
+    write_qexml(os.path.join(ph_dir, "data-file-schema.xml"), s, files, names, **xml_kw)
+    write_dyn0(os.path.join(ph_dir, "dyn0"), s, qgrid, qpts_crys)
+    if tensors is not None:
+        write_tensors(os.path.join(ph_dir, "tensors.xml"), *tensors)
+    if quadrupole is not None:
+        write_quadrupole(os.path.join(ph_dir, "quadrupole.fmt"), quadrupole)

@@ -1,5 +1,6 @@
-"""Host-side checks that need no GPU: the C-ABI library loads and exports
-every entry point declared in include/lelph_hip.h, the host-only entry points
+"""Host-side checks that need no GPU: the C-ABI libraries load and export
+every entry point declared in include/lelph_hip.h and include/lelph_nc4.h,
+the host-only entry points
 (status strings, k+q indices, q-star rotation) agree with the oracle and the
 reference fixture, the seeded generators are deterministic, and the bench
 refuses more than one GPU."""
@@ -21,8 +22,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "lelph_hip.h")
 
 
-def declared_functions():
-    txt = open(HEADER).read()
+def declared_functions(header=HEADER):
+    txt = open(header).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(lelph_[a-z0-9_]+)\s*\(", txt)))
 
@@ -32,6 +33,19 @@ def test_library_exports_header():
     names = declared_functions()
     assert len(names) >= 15
     L = C.CDLL(lelph.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_nc4_library_exports_header():
+    """include/lelph_nc4.h against liblelph_nc4.so (host netCDF-4 I/O)"""
+    from letzelphc_amd import ndb
+    if not os.path.exists(ndb.LIB_PATH):
+        import pytest
+        pytest.skip("liblelph_nc4.so not built (no HDF5)")
+    names = declared_functions(os.path.join(ROOT, "include", "lelph_nc4.h"))
+    assert len(names) == 7, names
+    L = C.CDLL(ndb.LIB_PATH)
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
 
