@@ -239,4 +239,5 @@ def read_save(save_dir, ph_save_dir, start_band=-1, end_band=-1, nc4_float=np.fl
         dimension=x["dimension"], syms=syms, timerev=timerev, kpt_ibz_crys=kibz @ A,
         kpt_ibz_cart=kibz, kpt_bz_crys=kbz, kmap=kmap, ecut=0.0, pseudo=ps,
         qpts=[q for q in ph.qpts_ibz], wfcs=wfcs)
+    sys_.band_window = (start_band, end_band)  # 1-based, inclusive
     return sys_, ph

@@ -1,0 +1,78 @@
+"""The real-input driver end to end (letzelphc_amd/driver.py): SAVE +
+ph_save files (dyn, dvscf, patterns per iBZ q) -> ndb.elph and ndb.Dmats
+written from the device, checked slab by slab against the FP64 oracle:
+D matrices, iBZ rows, and every star member's rows rebuilt from oracle g
+and oracle D with the host q-star rotation."""
+import numpy as np
+import pytest
+
+from letzelphc_amd import driver, lelph, ndb, save, synth
+from oracle import oracle as O
+
+from .test_save import _write_both
+from .util import rel_err, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _cplx(a):
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def test_run_elph_end_to_end(tmp_path):
+    s = synth.tiny(syms="cubic", q=(0.0, 0.5, 0.0), nbnds=4)
+    qibz = _write_both(tmp_path, s)
+    ph_dir = tmp_path / "ph_save"
+    rng = np.random.default_rng(17)
+    nm = s.nmodes
+    sh = (nm, s.nmag) + tuple(s.fft_dims)
+    masses = np.array([28.0855, 15.999])[np.asarray(s.atom_type)]
+    dvs = []
+    for iq in range(len(qibz)):
+        h = rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm))
+        lelph.write_dyn_qe(str(ph_dir / f"dyn{iq + 1}"), qibz[iq], h + h.conj().T, masses)
+        dv = rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
+        np.ascontiguousarray(dv.transpose(0, 1, 4, 3, 2)).tofile(ph_dir / f"dvscf{iq + 1}")
+        (ph_dir / f"patterns.{iq + 1}.xml").write_text(
+            '<?xml version="1.0"?>\n<DVSCF_IS_IN_CARTESIAN_BASIS/>\n')
+        dvs.append(dv)
+    say(f"case: run_elph tiny cubic, {len(qibz)} iBZ q")
+    p_elph, p_dmat = driver.run_elph(tmp_path / "SAVE", ph_dir, tmp_path / "out")
+    r, ph = save.read_save(tmp_path / "SAVE", ph_dir)
+    nk = len(r.kpt_bz_crys)
+    with ndb.NC4File(p_dmat, "r") as f:
+        Dg = _cplx(f.get("Dmats"))
+    with ndb.NC4File(p_elph, "r") as f:
+        G = _cplx(f.get("elph_mat"))
+        E = _cplx(f.get("POLARIZATION_VECTORS"))
+        assert f.get("qpoints").shape == (len(ph.qpts_bz), 3)
+    O.set_threads(8)
+    o = O.OracleSystem(r)
+    Do = np.empty_like(Dg)
+    for isym, (R, tau, tr) in enumerate(ph.ph_syms):
+        for k in range(nk):
+            Do[isym, k] = o.electronic_reps(R, tau, tr, k)
+    assert rel_err(Dg, Do) < 1e-10
+    eye = np.eye(nm, dtype=np.complex128)
+    for iq in range(len(qibz)):
+        qpos = int(np.sum(ph.nqstar[:iq]))
+        eig = np.ascontiguousarray(E[qpos].reshape(nm, nm))
+        mode = np.empty(sh, np.complex128)
+        O.check(O.lib().orc_read_dvscf_qe(str(ph_dir / f"dvscf{iq + 1}").encode(), o.lat,
+                                          O.dptr(eig), O.dptr(eye), O.dptr(mode)))
+        r._eig[iq] = eig
+        r._dv[iq] = mode
+        oq = O.OracleSystem(r)
+        gq = oq.compute_elphq(iq, oq.dv_total(iq), non_loc=True)
+        assert rel_err(G[qpos], gq) < 1e-10, iq
+        q = ph.qpts_ibz[iq]
+        for istar in range(1, int(ph.nqstar[iq])):
+            qs = qpos + istar
+            isym = int(ph.qmap[qs, 1])
+            R, _, tr = ph.ph_syms[isym]
+            for k in range(nk):
+                kq = save.find_kidx_in_list(r.kpt_bz_crys, r.kpt_bz_crys[k] + q)
+                kc = (r.blat / (2 * synth.RPI)) @ r.kpt_bz_crys[k]
+                Sk = save.find_kidx_in_list(r.kpt_bz_crys, r.alat.T @ (R @ kc))
+                want = lelph.elph_q_rotate(Do[isym, kq], gq[k], Do[isym, k], tr)
+                assert rel_err(G[qs, Sk], want) < 1e-10, (iq, istar, k)
