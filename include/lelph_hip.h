@@ -343,7 +343,7 @@ int lelph_interp_set_dvscf(lelph_ctx* ctx, const double qpt[3], const double* ei
  * Fourier interpolation of dynamical matrices
  * (src/interpolation/interpolation_driver.c:190-530 coarse side, :855-1000
  * fine side; src/phonon/).  3-D and 2-D long range (dipole, quadrupole),
- * acoustic sum rule none (0) or simple (1, src/phonon/asr.c:294-318).
+ * acoustic sum rule of enum lelph_asr_kind (src/phonon/asr.c).
  * ---------------------------------------------------------------------- */
 
 /* Starts it on the context's lattice: coarse grid qgrid, atomic masses
@@ -373,6 +373,34 @@ int lelph_dyn_interp_finish(lelph_ctx* ctx);
  * this solver does not reproduce LAPACK's choice. */
 int lelph_dyn_interp(lelph_ctx* ctx, int64_t nq, const double* qpts, double* dyn_out,
                      double* omega_out, double* eig_out);
+
+/* acoustic sum rule kinds (enum asr_kind, src/common/dtypes.h:71-78) */
+enum lelph_asr_kind
+{
+    LELPH_ASR_NONE = 0,
+    LELPH_ASR_SIMPLE = 1,
+    LELPH_ASR_CRYSTAL = 2,
+    LELPH_ASR_ALL = 3,
+    LELPH_ASR_ALL_HUANG = 4
+};
+
+/* apply_acoustic_sum_rule_fc (src/phonon/asr.c:269-512), host code: frc
+ * (ngrid, 3 natom, 3 natom) complex R-space force constants (without mass
+ * factors) on the q grid, in place.  simple: the on-site block takes minus
+ * the row sums.  crystal / all / all_huang: the real parts projected
+ * orthogonally onto the translational (+ rotational, + Huang) conditions,
+ * -R blocks the transposes and R = 0 symmetric, imaginary parts zero.
+ * ws_vecs / ws_degen as lelph_wigner_seitz gives them for (atoms, atoms);
+ * ws_vecs and lat_vecs are only read by all / all_huang. */
+int lelph_asr_fc(int mode, const int64_t qgrid[3], int natom, double* frc,
+                 const double* atomic_pos, const double* lat_vecs, const int64_t* ws_vecs,
+                 int64_t n_ws_vecs, const int64_t* ws_degen);
+
+/* apply_acoustic_sum_rule_born_charges (src/phonon/asr.c:155-267), host
+ * code: Zborn (natom, 3, 3) in place with sum_a Z_a = 0 (the mean removed;
+ * every mode but none gives this, the orthogonal projection onto the nine
+ * conditions). */
+int lelph_asr_born(int mode, double* Zborn, int natom);
 
 /* ------------------------------------------------------------------------
  * QE phonon files (ph_save)

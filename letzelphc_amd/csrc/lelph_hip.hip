@@ -159,7 +159,10 @@ struct lelph_ctx
     std::vector<double> dy_mass, dy_eps, dy_zb, dy_qp;
     std::vector<char> dy_have;
     DBuf dy_dyn, dy_fc, dy_X1, dy_X2, dy_P1, dy_P2, dy_asr, dy_ws, dy_off, dy_qs, dy_qs2, dy_out,
-        dy_freq, dy_eig, dy_zbd, dy_qpd, dy_massd, dy_P, dy_rmass;
+        dy_freq, dy_eig, dy_zbd, dy_qpd, dy_massd, dy_P, dy_rmass, dy_lr, dy_fclr;
+    // host copies for the projected sum rules: Wigner-Seitz vectors / degeneracies
+    std::vector<int64_t> dy_ws_h, dy_degen_h;
+    bool dy_keep_lr = false; // long-range part kept per coarse q (dy_lr)
     // device time of the last lelph_interp_dvscf: weights, R -> q, long range, total, nq
     hipEvent_t ip_ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double ip_ms[5] = {0, 0, 0, 0, 0};
@@ -805,7 +808,7 @@ static void free_ctx(lelph_ctx* c)
                     &c->dy_dyn, &c->dy_fc, &c->dy_X1, &c->dy_X2, &c->dy_P1, &c->dy_P2,
                     &c->dy_asr, &c->dy_ws, &c->dy_off, &c->dy_qs, &c->dy_qs2, &c->dy_out,
                     &c->dy_freq, &c->dy_eig, &c->dy_zbd, &c->dy_qpd, &c->dy_massd, &c->dy_P,
-                    &c->dy_rmass};
+                    &c->dy_rmass, &c->dy_lr, &c->dy_fclr};
     for (DBuf* b : bufs) b->release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1850,6 +1853,8 @@ int lelph_interp_begin(lelph_ctx* c, const int64_t qgrid[3], const double* masse
     }
     std::vector<int> wsi((size_t)3 * std::max<int64_t>(nws, 1));
     for (int64_t i = 0; i < 3 * nws; ++i) wsi[i] = (int)ws[i];
+    c->dy_ws_h.assign(ws.begin(), ws.begin() + 3 * nws);
+    c->dy_degen_h = degen;
     std::vector<int64_t> off((size_t)nR * na + 1, 0);
     for (int64_t i = 0; i < nR * na; ++i) off[i + 1] = off[i] + degen[i];
     LELPH_TRY(c->ip_ws.ensure(sizeof(int) * wsi.size()));
@@ -2290,7 +2295,8 @@ static int dyn_lr_apply(lelph_ctx* c, const double* qs, int64_t nq, int sign, bo
 int lelph_dyn_interp_begin(lelph_ctx* c, const int64_t qgrid[3], const double* masses,
                            const lelph_longrange* lr, double eta_ph, double EcutRy, int asr)
 {
-    if (!c || !qgrid || eta_ph <= 0 || asr < 0 || asr > 1) return LELPH_ERR_ARG;
+    if (!c || !qgrid || eta_ph <= 0 || asr < LELPH_ASR_NONE || asr > LELPH_ASR_ALL_HUANG)
+        return LELPH_ERR_ARG;
     for (int i = 0; i < 3; ++i)
         if (qgrid[i] < 1 || qgrid[i] > 1024) return LELPH_ERR_ARG;
     const lelph_lattice& L = c->lat;
@@ -2336,6 +2342,8 @@ int lelph_dyn_interp_begin(lelph_ctx* c, const int64_t qgrid[3], const double* m
     }
     std::vector<int> wsi((size_t)3 * std::max<int64_t>(nws, 1));
     for (int64_t i = 0; i < 3 * nws; ++i) wsi[i] = (int)ws[i];
+    c->dy_ws_h.assign(ws.begin(), ws.begin() + 3 * nws);
+    c->dy_degen_h = degen;
     std::vector<int64_t> off((size_t)nR * na * na + 1, 0);
     for (int64_t i = 0; i < nR * na * na; ++i) off[i + 1] = off[i] + degen[i];
     LELPH_TRY(c->dy_ws.ensure(sizeof(int) * wsi.size()));
@@ -2344,6 +2352,17 @@ int lelph_dyn_interp_begin(lelph_ctx* c, const int64_t qgrid[3], const double* m
     HIP_TRY(hipMemcpy(c->dy_off.p, off.data(), sizeof(int64_t) * off.size(),
                       hipMemcpyHostToDevice));
     LELPH_TRY(c->dy_dyn.ensure(sizeof(z64) * c->dy_nq * nm * nm));
+    // the rotational rules act on the full force constants: the long-range
+    // part of each coarse matrix is kept to be added in R space and removed
+    // again (interpolation_driver.c:432-442,499-511,533-545)
+    const bool keep_lr = asr >= LELPH_ASR_ALL && !c->dy_eps.empty() &&
+                         (!c->dy_zb.empty() || !c->dy_qp.empty());
+    c->dy_keep_lr = keep_lr;
+    if (keep_lr)
+    {
+        LELPH_TRY(c->dy_lr.ensure(sizeof(z64) * c->dy_nq * nm * nm));
+        HIP_TRY(hipMemsetAsync(c->dy_lr.p, 0, sizeof(z64) * c->dy_nq * nm * nm, c->stream));
+    }
     // long-range ASR correction (compute_dyn_lr_asr_correction,
     // dyn_long_range.c:70-123): the q = 0 term without masses summed over b
     std::vector<z64> asrv((size_t)9 * na, z64{0, 0});
@@ -2409,6 +2428,9 @@ int lelph_dyn_interp_add_coarse(lelph_ctx* c, const double qpt[3], const double*
     HIP_TRY(hipMemcpyAsync(slot_p, d.data(), sizeof(z64) * nm * nm, hipMemcpyHostToDevice,
                            c->stream));
     LELPH_TRY(dyn_lr_apply(c, qpt, 1, -1, !c->dy_mass.empty(), true, slot_p));
+    if (c->dy_keep_lr)
+        LELPH_TRY(dyn_lr_apply(c, qpt, 1, +1, !c->dy_mass.empty(), true,
+                               c->dy_lr.as<z64>() + slot * nm * nm));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->dy_have[slot] = 1;
     return LELPH_OK;
@@ -2433,11 +2455,61 @@ int lelph_dyn_interp_finish(lelph_ctx* c)
                        dim3(256), 0, c->stream, c->dy_P.as<z64>(), (int)nq, (int)nq, 1,
                        c->dy_dyn.as<z64>(), 1, cols, c->dy_fc.as<z64>(), nullptr, 0, 0, 0);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_fc_asr, dim3((unsigned)cdiv(9 * na, 64)), dim3(64), 0, c->stream,
-                       c->dy_fc.as<z64>(), (int)nq, na, c->dy_massd.as<double>(),
-                       c->dy_mass.empty() ? 0 : 1, c->dy_asr_kind);
-    HIP_TRY(hipGetLastError());
+    if (c->dy_asr_kind <= LELPH_ASR_SIMPLE)
+    {
+        hipLaunchKernelGGL(k_fc_asr, dim3((unsigned)cdiv(9 * na, 64)), dim3(64), 0, c->stream,
+                           c->dy_fc.as<z64>(), (int)nq, na, c->dy_massd.as<double>(),
+                           c->dy_mass.empty() ? 0 : 1, c->dy_asr_kind);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->dy_ready = true;
+        return LELPH_OK;
+    }
+    // projected rules (apply_acoustic_sum_rule_fc, src/phonon/asr.c:320-512):
+    // host code on the force constants without mass factors
+    const bool lr = c->dy_keep_lr;
+    std::vector<z64> fc((size_t)(nq * cols)), fl;
+    if (lr)
+    {
+        LELPH_TRY(c->dy_fclr.ensure(sizeof(z64) * nq * cols));
+        hipLaunchKernelGGL((k_phase_gemm<MB>),
+                           dim3((unsigned)cdiv(cols, 256), (unsigned)cdiv(nq, MB)), dim3(256), 0,
+                           c->stream, c->dy_P.as<z64>(), (int)nq, (int)nq, 1, c->dy_lr.as<z64>(), 1,
+                           cols, c->dy_fclr.as<z64>(), nullptr, 0, 0, 0);
+        HIP_TRY(hipGetLastError());
+        fl.resize((size_t)(nq * cols));
+        HIP_TRY(hipMemcpyAsync(fl.data(), c->dy_fclr.p, sizeof(z64) * nq * cols,
+                               hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(fc.data(), c->dy_fc.p, sizeof(z64) * nq * cols, hipMemcpyDeviceToHost,
+                           c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (lr)
+        for (size_t i = 0; i < fc.size(); ++i) fc[i] = cadd(fc[i], fl[i]);
+    // mass_normalize_force_constants (src/phonon/mass_normalize_phonons.c:36-71)
+    auto mass_scale = [&](std::vector<z64>& f, double power) {
+        if (c->dy_mass.empty()) return;
+        for (int a = 0; a < na; ++a)
+            for (int b = 0; b < na; ++b)
+            {
+                const double m = std::pow(c->dy_mass[a], power) * std::pow(c->dy_mass[b], power);
+                for (int64_t r = 0; r < nq; ++r)
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j)
+                        {
+                            z64& v = f[(size_t)(r * cols + (int64_t)(3 * a + i) * nm + 3 * b + j)];
+                            v = cscale(v, m);
+                        }
+            }
+    };
+    mass_scale(fc, 0.5);
+    LELPH_TRY(lelph_asr_fc(c->dy_asr_kind, c->dy_grid, na, reinterpret_cast<double*>(fc.data()),
+                           c->atomic_pos.data(), c->lat.alat_vec, c->dy_ws_h.data(),
+                           (int64_t)c->dy_ws_h.size() / 3, c->dy_degen_h.data()));
+    mass_scale(fc, -0.5);
+    if (lr)
+        for (size_t i = 0; i < fc.size(); ++i) fc[i] = {fc[i].x - fl[i].x, fc[i].y - fl[i].y};
+    HIP_TRY(hipMemcpy(c->dy_fc.p, fc.data(), sizeof(z64) * nq * cols, hipMemcpyHostToDevice));
     c->dy_ready = true;
     return LELPH_OK;
 }

@@ -39,6 +39,7 @@ EXPORTS = [
     "lelph_dyn_interp_add_coarse", "lelph_dyn_interp_finish", "lelph_dyn_interp",
     "lelph_read_dyn_qe", "lelph_read_patterns_qe", "lelph_write_dyn_qe",
     "lelph_parse_qexml", "lelph_parse_upf", "lelph_read_ph_tensors_qe", "lelph_read_qpts_qe",
+    "lelph_asr_fc", "lelph_asr_born",
 ]
 
 
@@ -127,6 +128,9 @@ def lib():
         L.lelph_read_ph_tensors_qe.argtypes = [C.c_char_p, C.c_char_p, C.c_int, c_double_p,
                                                c_double_p, c_double_p, c_int_p]
         L.lelph_read_qpts_qe.argtypes = [C.c_char_p, i64p, i64p, c_double_p]
+        L.lelph_asr_fc.argtypes = [C.c_int, i64p, C.c_int, c_double_p, c_double_p, c_double_p,
+                                   i64p, C.c_int64, i64p]
+        L.lelph_asr_born.argtypes = [C.c_int, c_double_p, C.c_int]
         L.lelph_write_dyn_qe.argtypes = [C.c_char_p, C.c_int, c_double_p, c_double_p,
                                          c_double_p]
         L.lelph_dyn_interp.argtypes = [C.c_void_p, C.c_int64, c_double_p, c_double_p, c_double_p,
@@ -402,7 +406,7 @@ class Context:
                                             None if self._dy_mass is None else dptr(self._dy_mass),
                                             None if self._dy_plr is None else C.byref(self._dy_plr),
                                             float(eta_ph), float(ecut),
-                                            {"none": 0, "simple": 1}[asr]),
+                                            0 if asr == "none" else asr_kind(asr)),
                "lelph_dyn_interp_begin")
 
     def dyn_interp_add_coarse(self, qpt, omega, pol_vecs):
@@ -597,3 +601,38 @@ def read_qpts_qe(dyn0):
     _check(lib().lelph_read_qpts_qe(_path(dyn0), C.byref(ni), C.byref(nb), dptr(q)),
            "lelph_read_qpts_qe")
     return q, nb.value
+
+
+ASR_KINDS = {"no": 0, "simple": 1, "crystal": 2, "all": 3, "all_huang": 4}
+
+
+def asr_kind(kind) -> int:
+    """asr_kind_from_string (src/phonon/asr.c:73-116): a name (any case) or
+    an int; unknown names mean no sum rule."""
+    if isinstance(kind, str):
+        return ASR_KINDS.get(kind.lower(), 0)
+    return int(kind)
+
+
+def asr_fc(kind, qgrid, frc, atomic_pos, lat_vecs=None, ws_vecs=None, ws_degen=None):
+    """apply_acoustic_sum_rule_fc (src/phonon/asr.c:269-512), host code of
+    the library: a copy of frc (ngrid, 3 natom, 3 natom) with the rule
+    applied; ws_vecs / ws_degen from wigner_seitz(qgrid, lat, eps, pos, pos)."""
+    out = np.ascontiguousarray(frc, np.complex128).copy()
+    g = np.ascontiguousarray(qgrid, np.int64)
+    pos = np.ascontiguousarray(atomic_pos, np.float64)
+    lat = None if lat_vecs is None else np.ascontiguousarray(lat_vecs, np.float64)
+    wv = None if ws_vecs is None else np.ascontiguousarray(ws_vecs, np.int64)
+    wd = None if ws_degen is None else np.ascontiguousarray(ws_degen, np.int64)
+    _check(lib().lelph_asr_fc(asr_kind(kind), i64ptr(g), len(pos), dptr(out), dptr(pos),
+                              None if lat is None else dptr(lat),
+                              None if wv is None else i64ptr(wv), 0 if wv is None else len(wv),
+                              None if wd is None else i64ptr(wd)), "lelph_asr_fc")
+    return out
+
+
+def asr_born(kind, Zborn):
+    """apply_acoustic_sum_rule_born_charges (src/phonon/asr.c:155-267)"""
+    z = np.ascontiguousarray(Zborn, np.float64).copy()
+    _check(lib().lelph_asr_born(asr_kind(kind), dptr(z), len(z)), "lelph_asr_born")
+    return z

@@ -398,6 +398,40 @@ def _opt(a):
     return None if a is None else dptr(np.ascontiguousarray(a, np.float64))
 
 
+_ref_lib = None
+
+
+def ref_lib():
+    """oracle/_ref/liblelph_ref.so: the reference's own helper TUs (built
+    here by `make -C oracle ref`); raises if it is not built"""
+    global _ref_lib
+    if _ref_lib is None:
+        if not os.path.exists(REF_LIB):
+            raise RuntimeError(f"{REF_LIB} missing: run `make -C oracle ref`")
+        L = C.CDLL(REF_LIB)
+        ll = C.c_longlong
+        L.apply_acoustic_sum_rule_fc.argtypes = [C.c_int, C.POINTER(ll), ll, c_double_p,
+                                                 c_double_p, c_double_p, C.POINTER(ll), ll,
+                                                 C.POINTER(ll)]
+        L.apply_acoustic_sum_rule_fc.restype = None
+        _ref_lib = L
+    return _ref_lib
+
+
+def ref_asr_fc(mode, grid, natom, fc, pos, lat, ws, degen):
+    """apply_acoustic_sum_rule_fc (src/phonon/asr.c:269-512) of the
+    reference, in place on fc (ngrid, nm * nm) complex"""
+    L = ref_lib()
+    ll = C.POINTER(C.c_longlong)
+    g = np.ascontiguousarray(grid, np.int64)
+    p = np.ascontiguousarray(pos, np.float64)
+    a = np.ascontiguousarray(lat, np.float64)
+    w = np.ascontiguousarray(ws, np.int64)
+    d = np.ascontiguousarray(degen, np.int64)
+    L.apply_acoustic_sum_rule_fc(int(mode), g.ctypes.data_as(ll), int(natom), dptr(fc), dptr(p),
+                                 dptr(a), w.ctypes.data_as(ll), len(w), d.ctypes.data_as(ll))
+
+
 class OracleDynInterp:
     """The oracle's dynamical-matrix interpolation, in the order of
     interpolation_driver.c:190-530 (coarse) and :855-992 (fine), with
@@ -424,6 +458,7 @@ class OracleDynInterp:
         lib().orc_dyn_lr_asr(C.byref(self.packed.lattice), _opt(self.eps), _opt(self.zb),
                              _opt(self.qp), i64ptr(self.Gg), self.eta, dptr(self.asr_lr))
         self.dyns = np.zeros((self.nq, nm, nm), np.complex128)
+        self.lrs = np.zeros((self.nq, nm, nm), np.complex128)
         self.have = np.zeros(self.nq, bool)
 
     def slot(self, q):
@@ -448,6 +483,7 @@ class OracleDynInterp:
         i = self.slot(q)
         assert not self.have[i]
         self.dyns[i] = d
+        self.lrs[i] = self.lr_term(q, np.zeros_like(d), 1)
         self.have[i] = True
 
     def finish(self):
@@ -456,10 +492,26 @@ class OracleDynInterp:
         fc = np.ascontiguousarray(self.dyns.reshape(self.nq, nm * nm)).copy()
         lib().orc_fft_q2R(dptr(fc), i64ptr(self.grid), nm * nm)
         m = _opt(self.masses)
+        rot = self.asr in ("all", "all_huang") and self.eps is not None and (
+            self.zb is not None or self.qp is not None)
+        if rot:
+            # the long-range force constants join before the rule and leave
+            # after it (interpolation_driver.c:432-442,499-511,533-545)
+            fl = np.ascontiguousarray(self.lrs.reshape(self.nq, nm * nm)).copy()
+            lib().orc_fft_q2R(dptr(fl), i64ptr(self.grid), nm * nm)
+            fc += fl
         lib().orc_mass_normalize_fc(m, self.nq, self.sys.natom, 0.5, dptr(fc))
         if self.asr == "simple":
             lib().orc_asr_simple_fc(i64ptr(self.grid), self.sys.natom, dptr(fc))
+        elif self.asr in ("crystal", "all", "all_huang"):
+            # the reference's own apply_acoustic_sum_rule_fc (LSMR), compiled
+            # into oracle/_ref
+            ref_asr_fc({"crystal": 2, "all": 3, "all_huang": 4}[self.asr], self.grid,
+                       self.sys.natom, fc, self.sys.atomic_pos, self.sys.alat, self.ws,
+                       self.degen)
         lib().orc_mass_normalize_fc(m, self.nq, self.sys.natom, -0.5, dptr(fc))
+        if rot:
+            fc -= fl
         self.fc = fc.reshape(self.nq, nm, nm)
 
     def dyn(self, q):

@@ -21,6 +21,11 @@ and tests/golden/interp_vectors.npz:
   rotate_dvscf                       src/symmetries/rotate_dvscf.c:22-368
   alloc_and_Compute_f_Coeff          src/nonloc/fcoeff.c:24-143
 
+and tests/golden/asr_vectors.npz:
+
+  apply_acoustic_sum_rule_fc         src/phonon/asr.c:269-512 (with lsmr_solver.c)
+  apply_acoustic_sum_rule_born_charges src/phonon/asr.c:155-267
+
 Run:  python tests/golden/make_ref_vectors.py
 The fixture is data (seeded inputs, the reference's outputs); the tests
 compare the oracle's restatement against it.
@@ -358,6 +363,69 @@ def make_interp(L, seed=77):
 
 OUT_INTERP = os.path.join(HERE, "interp_vectors.npz")
 
+ASR_CASES = {
+    # name: (lattice a[:, i], grid, atoms (crystal))
+    "fcc2": (np.array([[-5.13, 0, -5.13], [0, 5.13, 5.13], [5.13, 5.13, 0]]), (2, 2, 2),
+             [[0, 0, 0], [0.25, 0.25, 0.25]]),
+    "hex3": (np.array([[5.9, -2.95, 0], [0, 5.9 * np.sqrt(3) / 2, 0], [0, 0, 30.0]]), (3, 3, 1),
+             [[1 / 3, 2 / 3, 0.5], [2 / 3, 1 / 3, 0.58], [2 / 3, 1 / 3, 0.42]]),
+    "tri2": (np.array([[7.0, 0.9, 0.3], [0.4, 6.1, -0.7], [-0.2, 0.5, 8.3]]), (3, 2, 2),
+             [[0.1, 0.2, 0.05], [0.55, 0.4, 0.6]]),
+}
+
+
+def load_asr(L):
+    """apply_acoustic_sum_rule_fc / _born_charges (src/phonon/asr.c), called
+    directly: their arguments are plain pointers"""
+    L.apply_acoustic_sum_rule_fc.argtypes = [C.c_int, C.POINTER(ll), ll, dp, dp, dp,
+                                             C.POINTER(ll), ll, C.POINTER(ll)]
+    L.apply_acoustic_sum_rule_fc.restype = None
+    L.apply_acoustic_sum_rule_born_charges.argtypes = [C.c_int, dp, ll]
+    L.apply_acoustic_sum_rule_born_charges.restype = None
+    return L
+
+
+def make_asr(L, seed=31):
+    """The force-constant sum rules simple (1), crystal (2), all (3) and
+    all_huang (4) on seeded complex force constants, with the reference's
+    own Wigner-Seitz vectors of the atom pairs; the Born-charge rule for
+    modes 1 and 2."""
+    rng = np.random.default_rng(seed)
+    f = {}
+    for name, (lat, grid, crys) in ASR_CASES.items():
+        lat = np.ascontiguousarray(lat)
+        pos = np.ascontiguousarray(np.asarray(crys, np.float64) @ lat.T)
+        g = np.ascontiguousarray(grid, np.int64)
+        na = len(pos)
+        nm = 3 * na
+        ng = int(np.prod(g))
+        nb = ng * na * na
+        cap = 64 * nb
+        vecs = np.zeros((cap, 3), np.int64)
+        degen = np.zeros(nb, np.int64)
+        n = L.rg_ws(_lp(g), P(lat), 1e-6, P(pos), na, P(pos), na, cap, _lp(vecs), _lp(degen))
+        assert n > 0
+        vecs = np.ascontiguousarray(vecs[:n])
+        frc = (rng.standard_normal((ng, nm, nm)) + 0.1j * rng.standard_normal((ng, nm, nm)))
+        frc = np.ascontiguousarray(frc)
+        f.update({f"asr_{name}_lat": lat, f"asr_{name}_grid": g, f"asr_{name}_pos": pos,
+                  f"asr_{name}_ws": vecs, f"asr_{name}_degen": degen, f"asr_{name}_frc": frc})
+        for mode in (1, 2, 3, 4):
+            out = frc.copy()
+            L.apply_acoustic_sum_rule_fc(mode, _lp(g), na, out.ctypes.data_as(dp), P(pos),
+                                         P(lat), _lp(vecs), n, _lp(degen))
+            f[f"asr_{name}_out{mode}"] = out
+        zb = np.ascontiguousarray(rng.standard_normal((na, 3, 3)))
+        f[f"asr_{name}_zb"] = zb
+        for mode in (1, 2):
+            z = zb.copy()
+            L.apply_acoustic_sum_rule_born_charges(mode, P(z), na)
+            f[f"asr_{name}_zb{mode}"] = z
+    return f
+
+
+OUT_ASR = os.path.join(HERE, "asr_vectors.npz")
+
 
 def main():
     if not os.path.exists(REF_LIB):
@@ -369,6 +437,9 @@ def main():
     fi = make_interp(load_interp(L))
     np.savez_compressed(OUT_INTERP, **fi)
     print(f"wrote {OUT_INTERP} ({os.path.getsize(OUT_INTERP)} bytes, {len(fi)} arrays)")
+    fa = make_asr(load_asr(L))
+    np.savez_compressed(OUT_ASR, **fa)
+    print(f"wrote {OUT_ASR} ({os.path.getsize(OUT_ASR)} bytes, {len(fa)} arrays)")
 
 
 if __name__ == "__main__":

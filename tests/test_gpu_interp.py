@@ -5,6 +5,8 @@ members, and g(k,q) computed from an interpolated potential.
 
 Bar: FP64, |d|max / |ref|max <= 1e-10.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -256,12 +258,17 @@ def test_dvscf_qe_files(tmp_path):
         ctx.close()
 
 
-@pytest.mark.parametrize("case", ["3d", "2d"])
-def test_dyn_interpolation(case):
+@pytest.mark.parametrize("case,asr", [("3d", "simple"), ("2d", "simple"), ("3d", "crystal"),
+                                      ("3d", "all"), ("3d", "all_huang"), ("2d", "all")])
+def test_dyn_interpolation(case, asr):
     """dynamical matrices interpolated on the device against the oracle
-    (long range with Born charges and quadrupoles, simple sum rule), and the
-    device eigensolver against LAPACK's eigenvalues and a reconstruction of
-    the Hermitian part from its eigenvectors"""
+    (long range with Born charges and quadrupoles; the simple sum rule, and
+    the crystal / all / all_huang projections against the reference's own
+    LSMR code in oracle/_ref), and the device eigensolver against LAPACK's
+    eigenvalues and a reconstruction of the Hermitian part from its
+    eigenvectors"""
+    if asr != "simple" and not os.path.exists(O.REF_LIB):
+        pytest.skip("oracle/_ref not built")
     rng = np.random.default_rng(17)
     if case == "3d":
         s = synth.config("c0", with_wfc=False)
@@ -275,11 +282,13 @@ def test_dyn_interpolation(case):
     nm = s.nmodes
     masses = masses_for(s)
     lr = lr_params(s, quad=True)
-    od = O.OracleDynInterp(s, grid, masses=masses, lr=lr, eta_ph=1.0, ecut=30.0, asr="simple")
+    od = O.OracleDynInterp(s, grid, masses=masses, lr=lr, eta_ph=1.0, ecut=30.0, asr=asr)
+    # the projections: exact here, LSMR to 1e-12 in the reference
+    tol = TOL64 if asr == "simple" else 1e-8
     ctx = lelph.Context(s, "fp64", upload=False)
-    say(f"case: dynamical-matrix interpolation {case}")
+    say(f"case: dynamical-matrix interpolation {case} asr {asr}")
     try:
-        ctx.dyn_interp_begin(grid, masses=masses, lr=lr, eta_ph=1.0, ecut=30.0, asr="simple")
+        ctx.dyn_interp_begin(grid, masses=masses, lr=lr, eta_ph=1.0, ecut=30.0, asr=asr)
         for q in coarse_grid(grid):
             e, _ = np.linalg.qr(rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm)))
             om = np.sort(rng.uniform(0.05, 1.0, nm))
@@ -292,11 +301,11 @@ def test_dyn_interpolation(case):
         sq = np.sqrt(np.repeat(masses, 3))
         for i, q in enumerate(fine):
             ref = od.dyn(q)
-            assert rel_err(dyn[i], ref) < TOL64, (case, i)
+            assert rel_err(dyn[i], ref) < tol, (case, asr, i)
             f_ref, _ = od.modes(ref)
             w = np.sign(om[i]) * om[i] ** 2
             w_ref = np.sign(f_ref) * f_ref ** 2
-            assert np.abs(w - w_ref).max() < 1e-10 * np.abs(w_ref).max(), (case, i)
+            assert np.abs(w - w_ref).max() < 10 * tol * np.abs(w_ref).max(), (case, i)
             V = (eig[i] * sq[None, :]).T  # columns: unit eigenvectors
             assert np.abs(V.conj().T @ V - np.eye(nm)).max() < 1e-10
             lo = np.tril(0.5 * (ref + ref.conj().T))
