@@ -402,6 +402,20 @@ int lelph_asr_fc(int mode, const int64_t qgrid[3], int natom, double* frc,
  * conditions). */
 int lelph_asr_born(int mode, double* Zborn, int natom);
 
+/* quadrupole_3d_to_2d (src/phonon/quadrupole_3Dto2D.c:18-117), host code:
+ * the 3-D dynamical quadrupoles Qpole (natom, 3, 3, 3) of a slab turned in
+ * place into the 2-D ones the 2-D long range takes (Phys. Rev. X 11,
+ * 041027, App. B), from epsilon (3, 3), the cartesian atomic positions and
+ * Zborn (natom, 3, 3, or NULL). */
+int lelph_quadrupole_3d_to_2d(int natom, const double* epsilon, const double* atomic_pos,
+                              const double* Zborn, double* Qpole);
+
+/* LO-TO direction (src/interpolation/interpolation_driver.c:734-786), host
+ * code: q_crys = A^T (1e-5 dir / |dir|), the point that stands for Gamma
+ * approached along the cartesian direction dir; LELPH_ERR_ARG when
+ * |dir| < 1e-6. */
+int lelph_loto_qpoint(const double lat_vecs[9], const double dir[3], double q_crys[3]);
+
 /* ------------------------------------------------------------------------
  * QE phonon files (ph_save)
  * ---------------------------------------------------------------------- */

@@ -39,7 +39,7 @@ EXPORTS = [
     "lelph_dyn_interp_add_coarse", "lelph_dyn_interp_finish", "lelph_dyn_interp",
     "lelph_read_dyn_qe", "lelph_read_patterns_qe", "lelph_write_dyn_qe",
     "lelph_parse_qexml", "lelph_parse_upf", "lelph_read_ph_tensors_qe", "lelph_read_qpts_qe",
-    "lelph_asr_fc", "lelph_asr_born",
+    "lelph_asr_fc", "lelph_asr_born", "lelph_quadrupole_3d_to_2d", "lelph_loto_qpoint",
 ]
 
 
@@ -131,6 +131,9 @@ def lib():
         L.lelph_asr_fc.argtypes = [C.c_int, i64p, C.c_int, c_double_p, c_double_p, c_double_p,
                                    i64p, C.c_int64, i64p]
         L.lelph_asr_born.argtypes = [C.c_int, c_double_p, C.c_int]
+        L.lelph_quadrupole_3d_to_2d.argtypes = [C.c_int, c_double_p, c_double_p, c_double_p,
+                                                c_double_p]
+        L.lelph_loto_qpoint.argtypes = [c_double_p, c_double_p, c_double_p]
         L.lelph_write_dyn_qe.argtypes = [C.c_char_p, C.c_int, c_double_p, c_double_p,
                                          c_double_p]
         L.lelph_dyn_interp.argtypes = [C.c_void_p, C.c_int64, c_double_p, c_double_p, c_double_p,
@@ -636,3 +639,25 @@ def asr_born(kind, Zborn):
     z = np.ascontiguousarray(Zborn, np.float64).copy()
     _check(lib().lelph_asr_born(asr_kind(kind), dptr(z), len(z)), "lelph_asr_born")
     return z
+
+
+def quadrupole_3d_to_2d(epsilon, atomic_pos, Zborn, Qpole):
+    """quadrupole_3d_to_2d (src/phonon/quadrupole_3Dto2D.c:18-117): the
+    2-D quadrupoles (natom, 3, 3, 3) of a slab from its 3-D ones"""
+    q = np.ascontiguousarray(Qpole, np.float64).copy()
+    z = None if Zborn is None else np.ascontiguousarray(Zborn, np.float64)
+    _check(lib().lelph_quadrupole_3d_to_2d(
+        len(q), dptr(np.ascontiguousarray(epsilon, np.float64)),
+        dptr(np.ascontiguousarray(atomic_pos, np.float64)), None if z is None else dptr(z),
+        dptr(q)), "lelph_quadrupole_3d_to_2d")
+    return q
+
+
+def loto_qpoint(lat_vecs, direction):
+    """interpolation_driver.c:734-786: the crystal q standing for Gamma
+    approached along the cartesian direction"""
+    out = np.zeros(3)
+    _check(lib().lelph_loto_qpoint(dptr(np.ascontiguousarray(lat_vecs, np.float64)),
+                                   dptr(np.ascontiguousarray(direction, np.float64)),
+                                   dptr(out)), "lelph_loto_qpoint")
+    return out

@@ -107,3 +107,21 @@ int rg_fcoeff(int l, int twoj, double* out)
     free_f_Coeff(&P);
     return 0;
 }
+
+/* quadrupole_3d_to_2d (src/phonon/quadrupole_3Dto2D.c:18) on a 2-D
+ * lattice; Q (natom, 3, 3, 3) in place */
+void quadrupole_3d_to_2d(struct Lattice* lattice, struct Phonon* phonon);
+void rg_quad2d(long long natom, double* eps, double* pos, double* zb, double* Q)
+{
+    struct Lattice L;
+    struct Phonon P;
+    memset(&L, 0, sizeof(L));
+    memset(&P, 0, sizeof(P));
+    L.dimension = '2';
+    L.natom = natom;
+    L.atomic_pos = pos;
+    P.epsilon = eps;
+    P.Zborn = zb;
+    P.Qpole = Q;
+    quadrupole_3d_to_2d(&L, &P);
+}

@@ -25,6 +25,7 @@ and tests/golden/asr_vectors.npz:
 
   apply_acoustic_sum_rule_fc         src/phonon/asr.c:269-512 (with lsmr_solver.c)
   apply_acoustic_sum_rule_born_charges src/phonon/asr.c:155-267
+  quadrupole_3d_to_2d                src/phonon/quadrupole_3Dto2D.c:18-117
 
 Run:  python tests/golden/make_ref_vectors.py
 The fixture is data (seeded inputs, the reference's outputs); the tests
@@ -382,6 +383,8 @@ def load_asr(L):
     L.apply_acoustic_sum_rule_fc.restype = None
     L.apply_acoustic_sum_rule_born_charges.argtypes = [C.c_int, dp, ll]
     L.apply_acoustic_sum_rule_born_charges.restype = None
+    L.rg_quad2d.argtypes = [ll, dp, dp, dp, dp]
+    L.rg_quad2d.restype = None
     return L
 
 
@@ -421,6 +424,13 @@ def make_asr(L, seed=31):
             z = zb.copy()
             L.apply_acoustic_sum_rule_born_charges(mode, P(z), na)
             f[f"asr_{name}_zb{mode}"] = z
+        # quadrupole_3d_to_2d (src/phonon/quadrupole_3Dto2D.c:18-117)
+        A = rng.standard_normal((3, 3))
+        eps = np.ascontiguousarray(5.0 * np.eye(3) + 0.4 * (A + A.T))
+        Q = np.ascontiguousarray(rng.standard_normal((na, 3, 3, 3)))
+        q2 = Q.copy()
+        L.rg_quad2d(na, P(eps), P(pos), P(zb), P(q2))
+        f.update({f"q2d_{name}_eps": eps, f"q2d_{name}_Q": Q, f"q2d_{name}_out": q2})
     return f
 
 

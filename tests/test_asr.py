@@ -1,5 +1,5 @@
 """Acoustic sum rules (host code of the library, lelph_asr_fc /
-lelph_asr_born) against the reference's own apply_acoustic_sum_rule_fc /
+lelph_asr_born) and the slab quadrupole conversion against the reference's own apply_acoustic_sum_rule_fc /
 _born_charges compiled from src/phonon/asr.c and src/common/lsmr_solver.c
 into oracle/_ref: the committed fixture tests/golden/asr_vectors.npz and,
 where the library is built, a second seed live.  The library computes the
@@ -99,6 +99,23 @@ def test_asr_invariants(mode):
         again = lelph.asr_fc(mode, grid, out, f[f"asr_{name}_pos"], f[f"asr_{name}_lat"],
                              f[f"asr_{name}_ws"], f[f"asr_{name}_degen"])
         assert np.abs(again - out).max() < 1e-12 * np.abs(out).max(), (name, mode)
+
+
+def test_quadrupole_3d_to_2d_vs_reference():
+    with np.load(FIX) as f:
+        for name in CASES:
+            out = lelph.quadrupole_3d_to_2d(f[f"q2d_{name}_eps"], f[f"asr_{name}_pos"],
+                                            f[f"asr_{name}_zb"], f[f"q2d_{name}_Q"])
+            assert np.abs(out - f[f"q2d_{name}_out"]).max() < 1e-14, name
+
+
+def test_loto_qpoint():
+    lat = np.array([[-5.13, 0, -5.13], [0, 5.13, 5.13], [5.13, 5.13, 0]])
+    q = lelph.loto_qpoint(lat, [0.0, 0.0, 2.0])
+    # back to cartesian: (B / 2 pi) q = 1e-5 z
+    assert np.allclose(np.linalg.inv(lat).T @ q, [0, 0, 1e-5], atol=1e-18)
+    with pytest.raises(lelph.LelphError):
+        lelph.loto_qpoint(lat, [0, 0, 1e-9])
 
 
 def test_asr_kinds_and_born():
