@@ -44,13 +44,39 @@ static int qe_line(FILE* fp, char* buf)
     return fgets(buf, QE_LINE, fp) != NULL;
 }
 
-/* Parses the first dynamical matrix of an old-format dyn file:
- * qpt (3), masses (natom) and dyn (nm x nm, row major, D[(a,x)][(b,y)]
- * divided by sqrt(Ma Mb)).  Returns LELPH_OK or LELPH_ERR_ARG. */
+static int parse_dyn_xml(const char* path, int natom, double* qpt, double* masses,
+                         double* dynd);
+
+/* first non-blank line holds "<?xml" (is_dyn_xml, src/io/qe/read_dyn.c:611-641) */
+static int is_dyn_xml(FILE* fp)
+{
+    char line[1024];
+    int xml = 0;
+    while (fgets(line, sizeof(line), fp))
+    {
+        const char* p = line;
+        while (*p && isspace((unsigned char)*p)) ++p;
+        if (!*p) continue;
+        xml = strstr(p, "<?xml") != NULL;
+        break;
+    }
+    rewind(fp);
+    return xml;
+}
+
+/* Parses the first dynamical matrix of a dyn file, the old text format or
+ * the XML one: qpt (3), masses (natom) and dyn (nm x nm, row major,
+ * D[(a,x)][(b,y)] divided by sqrt(Ma Mb)).  Returns LELPH_OK or
+ * LELPH_ERR_ARG. */
 int lelph_qe_parse_dyn(const char* path, int natom, double* qpt, double* masses, double* dynd)
 {
     FILE* fp = fopen(path, "r");
     if (!fp) return LELPH_ERR_ARG;
+    if (is_dyn_xml(fp))
+    {
+        fclose(fp);
+        return parse_dyn_xml(path, natom, qpt, masses, dynd);
+    }
     char* buf = malloc(QE_LINE);
     double f[16];
     double* tm = NULL;
@@ -888,5 +914,73 @@ int lelph_read_qpts_qe(const char* dyn0, int64_t* nq_ibz, int64_t* nq_bz, double
     st = LELPH_OK;
 done:
     fclose(fp);
+    return st;
+}
+
+/* read_dyn_xml (src/io/qe/read_dyn.c:306-548): GEOMETRY_INFO's
+ * NUMBER_OF_TYPES, NUMBER_OF_ATOMS, MASS.t (amu, times 911.444243096 to the
+ * Rydberg mass unit) and ATOM.i's INDEX (type), then DYNAMICAL_MAT_.1's
+ * Q_POINT and PHI.a.b: 18 numbers with the first cartesian index fastest,
+ * element (x, y) = phi[2x + 6y] + i phi[2x + 6y + 1]. */
+static int parse_dyn_xml(const char* path, int natom, double* qpt, double* masses,
+                         double* dynd)
+{
+    long sz = 0;
+    char* s = slurp(path, &sz);
+    if (!s) return LELPH_ERR_ARG;
+    const char* e = s + sz;
+    int st = LELPH_ERR_ARG;
+    double* tm = NULL;
+    struct xspan g, x;
+    char tag[64], buf[64];
+    const int nm = 3 * natom;
+    if (!XPATH(s, e, &g, "GEOMETRY_INFO")) goto done;
+    if (!xml_nth(g.b, g.e, "NUMBER_OF_TYPES", 0, &x)) goto done;
+    {
+        double v;
+        if (xml_floats(&x, &v, 1) != 1) goto done;
+        const int nt = (int)rint(v);
+        if (!xml_nth(g.b, g.e, "NUMBER_OF_ATOMS", 0, &x) || xml_floats(&x, &v, 1) != 1) goto done;
+        if ((int)rint(v) != natom || nt < 1) goto done;
+        tm = malloc(sizeof(double) * (size_t)nt);
+        if (!tm) goto done;
+        for (int t = 0; t < nt; ++t)
+        {
+            snprintf(tag, sizeof(tag), "MASS.%d", t + 1);
+            if (!xml_nth(g.b, g.e, tag, 0, &x) || xml_floats(&x, &v, 1) != 1) goto done;
+            tm[t] = v * 911.444243096;
+        }
+        for (int a = 0; a < natom; ++a)
+        {
+            snprintf(tag, sizeof(tag), "ATOM.%d", a + 1);
+            if (!xml_nth(g.b, g.e, tag, 0, &x) || !xml_attr(&x, "INDEX", buf, sizeof(buf)))
+                goto done;
+            const int it = atoi(buf) - 1;
+            if (it < 0 || it >= nt) goto done;
+            masses[a] = tm[it];
+        }
+    }
+    struct xspan dm;
+    if (!XPATH(s, e, &dm, "DYNAMICAL_MAT_.1")) goto done;
+    if (!xml_nth(dm.b, dm.e, "Q_POINT", 0, &x) || xml_floats(&x, qpt, 3) != 3) goto done;
+    for (int a = 0; a < natom; ++a)
+        for (int b = 0; b < natom; ++b)
+        {
+            double phi[18];
+            snprintf(tag, sizeof(tag), "PHI.%d.%d", a + 1, b + 1);
+            if (!xml_nth(dm.b, dm.e, tag, 0, &x) || xml_floats(&x, phi, 18) != 18) goto done;
+            const double inv = 1.0 / sqrt(masses[a] * masses[b]);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                {
+                    const int64_t o = 2 * ((int64_t)(3 * a + i) * nm + 3 * b + j);
+                    dynd[o] = phi[2 * i + 6 * j] * inv;
+                    dynd[o + 1] = phi[2 * i + 6 * j + 1] * inv;
+                }
+        }
+    st = LELPH_OK;
+done:
+    free(tm);
+    free(s);
     return st;
 }

@@ -1,6 +1,7 @@
 """QE phonon files: patterns.N.xml (read_pattern_qe) and dyn files
 (write_dyn_qe / read_dyn_qe, old text format), host parsing on the CPU and
 the device diagonalisation of read_dyn_qe against LAPACK on the GPU."""
+import os
 import re
 
 import numpy as np
@@ -115,3 +116,62 @@ def test_read_dyn_gpu(tmp_path):
     V = (pv * np.sqrt(np.repeat(am, 3))[None, :]).T
     assert np.abs(V.conj().T @ V - np.eye(6)).max() < 1e-12
     assert rel_err(V @ np.diag(w) @ V.conj().T, H) < 1e-12
+
+
+def write_dyn_xml(path, q, D, type_masses_amu, atom_types):
+    """QE's XML dyn layout as read_dyn_xml reads it (src/io/qe/read_dyn.c:
+    306-548): PHI.a.b with the first cartesian index fastest"""
+    na = len(atom_types)
+    L = ['<?xml version="1.0"?>', "<Root>", "  <GEOMETRY_INFO>",
+         f"    <NUMBER_OF_TYPES>{len(type_masses_amu)}</NUMBER_OF_TYPES>",
+         f"    <NUMBER_OF_ATOMS>{na}</NUMBER_OF_ATOMS>"]
+    for t, m in enumerate(type_masses_amu):
+        L.append(f"    <MASS.{t + 1}>{m:.17e}</MASS.{t + 1}>")
+    for a, t in enumerate(atom_types):
+        L.append(f'    <ATOM.{a + 1} SPECIES="X{t}" INDEX="{t + 1}" TAU="0 0 0"/>')
+    L += ["  </GEOMETRY_INFO>", "  <DYNAMICAL_MAT_.1>",
+          f"    <Q_POINT>{q[0]:.17e} {q[1]:.17e} {q[2]:.17e}</Q_POINT>"]
+    for a in range(na):
+        for b in range(na):
+            blk = D[3 * a:3 * a + 3, 3 * b:3 * b + 3]
+            vals = []
+            for y in range(3):
+                for x in range(3):
+                    vals += [blk[x, y].real, blk[x, y].imag]
+            L.append(f"    <PHI.{a + 1}.{b + 1}>" + " ".join(f"{v:.17e}" for v in vals)
+                     + f"</PHI.{a + 1}.{b + 1}>")
+    L += ["  </DYNAMICAL_MAT_.1>", "  <DYNAMICAL_MAT_.2>", "    <Q_POINT>9 9 9</Q_POINT>",
+          "  </DYNAMICAL_MAT_.2>", "</Root>"]
+    open(path, "w").write("\n".join(L) + "\n")
+
+
+def test_parse_dyn_xml(tmp_path):
+    """the XML dyn format gives the file's matrix / sqrt(Ma Mb) bit for bit,
+    masses in amu times 911.444243096, the first dynamical matrix only;
+    the text format of the same matrix agrees to its printed digits"""
+    say("case: XML dyn file parsing")
+    import ctypes as C
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6))
+    D = (A + A.conj().T) * 1e-3
+    amu = [12.0107, 15.999]
+    f = tmp_path / "dyn1.xml"
+    write_dyn_xml(f, [0.0, 0.5, 0.25], D, amu, [1, 0])
+    L = C.CDLL(lelph.LIB_PATH)
+    q = np.zeros(3)
+    m = np.zeros(2)
+    d = np.zeros((6, 6), np.complex128)
+    dp = C.POINTER(C.c_double)
+    assert L.lelph_qe_parse_dyn(os.fsencode(f), 2, q.ctypes.data_as(dp), m.ctypes.data_as(dp),
+                                d.ctypes.data_as(dp)) == 0
+    mass = np.array([amu[1], amu[0]]) * 911.444243096
+    assert np.array_equal(q, [0.0, 0.5, 0.25]) and np.array_equal(m, mass)
+    ms = np.sqrt(np.outer(np.repeat(mass, 3), np.repeat(mass, 3)))
+    assert np.allclose(d, D / ms, rtol=0, atol=1e-18)
+    g = tmp_path / "dyn1"
+    lelph.write_dyn_qe(str(g), np.array([0.0, 0.5, 0.25]), D / ms, mass)
+    d2 = np.zeros_like(d)
+    assert L.lelph_qe_parse_dyn(os.fsencode(g), 2, q.ctypes.data_as(dp), m.ctypes.data_as(dp),
+                                d2.ctypes.data_as(dp)) == 0
+    # the text form prints D sqrt(Ma Mb) with 8 decimals
+    assert np.abs(d2 - d).max() <= 1e-8 / ms.min()
