@@ -420,8 +420,8 @@ int lelph_loto_qpoint(const double lat_vecs[9], const double dir[3], double q_cr
  * QE phonon files (ph_save)
  * ---------------------------------------------------------------------- */
 
-/* read_dyn_qe (src/io/qe/read_dyn.c:38-293, old text format, first
- * matrix of the file): qpt (3) as written, omega (nmodes) sign(w) sqrt|w|
+/* read_dyn_qe (src/io/qe/read_dyn.c:38-548, the old text format or the
+ * XML one, first matrix of the file): qpt (3) as written, omega (nmodes) sign(w) sqrt|w|
  * ascending, pol_vecs (nmodes, 3 natom) eigenvector rows / sqrt(M), amass
  * (natom, or NULL) the masses of the file.  The Hermitian part is
  * diagonalised on the device (eigenvector phases are this solver's own). */
