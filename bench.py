@@ -155,9 +155,21 @@ def aux_measurements(s, ctx):
         t_dc = time.perf_counter() - t0
         t0 = time.perf_counter()
         ic.dyn_interp(fine)
-        out["dyn_interp_c4"] = {"coarse_side_s": t_dc,
-                                "fine_wall_s": time.perf_counter() - t0,
-                                "nq_fine": len(fine)}
+        t_df = time.perf_counter() - t0
+        # the same with the rotational + Huang sum rules (host projection)
+        rng = np.random.default_rng(5)
+        t0 = time.perf_counter()
+        ic.dyn_interp_begin((4, 4, 4), masses=masses, lr=lr, eta_ph=1.0, ecut=30.0,
+                            asr="all_huang")
+        for q in c0.qpts:
+            e, _ = np.linalg.qr(rng.standard_normal((c0.nmodes,) * 2)
+                                + 1j * rng.standard_normal((c0.nmodes,) * 2))
+            ic.dyn_interp_add_coarse(q, np.sort(rng.uniform(0.001, 0.01, c0.nmodes)),
+                                     e / np.sqrt(np.repeat(masses, 3))[None, :])
+        ic.dyn_interp_finish()
+        out["dyn_interp_c4"] = {"coarse_side_s": t_dc, "fine_wall_s": t_df,
+                                "nq_fine": len(fine),
+                                "coarse_side_s_asr_all_huang": time.perf_counter() - t0}
     finally:
         ic.close()
     # c3: eight uneven cells, 16 pairs each, mixed FP64 / FP32, one context
@@ -186,6 +198,30 @@ def aux_measurements(s, ctx):
     finally:
         for cx in ctxs:
             cx.close()
+    # the driver on real-format inputs: a Yambo SAVE and a QE ph_save of the
+    # c0 cell written in the reference's layouts, q grid 4x4x4 -> ndb.elph
+    # and ndb.Dmats (reading, D matrices, dV per iBZ q from its files, g for
+    # every k, star unfolding, writing)
+    import shutil
+    import tempfile
+    from letzelphc_amd import driver, savefiles
+    tmp = tempfile.mkdtemp(prefix="lelph_driver_")
+    try:
+        cs = synth.config("c0")
+        qibz = savefiles.write_inputs(tmp, cs, (4, 4, 4), np.full(cs.natom, 28.0855))
+        t0 = time.perf_counter()
+        p_elph, _ = driver.run_elph(os.path.join(tmp, "SAVE"), os.path.join(tmp, "ph_save"),
+                                    os.path.join(tmp, "out"))
+        t_drv = time.perf_counter() - t0
+        nel = 64 * len(cs.kpt_bz_crys) * cs.nmodes * cs.nspin * cs.nbnds ** 2
+        out["driver_c0"] = {
+            "workload": "c0 cell (64 k, 8 bands, 6 modes, 24^3) as Yambo SAVE + QE ph_save "
+                        f"files, q grid 4x4x4 ({len(qibz)} iBZ q, 64 in the full grid) -> "
+                        "ndb.elph + ndb.Dmats, FP64",
+            "wall_s": t_drv, "g_elements": nel, "g_per_s_end_to_end": nel / t_drv,
+            "ndb_elph_bytes": os.path.getsize(p_elph)}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
     # D-matrices and q-star rotation at the headline sizes
     t0 = time.perf_counter()
     R, tau, tr = s.syms[0]

@@ -1,16 +1,15 @@
-"""Writers of Yambo SAVE and QE ph_save directories from a synthetic
-System, in the layouts the reference's readers take
-(src/io/get_save_data.c, src/io/qe/read_qexml.c, parse_upf.c,
-read_tensors.c, read_dyn.c:551-609).  Test infrastructure: the readers
-under test (letzelphc_amd/save.py and the host C parsers) must give back
-the System these files were written from."""
+"""Writers of Yambo SAVE and QE ph_save directories from a System, in the
+layouts the reference's readers take (src/io/get_save_data.c,
+src/io/qe/read_qexml.c, parse_upf.c, read_tensors.c, read_dyn.c:551-609):
+synthetic inputs in the on-disk form of real ones, for the readers
+(letzelphc_amd/save.py) and the driver (letzelphc_amd/driver.py)."""
 from __future__ import annotations
 
 import os
 
 import numpy as np
 
-from letzelphc_amd import ndb, synth
+from . import ndb, synth
 
 
 def _fmt(a, per_line=4):
@@ -290,3 +289,42 @@ def write_ph_save(ph_dir, s, qgrid, qpts_crys, upf="2", tensors=None, quadrupole
         write_tensors(os.path.join(ph_dir, "tensors.xml"), *tensors)
     if quadrupole is not None:
         write_quadrupole(os.path.join(ph_dir, "quadrupole.fmt"), quadrupole)
+
+
+def ibz_qpoints(s, qgrid):
+    """iBZ q points (crystal) of the q grid under the System's symmetries"""
+    g = [(i / qgrid[0], j / qgrid[1], k / qgrid[2]) for i in range(qgrid[0])
+         for j in range(qgrid[1]) for k in range(qgrid[2])]
+    ibz, _ = synth._reduce_kgrid(np.array(g), [(R, v, bool(t)) for (R, v, t) in s.syms],
+                                 s.alat, s.blat)
+    return ibz
+
+
+def write_q_files(ph_dir, s, qibz, masses, seed=17):
+    """per iBZ q: dyn<i> (a random Hermitian matrix in the text format),
+    dvscf<i> (random, cartesian basis, QE's (Nz, Ny, Nx) order) and the
+    cartesian-basis patterns placeholder"""
+    from . import lelph
+    rng = np.random.default_rng(seed)
+    nm = s.nmodes
+    sh = (nm, s.nmag) + tuple(s.fft_dims)
+    for iq in range(len(qibz)):
+        h = rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm))
+        lelph.write_dyn_qe(os.path.join(ph_dir, f"dyn{iq + 1}"), qibz[iq], h + h.conj().T,
+                           masses)
+        dv = rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
+        np.ascontiguousarray(dv.transpose(0, 1, 4, 3, 2)).tofile(
+            os.path.join(ph_dir, f"dvscf{iq + 1}"))
+        with open(os.path.join(ph_dir, f"patterns.{iq + 1}.xml"), "w") as f:
+            f.write('<?xml version="1.0"?>\n<DVSCF_IS_IN_CARTESIAN_BASIS/>\n')
+
+
+def write_inputs(root, s, qgrid, masses, dtype=np.float64, **xml_kw):
+    """root/SAVE and root/ph_save of the System with every per-q file;
+    returns the iBZ q points"""
+    qibz = ibz_qpoints(s, qgrid)
+    write_save(os.path.join(root, "SAVE"), s, dtype=dtype)
+    ph = os.path.join(root, "ph_save")
+    write_ph_save(ph, s, qgrid, qibz, **xml_kw)
+    write_q_files(ph, s, qibz, masses)
+    return qibz

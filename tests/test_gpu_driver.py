@@ -7,9 +7,9 @@ import numpy as np
 import pytest
 
 from letzelphc_amd import driver, lelph, ndb, save, synth
+from letzelphc_amd import savefiles as SF
 from oracle import oracle as O
 
-from .test_save import _write_both
 from .util import rel_err, say
 
 pytestmark = pytest.mark.gpu
@@ -21,21 +21,11 @@ def _cplx(a):
 
 def test_run_elph_end_to_end(tmp_path):
     s = synth.tiny(syms="cubic", q=(0.0, 0.5, 0.0), nbnds=4)
-    qibz = _write_both(tmp_path, s)
+    masses = np.array([28.0855, 15.999])[np.asarray(s.atom_type)]
+    qibz = SF.write_inputs(tmp_path, s, (2, 2, 2), masses)
     ph_dir = tmp_path / "ph_save"
-    rng = np.random.default_rng(17)
     nm = s.nmodes
     sh = (nm, s.nmag) + tuple(s.fft_dims)
-    masses = np.array([28.0855, 15.999])[np.asarray(s.atom_type)]
-    dvs = []
-    for iq in range(len(qibz)):
-        h = rng.standard_normal((nm, nm)) + 1j * rng.standard_normal((nm, nm))
-        lelph.write_dyn_qe(str(ph_dir / f"dyn{iq + 1}"), qibz[iq], h + h.conj().T, masses)
-        dv = rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
-        np.ascontiguousarray(dv.transpose(0, 1, 4, 3, 2)).tofile(ph_dir / f"dvscf{iq + 1}")
-        (ph_dir / f"patterns.{iq + 1}.xml").write_text(
-            '<?xml version="1.0"?>\n<DVSCF_IS_IN_CARTESIAN_BASIS/>\n')
-        dvs.append(dv)
     say(f"case: run_elph tiny cubic, {len(qibz)} iBZ q")
     p_elph, p_dmat = driver.run_elph(tmp_path / "SAVE", ph_dir, tmp_path / "out")
     r, ph = save.read_save(tmp_path / "SAVE", ph_dir)

@@ -2,7 +2,7 @@
 QE ph_save (dyn0, data-file-schema.xml, UPF-1/2, tensors.xml,
 quadrupole.fmt) and the Yambo SAVE netCDF-4 databases.  Each test writes
 the files from a synthetic System in the reference's layouts
-(tests/savefiles.py) and checks that the readers give the System back,
+(letzelphc_amd/savefiles.py) and checks that the readers give the System back,
 plus the conversions the reference applies on the way (phonon symmetries
 to cartesian, time-reversal doubling, magnetic reordering, q/k star
 expansion, band window, precision check)."""
@@ -12,7 +12,7 @@ import pytest
 from letzelphc_amd import lelph, save, synth
 from oracle import oracle as O
 
-from . import savefiles as SF
+from letzelphc_amd import savefiles as SF
 
 pytestmark = pytest.mark.filterwarnings("ignore::DeprecationWarning")
 
