@@ -31,7 +31,7 @@
 #include "split.hpp"
 
 #define PC_TR 64  // rows of a block tile
-#define PC_FLUSH 8 // chunks per FP32 accumulation chain (256 points)
+#define PC_FLUSH 16 // chunks per FP32 accumulation chain (512 points)
 #define PC_TC 128 // columns of a block tile
 
 template <int NP>
