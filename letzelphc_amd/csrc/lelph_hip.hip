@@ -41,6 +41,8 @@ struct DevWfc
 };
 
 // a growable device buffer
+#define LELPH_OUT_SLOTS 3
+
 struct DBuf
 {
     void* p = nullptr;
@@ -181,6 +183,13 @@ struct lelph_ctx
     int64_t ev_pairs = 0;
     double timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+
+    // copy-out of a batch: each pair's g leaves on its own stream into a
+    // pinned ring while later pairs compute
+    hipStream_t out_stream = nullptr;
+    void* out_ring = nullptr;
+    size_t out_ring_bytes = 0;
+    hipEvent_t out_ev[LELPH_OUT_SLOTS] = {};
 };
 
 /* ------------------------------------------------------------------------ */
@@ -816,6 +825,10 @@ static void free_ctx(lelph_ctx* c)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : c->ip_ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->out_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->out_stream) (void)hipStreamDestroy(c->out_stream);
+    if (c->out_ring) (void)hipHostFree(c->out_ring);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1296,6 +1309,28 @@ static int gkq_device(lelph_ctx* c, const double* qpt, const lelph_pair& pr, int
 }
 
 // sums the per-pair phase times of the last call (after the stream drained)
+// the batch copy-out's stream, events and pinned ring of LELPH_OUT_SLOTS slices
+static int ensure_out_ring(lelph_ctx* c, size_t slice)
+{
+    if (!c->out_stream) HIP_TRY(hipStreamCreateWithFlags(&c->out_stream, hipStreamNonBlocking));
+    for (auto& e : c->out_ev)
+        if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const size_t need = slice * LELPH_OUT_SLOTS;
+    if (c->out_ring_bytes < need)
+    {
+        if (c->out_ring) (void)hipHostFree(c->out_ring);
+        c->out_ring = nullptr;
+        c->out_ring_bytes = 0;
+        if (hipHostMalloc(&c->out_ring, need) != hipSuccess)
+        {
+            c->out_ring = nullptr;
+            return LELPH_ERR_ALLOC;
+        }
+        c->out_ring_bytes = need;
+    }
+    return LELPH_OK;
+}
+
 static int finish_timings(lelph_ctx* c, int64_t npairs)
 {
     double t[5] = {0, 0, 0, 0, 0};
@@ -1406,24 +1441,49 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
                 }
             }
     }
+    // copy-out: pair n's g goes device -> pinned slot n % LELPH_OUT_SLOTS on
+    // the copy stream once its last event has fired, and the host moves it
+    // into g_out while the GPU works on later pairs
+    const size_t slice = sizeof(z64) * (size_t)gsz;
+    LELPH_TRY(ensure_out_ring(c, slice));
+    z64* ring = static_cast<z64*>(c->out_ring);
+    auto land = [&](int64_t m) -> int {
+        const int slot = (int)(m % LELPH_OUT_SLOTS);
+        HIP_TRY(hipEventSynchronize(c->out_ev[slot]));
+        std::memcpy(g_out + 2 * order[m] * gsz, ring + slot * gsz, slice);
+        return LELPH_OK;
+    };
     c->box_reuse = true;
     c->box_prev_valid = false;
-    for (int64_t n = 0; n < npairs; ++n)
+    int st = LELPH_OK;
+    int64_t landed = 0;
+    for (int64_t n = 0; n < npairs && st == LELPH_OK; ++n)
     {
         const int64_t i = order[n];
-        const int st = gkq_device(c, qpt, pairs[i], 1, non_loc, c->gbatch.as<z64>() + i * gsz,
-                                  c->evs.data() + 5 * i);
-        if (st != LELPH_OK)
+        st = gkq_device(c, qpt, pairs[i], 1, non_loc, c->gbatch.as<z64>() + i * gsz,
+                        c->evs.data() + 5 * i);
+        if (st != LELPH_OK) break;
+        if (n >= LELPH_OUT_SLOTS)
         {
-            c->box_reuse = false;
-            c->box_prev_valid = false;
-            return st;
+            st = land(landed++);
+            if (st != LELPH_OK) break;
         }
+        const int slot = (int)(n % LELPH_OUT_SLOTS);
+        if (hipStreamWaitEvent(c->out_stream, c->evs[5 * i + 4], 0) != hipSuccess ||
+            hipMemcpyAsync(ring + slot * gsz, c->gbatch.as<z64>() + i * gsz, slice,
+                           hipMemcpyDeviceToHost, c->out_stream) != hipSuccess ||
+            hipEventRecord(c->out_ev[slot], c->out_stream) != hipSuccess)
+            st = LELPH_ERR_HIP;
     }
     c->box_reuse = false;
     c->box_prev_valid = false;
-    HIP_TRY(hipMemcpyAsync(g_out, c->gbatch.p, sizeof(z64) * npairs * gsz, hipMemcpyDeviceToHost,
-                           c->stream));
+    if (st != LELPH_OK)
+    {
+        (void)hipStreamSynchronize(c->out_stream);
+        (void)hipStreamSynchronize(c->stream);
+        return st;
+    }
+    while (landed < npairs) LELPH_TRY(land(landed++));
     LELPH_TRY(check_err(c));
     return finish_timings(c, npairs);
 }
