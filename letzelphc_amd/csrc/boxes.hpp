@@ -77,9 +77,9 @@ __global__ __launch_bounds__(256) void k_plane_fill(const int* __restrict__ idx,
         perm[atomicAdd(&h[idx[g] / nyz], 1)] = (int)g;
 }
 
-// Stockham stages on tl contiguous lines of length N in LDS (runtime line
-// count; same butterflies as FftStagesC).
-template <typename T, int SIGN, int N, int S, int NT>
+// Stockham stages on tl lines of length N in LDS, line stride LS >= N
+// (runtime line count; same butterflies as FftStagesC).
+template <typename T, int SIGN, int N, int S, int NT, int LS = N>
 struct FftStagesP
 {
     static __device__ inline void run(cpx<T>*& in, cpx<T>*& out, const cpx<T>* W, int tid,
@@ -96,7 +96,7 @@ struct FftStagesP
                 const int t = w / NBF, j = w - t * NBF;
                 const int k = j % Ns;
                 cpx<T> v[R];
-                const cpx<T>* src = in + t * N + j;
+                const cpx<T>* src = in + t * LS + j;
                 v[0] = src[0];
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[r] = cmul(src[r * NBF], tw<T, SIGN>(W, r * k * TWS));
@@ -117,7 +117,7 @@ struct FftStagesP
 #pragma unroll
                     for (int s = 0; s < R; ++s) v[s] = o[s];
                 }
-                cpx<T>* dst = out + t * N + (j / Ns) * Ns * R + k;
+                cpx<T>* dst = out + t * LS + (j / Ns) * Ns * R + k;
 #pragma unroll
                 for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
             }
@@ -125,27 +125,31 @@ struct FftStagesP
             cpx<T>* tmp = in;
             in = out;
             out = tmp;
-            FftStagesP<T, SIGN, N, S + 1, NT>::run(in, out, W, tid, tl);
+            FftStagesP<T, SIGN, N, S + 1, NT, LS>::run(in, out, W, tid, tl);
         }
     }
 };
 
-// LDS bytes of k_plane_z for a (ny, nz) plane
-static inline size_t plane_lds_bytes(int ny, int nz)
+// LDS bytes of k_plane_z for a (ny, nz) plane (z twiddles; with the y pass
+// fused, the y twiddles and the [z][y] layout's padded line stride ny + 1)
+static inline size_t plane_lds_bytes(int ny, int nz, bool fuse_y = false)
 {
-    return sizeof(c32) * ((size_t)2 * ny * nz + nz);
+    return sizeof(c32) * (fuse_y ? (size_t)2 * (ny + 1) * nz + nz + ny : (size_t)2 * ny * nz + nz);
 }
 
 // One block per (x plane, box b = set * nspinor + sigma): gather the plane's
 // coefficients into LDS, backward FFT along z (e^{+i}, unnormalised), write
-// the plane.  Empty planes are skipped (never written).
+// the plane.  Empty planes are skipped (never written).  NY > 0 (= ny): the
+// y pass too, in LDS: the z-transformed plane transposed to [z][y], FFT of
+// its NZ lines of length NY, written back in [y][z] order, so the plane
+// leaves the kernel transformed along y and z.
 #define PZ_THREADS 512
-template <int NZ>
+template <int NZ, int NY = 0>
 __global__ __launch_bounds__(PZ_THREADS) void k_plane_z(
     const z64* __restrict__ wfc, int nspinor, int64_t npw, const int* __restrict__ idx,
     const z64* __restrict__ ph, SideGeom geo, const int* __restrict__ perm,
     const int* __restrict__ off, const int* __restrict__ cnt, int ny, int64_t nf,
-    const c32* __restrict__ Wg, c32* __restrict__ box)
+    const c32* __restrict__ Wg, const c32* __restrict__ Wgy, c32* __restrict__ box)
 {
     extern __shared__ __attribute__((aligned(16))) char pz_smem[];
     const int x = blockIdx.x;
@@ -155,11 +159,19 @@ __global__ __launch_bounds__(PZ_THREADS) void k_plane_z(
     const int64_t s = b / nspinor;
     const int sig = b - (int)s * nspinor;
     const int nyz = ny * NZ;
+    // buffers hold the [y][z] plane, or with the fused y pass the [z][y] one
+    // with line stride NY + 1 (a stride of NY lands every lane of a
+    // transpose on one bank)
+    constexpr int LY = NY > 0 ? NY + 1 : 0;
+    const int cap = NY > 0 ? NZ * LY : nyz;
     c32* p0 = reinterpret_cast<c32*>(pz_smem);
-    c32* p1 = p0 + nyz;
-    c32* W = p1 + nyz;
+    c32* p1 = p0 + cap;
+    c32* W = p1 + cap;
+    c32* Wy = W + NZ;
     const int tid = threadIdx.x;
     for (int t = tid; t < NZ; t += PZ_THREADS) W[t] = Wg[t];
+    if constexpr (NY > 0)
+        for (int t = tid; t < NY; t += PZ_THREADS) Wy[t] = Wgy[t];
     for (int t = tid; t < nyz; t += PZ_THREADS) p0[t] = c32{0.f, 0.f};
     __syncthreads();
     const z64* w = wfc + s * nspinor * npw;
@@ -186,22 +198,46 @@ __global__ __launch_bounds__(PZ_THREADS) void k_plane_z(
     c32* out = p1;
     FftStagesP<float, 1, NZ, 0, PZ_THREADS>::run(in, out, W, tid, ny);
     c32* dst = box + (int64_t)b * nf + (int64_t)x * nyz;
-    for (int t = tid; t < nyz; t += PZ_THREADS) dst[t] = in[t];
+    if constexpr (NY > 0)
+    {
+        for (int t = tid; t < nyz; t += PZ_THREADS) out[(t % NZ) * LY + t / NZ] = in[t];
+        __syncthreads();
+        c32* in2 = out;
+        c32* out2 = in;
+        FftStagesP<float, 1, NY, 0, PZ_THREADS, LY>::run(in2, out2, Wy, tid, NZ);
+        for (int t = tid; t < nyz; t += PZ_THREADS) dst[t] = in2[(t % NZ) * LY + t / NZ];
+    }
+    else
+        for (int t = tid; t < nyz; t += PZ_THREADS) dst[t] = in[t];
 }
 
-// launch k_plane_z for a compile-time NZ; false if NZ is not compiled
+// launch k_plane_z for a compile-time NZ; false if NZ is not compiled.
+// Wy not NULL asks for the fused y pass (cubic y-z planes, ny == nz); false
+// if that form is not compiled for this size or does not fit in LDS.
 static bool plane_z_launch(int nz, int nx, int nbox, size_t lds, const z64* wfc, int nspinor,
                            int64_t npw, const int* idx, const z64* ph, const SideGeom& geo,
                            const int* perm, const int* off, const int* cnt, int ny, int64_t nf,
-                           const c32* W, c32* box, hipStream_t st)
+                           const c32* W, const c32* Wy, c32* box, hipStream_t st)
 {
+    if (Wy && (ny != nz || lds > 160 * 1024)) return false;
 #define PZ_CASE(NZV)                                                                              \
     case NZV:                                                                                     \
+        if (Wy)                                                                                   \
+        {                                                                                         \
+            if (hipFuncSetAttribute((const void*)k_plane_z<NZV, NZV>,                             \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=      \
+                hipSuccess)                                                                       \
+                return false;                                                                     \
+            hipLaunchKernelGGL((k_plane_z<NZV, NZV>), dim3((unsigned)nx, (unsigned)nbox),         \
+                               dim3(PZ_THREADS), lds, st, wfc, nspinor, npw, idx, ph, geo, perm,  \
+                               off, cnt, ny, nf, W, Wy, box);                                     \
+            return true;                                                                          \
+        }                                                                                         \
         if (hipFuncSetAttribute((const void*)k_plane_z<NZV>,                                      \
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) \
             return false;                                                                         \
         hipLaunchKernelGGL(k_plane_z<NZV>, dim3((unsigned)nx, (unsigned)nbox), dim3(PZ_THREADS), lds, st, \
-                           wfc, nspinor, npw, idx, ph, geo, perm, off, cnt, ny, nf, W, box);      \
+                           wfc, nspinor, npw, idx, ph, geo, perm, off, cnt, ny, nf, W, nullptr, box); \
         return true;
     switch (nz)
     {

@@ -354,8 +354,9 @@ static int ensure_dv32(lelph_ctx* c)
 
 // psi_k(r), psi_kq(r) boxes: scatter + backward FFT
 // one side of the pruned FP32 transform (boxes.hpp): group the sphere by x
-// plane, gather + z transform per occupied plane, y pass on occupied planes,
-// x pass reading occupied planes only
+// plane, gather + z transform per occupied plane (and its y transform in
+// LDS when the y-z plane is square), else a y pass on occupied planes; the
+// x pass reads occupied planes only
 static int pruned_side(lelph_ctx* c, const DevWfc& w, const SideGeom& geo, const int* idx,
                        const z64* ph, int side, c32* box)
 {
@@ -376,14 +377,22 @@ static int pruned_side(lelph_ctx* c, const DevWfc& w, const SideGeom& geo, const
     hipLaunchKernelGGL(k_plane_fill, dim3(gb), dim3(256), 0, c->stream, idx, w.npw, nyz, Nx,
                        c->pl_cur[side].as<int>(), c->pl_perm[side].as<int>());
     HIP_TRY(hipGetLastError());
-    if (!plane_z_launch(Nz, Nx, nbox, plane_lds_bytes(Ny, Nz), w.wfc, L.nspinor, w.npw, idx, ph,
-                        geo, c->pl_perm[side].as<int>(), c->pl_off[side].as<int>(), cnt, Ny,
-                        c->nf, c->ax[2].w32, box, c->stream))
-        return LELPH_ERR_UNSUPPORTED;
-    HIP_TRY(hipGetLastError());
-    if (!fft_try_c<float>(Ny, box, (int64_t)nbox * Nx, Nz, nyz, 1, Nz, +1, c->ax[1].w32, c->stream,
-                          cnt, Nx, nullptr))
-        return LELPH_ERR_UNSUPPORTED;
+    // cubic y-z planes: the y transform joins the z one in LDS
+    const bool fused = plane_z_launch(Nz, Nx, nbox, plane_lds_bytes(Ny, Nz, true), w.wfc,
+                                      L.nspinor, w.npw, idx, ph, geo, c->pl_perm[side].as<int>(),
+                                      c->pl_off[side].as<int>(), cnt, Ny, c->nf, c->ax[2].w32,
+                                      c->ax[1].w32, box, c->stream);
+    if (!fused)
+    {
+        if (!plane_z_launch(Nz, Nx, nbox, plane_lds_bytes(Ny, Nz), w.wfc, L.nspinor, w.npw, idx,
+                            ph, geo, c->pl_perm[side].as<int>(), c->pl_off[side].as<int>(), cnt,
+                            Ny, c->nf, c->ax[2].w32, nullptr, box, c->stream))
+            return LELPH_ERR_UNSUPPORTED;
+        HIP_TRY(hipGetLastError());
+        if (!fft_try_c<float>(Ny, box, (int64_t)nbox * Nx, Nz, nyz, 1, Nz, +1, c->ax[1].w32,
+                              c->stream, cnt, Nx, nullptr))
+            return LELPH_ERR_UNSUPPORTED;
+    }
     HIP_TRY(hipGetLastError());
     if (!fft_try_c<float>(Nx, box, nbox, nyz, c->nf, 1, nyz, +1, c->ax[0].w32, c->stream, nullptr,
                           1, cnt))
