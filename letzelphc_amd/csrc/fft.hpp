@@ -313,7 +313,9 @@ constexpr int fft_ns_c(int n, int s) // product of the radices before stage s
     return r;
 }
 
-template <typename T, int SIGN, int N, int TL, int S>
+// Lines sit LS elements apart in LDS (LS = N + 1 keeps the transposing
+// loads and stores of k_fft_lines_c free of bank conflicts).
+template <typename T, int SIGN, int N, int TL, int S, int LS = N>
 struct FftStagesC
 {
     static __device__ inline void run(cpx<T>*& in, cpx<T>*& out, const cpx<T>* W, int tid)
@@ -329,7 +331,7 @@ struct FftStagesC
                 const int t = w / NBF, j = w - t * NBF;
                 const int k = j % Ns;
                 cpx<T> v[R];
-                const cpx<T>* src = in + t * N + j;
+                const cpx<T>* src = in + t * LS + j;
                 v[0] = src[0];
 #pragma unroll
                 for (int r = 1; r < R; ++r) v[r] = cmul(src[r * NBF], tw<T, SIGN>(W, r * k * TWS));
@@ -350,7 +352,7 @@ struct FftStagesC
 #pragma unroll
                     for (int s = 0; s < R; ++s) v[s] = o[s];
                 }
-                cpx<T>* dst = out + t * N + (j / Ns) * Ns * R + k;
+                cpx<T>* dst = out + t * LS + (j / Ns) * Ns * R + k;
 #pragma unroll
                 for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
             }
@@ -358,13 +360,17 @@ struct FftStagesC
             cpx<T>* tmp = in;
             in = out;
             out = tmp;
-            FftStagesC<T, SIGN, N, TL, S + 1>::run(in, out, W, tid);
+            FftStagesC<T, SIGN, N, TL, S + 1, LS>::run(in, out, W, tid);
         }
     }
 };
 
 // One axis pass, compile-time N and TL (see k_fft_lines for the addressing).
 // Lines i >= n_inner of the last tile are computed on zeros and not stored.
+// In LDS the lines are N + 1 elements apart: where adjacent lines are
+// adjacent in memory (lstride 1), consecutive threads move one element of
+// consecutive lines, which a stride of N (a multiple of the bank count for
+// N = 96) would put on one bank.
 // Pruning (both optional): blocks whose outer index o has ocnt[o % omod] == 0
 // do nothing (their lines are all zero and are never read); elements j with
 // emask[j] == 0 are taken as zero without being read.
@@ -376,9 +382,10 @@ __global__ __launch_bounds__(256) void k_fft_lines_c(cpx<T>* __restrict__ data, 
                                                      const int* __restrict__ ocnt, int omod,
                                                      const int* __restrict__ emask)
 {
+    constexpr int LS = N + 1;
     __shared__ cpx<T> W[N];
-    __shared__ cpx<T> b0[TL * N];
-    __shared__ cpx<T> b1[TL * N];
+    __shared__ cpx<T> b0[TL * LS];
+    __shared__ cpx<T> b1[TL * LS];
     const int tid = threadIdx.x;
     const int64_t tiles = (n_inner + TL - 1) / TL;
     const int64_t o = blockIdx.x / tiles;
@@ -393,8 +400,8 @@ __global__ __launch_bounds__(256) void k_fft_lines_c(cpx<T>* __restrict__ data, 
         for (int f = tid; f < TL * N; f += 256)
         {
             const int j = f / TL, t = f - j * TL;
-            b0[t * N + j] = (t < nl && (!emask || emask[j])) ? base[(int64_t)j * estride + t]
-                                                             : cpx<T>{0, 0};
+            b0[t * LS + j] = (t < nl && (!emask || emask[j])) ? base[(int64_t)j * estride + t]
+                                                              : cpx<T>{0, 0};
         }
     }
     else
@@ -402,21 +409,21 @@ __global__ __launch_bounds__(256) void k_fft_lines_c(cpx<T>* __restrict__ data, 
         for (int f = tid; f < TL * N; f += 256)
         {
             const int t = f / N, j = f - t * N;
-            b0[t * N + j] = (t < nl && (!emask || emask[j]))
-                                ? base[(int64_t)t * lstride + (int64_t)j * estride]
-                                : cpx<T>{0, 0};
+            b0[t * LS + j] = (t < nl && (!emask || emask[j]))
+                                 ? base[(int64_t)t * lstride + (int64_t)j * estride]
+                                 : cpx<T>{0, 0};
         }
     }
     __syncthreads();
     cpx<T>* in = b0;
     cpx<T>* out = b1;
-    FftStagesC<T, SIGN, N, TL, 0>::run(in, out, W, tid);
+    FftStagesC<T, SIGN, N, TL, 0, LS>::run(in, out, W, tid);
     if (lstride == 1)
     {
         for (int f = tid; f < TL * N; f += 256)
         {
             const int j = f / TL, t = f - j * TL;
-            if (t < nl) base[(int64_t)j * estride + t] = in[t * N + j];
+            if (t < nl) base[(int64_t)j * estride + t] = in[t * LS + j];
         }
     }
     else
@@ -424,17 +431,17 @@ __global__ __launch_bounds__(256) void k_fft_lines_c(cpx<T>* __restrict__ data, 
         for (int f = tid; f < TL * N; f += 256)
         {
             const int t = f / N, j = f - t * N;
-            if (t < nl) base[(int64_t)t * lstride + (int64_t)j * estride] = in[t * N + j];
+            if (t < nl) base[(int64_t)t * lstride + (int64_t)j * estride] = in[t * LS + j];
         }
     }
 }
 
-// lines per block: two buffers of TL*N plus the twiddles within ~60 KB
+// lines per block: two buffers of TL*(N+1) plus the twiddles within ~60 KB
 template <typename T, int N>
 constexpr int fft_tl_c()
 {
-    return (N * (int)sizeof(cpx<T>) * 2 * 32 <= 56 * 1024) ? 32
-           : (N * (int)sizeof(cpx<T>) * 2 * 16 <= 56 * 1024) ? 16
+    return ((N + 1) * (int)sizeof(cpx<T>) * 2 * 32 <= 56 * 1024) ? 32
+           : ((N + 1) * (int)sizeof(cpx<T>) * 2 * 16 <= 56 * 1024) ? 16
            : 8;
 }
 

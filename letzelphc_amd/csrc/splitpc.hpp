@@ -33,6 +33,9 @@
 #define PC_TR 64  // rows of a block tile
 #define PC_FLUSH 16 // chunks per FP32 accumulation chain (512 points)
 #define PC_TC 128 // columns of a block tile
+#ifndef PC_BRING
+#define PC_BRING 3 // B buffers in the ring (copies run PC_BRING - 1 chunks ahead)
+#endif
 
 template <int NP>
 __host__ __device__ constexpr int pc_a_bytes()
@@ -44,12 +47,12 @@ __host__ __device__ constexpr int pc_b_bytes()
 {
     return NP * 2 * 4 * 2048;
 }
-// A pieces double buffered, B pieces in a ring of 3 (copies run two chunks
-// ahead of their use)
+// A pieces double buffered, B pieces in a ring of PC_BRING (copies run
+// PC_BRING - 1 chunks ahead of their use)
 template <int NP>
 static inline size_t pc_lds_bytes()
 {
-    return 2 * (size_t)pc_a_bytes<NP>() + 3 * (size_t)pc_b_bytes<NP>();
+    return 2 * (size_t)pc_a_bytes<NP>() + PC_BRING * (size_t)pc_b_bytes<NP>();
 }
 
 // one chunk of this lane's A row and dV: 8 complex each
@@ -274,7 +277,7 @@ __device__ __forceinline__ void pc_consume16(const lds_char* L, int n, int w4, i
         }
         lds_barrier();
         u ^= 1;
-        u3 = (u3 == 2) ? 0 : u3 + 1;
+        u3 = (u3 == PC_BRING - 1) ? 0 : u3 + 1;
     }
     const int row = n0 + 16 * w4;
 #pragma unroll
@@ -301,7 +304,8 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
     constexpr int B_BYTES = pc_b_bytes<NP>();
     extern __shared__ __attribute__((aligned(16))) char pc_smem[];
     lds_char* L = (lds_char*)pc_smem;
-    // A buffer u at L + u*A_BYTES (u = 0, 1), B buffer t at L + 2*A_BYTES + t*B_BYTES (t = 0..2)
+    // A buffer u at L + u*A_BYTES (u = 0, 1), B buffer t at
+    // L + 2*A_BYTES + t*B_BYTES (t = 0 .. PC_BRING - 1)
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool producer = wave >= 4;
@@ -324,30 +328,38 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
     const int n = (int)(c1 - c0); // chunks of this block
     if (wave >= 6)
     {
-        // copy waves: B(i+2) is started in step i, B(i+1) must have landed
-        // before the barrier that ends step i; a wave waits only for its own
-        // copies, PC_BCOPY per chunk
+        // copy waves: B(i + PC_BRING - 1) is started in step i, B(i+1) must
+        // have landed before the barrier that ends step i; a wave waits only
+        // for its own copies, PC_BCOPY per chunk
+        static_assert(PC_BRING == 3 || PC_BRING == 4, "B ring of 3 or 4");
         const int pw = wave - 6;
         lds_char* LB = L + 2 * A_BYTES;
-        pc_stage_B<NP>(Bp, a, is, m0, c0, LB, pw, lane);
-        if (n > 1) pc_stage_B<NP>(Bp, a, is, m0, c0 + 1, LB + B_BYTES, pw, lane);
-        if (n > 1)
-            __builtin_amdgcn_s_waitcnt(waitcnt_vm(PC_BCOPY(NP))); // B(0) landed
+        for (int k = 0; k < PC_BRING - 1 && k < n; ++k)
+            pc_stage_B<NP>(Bp, a, is, m0, c0 + k, LB + k * B_BYTES, pw, lane);
+        // B(0) landed: later prologue copies may stay in flight
+        if (PC_BRING == 4 && n > 2)
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(2 * PC_BCOPY(NP)));
+        else if (n > 1)
+            __builtin_amdgcn_s_waitcnt(waitcnt_vm(PC_BCOPY(NP)));
         else
             __builtin_amdgcn_s_waitcnt(0);
         lds_barrier();
-        int t3 = 2;
+        int t3 = PC_BRING - 1;
         for (int i = 0; i < n; ++i)
         {
-            if (i + 2 < n)
+            if (i + PC_BRING - 1 < n)
             {
-                pc_stage_B<NP>(Bp, a, is, m0, c0 + i + 2, LB + t3 * B_BYTES, pw, lane);
-                __builtin_amdgcn_s_waitcnt(waitcnt_vm(PC_BCOPY(NP))); // B(i+1) landed
+                pc_stage_B<NP>(Bp, a, is, m0, c0 + i + PC_BRING - 1, LB + t3 * B_BYTES, pw,
+                               lane);
+                // B(i+1) landed, the PC_BRING - 2 later chunks may be in flight
+                __builtin_amdgcn_s_waitcnt(waitcnt_vm((PC_BRING - 2) * PC_BCOPY(NP)));
             }
+            else if (PC_BRING == 4 && i + 3 == n)
+                __builtin_amdgcn_s_waitcnt(waitcnt_vm(PC_BCOPY(NP))); // B(i+2) may fly
             else
                 __builtin_amdgcn_s_waitcnt(0);
             lds_barrier();
-            t3 = (t3 == 2) ? 0 : t3 + 1;
+            t3 = (t3 == PC_BRING - 1) ? 0 : t3 + 1;
         }
         return;
     }
