@@ -99,6 +99,7 @@ int lelph_asr_fc(int mode, const int64_t qgrid[3], int natom, double* frc,
     if (mode == LELPH_ASR_NONE) return LELPH_OK;
     if (!qgrid || natom < 1 || !frc || mode < 0 || mode > LELPH_ASR_ALL_HUANG)
         return LELPH_ERR_ARG;
+    if (qgrid[0] < 1 || qgrid[1] < 1 || qgrid[2] < 1) return LELPH_ERR_ARG;
     const int nm = 3 * natom;
     const int64_t ng = qgrid[0] * qgrid[1] * qgrid[2], gyz = qgrid[1] * qgrid[2];
     const int64_t nn = (int64_t)nm * nm;

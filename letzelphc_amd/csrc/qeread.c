@@ -46,6 +46,7 @@ static int qe_line(FILE* fp, char* buf)
 
 static int parse_dyn_xml(const char* path, int natom, double* qpt, double* masses,
                          double* dynd);
+static char* slurp(const char* path, long* size);
 
 /* first non-blank line holds "<?xml" (is_dyn_xml, src/io/qe/read_dyn.c:611-641) */
 static int is_dyn_xml(FILE* fp)
@@ -70,6 +71,7 @@ static int is_dyn_xml(FILE* fp)
  * LELPH_ERR_ARG. */
 int lelph_qe_parse_dyn(const char* path, int natom, double* qpt, double* masses, double* dynd)
 {
+    if (!path || natom < 1 || !qpt || !masses || !dynd) return LELPH_ERR_ARG;
     FILE* fp = fopen(path, "r");
     if (!fp) return LELPH_ERR_ARG;
     if (is_dyn_xml(fp))
@@ -223,20 +225,9 @@ static int xml_elem(const char* b, const char* e, const char* tag, const char** 
 int lelph_read_patterns_qe(const char* path, int natom, double* pat_vecs)
 {
     if (!path || natom < 1 || !pat_vecs) return LELPH_ERR_ARG;
-    FILE* fp = fopen(path, "rb");
-    if (!fp) return LELPH_ERR_ARG;
-    fseek(fp, 0, SEEK_END);
-    const long sz = ftell(fp);
-    fseek(fp, 0, SEEK_SET);
-    char* s = malloc((size_t)sz + 1);
-    if (!s || fread(s, 1, (size_t)sz, fp) != (size_t)sz)
-    {
-        free(s);
-        fclose(fp);
-        return LELPH_ERR_ARG;
-    }
-    fclose(fp);
-    s[sz] = 0;
+    long sz = 0;
+    char* s = slurp(path, &sz);
+    if (!s) return LELPH_ERR_ARG;
     const int nm = 3 * natom;
     const char* e = s + sz;
     const char *ib, *ie;
@@ -644,7 +635,7 @@ int lelph_parse_qexml(const char* path, lelph_qexml_info* info, int* atom_type,
         goto done;
     }
     I.natom = atoll(buf);
-    if (!xml_attr(&as, "alat", buf, 64))
+    if (I.natom < 1 || !xml_attr(&as, "alat", buf, 64))
     {
         free(names);
         goto done;
@@ -902,6 +893,7 @@ int lelph_read_qpts_qe(const char* dyn0, int64_t* nq_ibz, int64_t* nq_bz, double
     int st = LELPH_ERR_ARG;
     if (!fgets(line, sizeof(line), fp) || sscanf(line, "%lld %lld %lld", g, g + 1, g + 2) != 3)
         goto done;
+    if (g[0] < 1 || g[1] < 1 || g[2] < 1) goto done;
     if (!fgets(line, sizeof(line), fp) || sscanf(line, "%lld", &n) != 1 || n < 1) goto done;
     *nq_ibz = n;
     *nq_bz = g[0] * g[1] * g[2];
