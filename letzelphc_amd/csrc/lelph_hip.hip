@@ -669,14 +669,15 @@ static int run_local_split_t(lelph_ctx* c, const double* qpt, const lelph_pair& 
     sa.nmag = nmag_eff;
     const bool fast = L.nspinor == 1 && nmag_eff != 4 && c->nf % SP_KC == 0;
     // producer/consumer kernel: bf16x3 on the aligned path with enough bands
-    // to fill its 64 x 128 tiles
+    // to fill its 128-column tiles
     const bool pc = fast && NP == 2 && nb >= 64;
-    sa.ntn = pc ? (int)cdiv(nb, PC_TR) : nbp / SP_TR;
+    sa.nmt = pc ? (int)cdiv(nm, PC_MG) : nm;
+    sa.ntn = pc ? (int)cdiv(nb, PC_TR / PC_MG) : nbp / SP_TR;
     sa.ntm = pc ? nbp / PC_TC : (int)cdiv(nb, SP_TC);
     sa.N = c->nf;
     sa.nchunks = nchunks;
     // nspin * nsplit must be a multiple of 8 (XCD grouping); ~768+ blocks
-    const int per_group = nm * sa.ntn * sa.ntm;
+    const int per_group = sa.nmt * sa.ntn * sa.ntm;
     int64_t ns = cdiv(768, (int64_t)per_group * nspin);
     ns = std::max<int64_t>(8, cdiv(ns, 8) * 8);
     ns = std::min<int64_t>(ns, std::max<int64_t>(8, (nchunks / 8) * 8));

@@ -124,7 +124,8 @@ __global__ void k_presplit_B(const c32* __restrict__ box, int nb, int nbp, int n
 struct SplitArgs
 {
     int nm, nb, nbp, nspin, nspinor, nmag;
-    int ntn, ntm; // row / column tiles per mode
+    int nmt;      // mode tiles (k_contract_split_pc: modes in groups; otherwise nm)
+    int ntn, ntm; // row / column tiles per mode tile
     int nsplit;   // K splits; nspin * nsplit is a multiple of 8
     int64_t N, nchunks, chunks_per_split;
 };

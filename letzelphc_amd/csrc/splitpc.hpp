@@ -6,14 +6,17 @@
 // flushed into FP64.  The difference is the division of labour inside a
 // block, which lets the VALU work of the split overlap the MFMAs:
 //
-//   block tile: one mode x 64 k bands (rows) x 128 k+q bands (columns) x one
-//   K range; 8 waves, 4 producers and 4 consumers (wave w runs on SIMD
-//   w % 4, so every SIMD holds one of each).
-//   producer p: loads its 16 rows of psi_k and the chunk's dV straight into
-//     registers (two chunks ahead), forms dV psi_k in FP32, splits it and
-//     writes the A fragments of row block p into LDS in MFMA lane order; it
-//     also issues the global -> LDS copies (global_load_lds) of the next
-//     chunk's B pieces.
+//   block tile: PC_MG modes x PC_TR / PC_MG k bands (rows) x 128 k+q bands
+//   (columns) x one K range, i.e. four A row blocks of 16: with PC_MG = 4
+//   one per mode over the same 16 rows (psi_k is loaded once for four
+//   modes), with PC_MG = 1 four row blocks of one mode.  8 waves: 4
+//   consumers, 2 producers, 2 copy waves (wave w runs on SIMD w % 4).
+//   producer p (0, 1): loads psi_k and dV of the A row blocks 2p, 2p+1
+//     straight into registers (three chunks ahead), forms dV psi_k in
+//     FP32, splits it and writes their A fragments into LDS in MFMA lane
+//     order.
+//   copy waves: global -> LDS copies (global_load_lds) of the B pieces of
+//     the chunks ahead.
 //   consumer p: reads the A fragments of row block p and the B fragments of
 //     the 8 column blocks from LDS and runs the 96 MFMAs of the chunk.
 //   One barrier per chunk; A pieces are double buffered (chunk c+1 is
@@ -30,7 +33,11 @@
 
 #include "split.hpp"
 
-#define PC_TR 64  // rows of a block tile
+#define PC_TR 64  // A rows of a block tile (modes x bands)
+#ifndef PC_MG
+#define PC_MG 4   // modes per block tile (1 or 4)
+#endif
+static_assert(PC_MG == 1 || PC_MG == 4, "PC_MG is 1 or 4");
 #define PC_FLUSH 16 // chunks per FP32 accumulation chain (512 points)
 #define PC_TC 128 // columns of a block tile
 #ifndef PC_BRING
@@ -140,21 +147,31 @@ __device__ inline void lds_barrier()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// one chunk of this lane's two A rows and dV
+// one chunk of this lane's two A row blocks: two psi_k rows and one dV
+// (PC_MG = 1), or one psi_k row and the dV of two modes (PC_MG = 4)
 struct PcRaw2
 {
+#if PC_MG == 1
     uint4 a[2][4];
-    uint4 d[4];
+    uint4 d[1][4];
+#else
+    uint4 a[1][4];
+    uint4 d[2][4];
+#endif
 };
 
-__device__ inline void pc_load_raw2(PcRaw2& r, const uint4* const* ap, const uint4* dp, int i)
+__device__ inline void pc_load_raw2(PcRaw2& r, const uint4* const* ap, const uint4* const* dp,
+                                    int i)
 {
+    constexpr int NA = PC_MG == 1 ? 2 : 1;
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+    for (int h = 0; h < NA; ++h)
 #pragma unroll
         for (int k = 0; k < 4; ++k) r.a[h][k] = ap[h][i * (SP_KC / 2) + k];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) r.d[k] = dp[i * (SP_KC / 2) + k];
+    for (int h = 0; h < 3 - NA; ++h)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r.d[h][k] = dp[h][i * (SP_KC / 2) + k];
 }
 
 template <int NP>
@@ -167,8 +184,8 @@ __device__ inline void pc_produce2(const PcRaw2& r, lds_char* LA, int pw, int la
 #pragma unroll
         for (int k = 0; k < 4; ++k)
         {
-            one.a[k] = r.a[h][k];
-            one.d[k] = r.d[k];
+            one.a[k] = r.a[PC_MG == 1 ? h : 0][k];
+            one.d[k] = r.d[PC_MG == 1 ? 0 : h][k];
         }
         pc_produce<NP>(one, LA, 2 * pw + h, lane);
     }
@@ -178,7 +195,7 @@ __device__ inline void pc_produce2(const PcRaw2& r, lds_char* LA, int pw, int la
 // barrier (its lgkmcnt wait covers the LDS writes)
 template <int NP>
 __device__ __forceinline__ void pc_split_step(int i, int n, const PcRaw2& rp, PcRaw2& rl,
-                                              const uint4* const* ap, const uint4* dp,
+                                              const uint4* const* ap, const uint4* const* dp,
                                               lds_char* L, int pw, int lane)
 {
     constexpr int A_BYTES = pc_a_bytes<NP>();
@@ -243,12 +260,13 @@ __device__ __forceinline__ void pc_flush_block(flt4v& ar, flt4v& ai, double* dr,
     }
 }
 
-// consumer: rows n0 + 16 w4 + .., columns m0 .. m0 + 127 (8 blocks of 16);
-// FP32 chains of PC_FLUSH chunks flushed into FP64
+// consumer: A row block w4 = mode v, rows row .. row + 15, columns m0 ..
+// m0 + 127 (8 blocks of 16); FP32 chains of PC_FLUSH chunks flushed into
+// FP64; modes v >= nm (padding of the last mode group) are not stored
 template <int NP>
 __device__ __forceinline__ void pc_consume16(const lds_char* L, int n, int w4, int lane,
                                              const SplitArgs& a, z64* __restrict__ part,
-                                             int split, int v, int is, int n0, int m0)
+                                             int split, int v, int is, int row, int m0)
 {
     constexpr int A_BYTES = pc_a_bytes<NP>();
     constexpr int B_BYTES = pc_b_bytes<NP>();
@@ -279,7 +297,7 @@ __device__ __forceinline__ void pc_consume16(const lds_char* L, int n, int w4, i
         u ^= 1;
         u3 = (u3 == PC_BRING - 1) ? 0 : u3 + 1;
     }
-    const int row = n0 + 16 * w4;
+    if (v >= a.nm) return;
 #pragma unroll
     for (int cb = 0; cb < 8; ++cb)
 #pragma unroll
@@ -311,16 +329,16 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
     const bool producer = wave >= 4;
     const int w4 = wave & 3;
     // XCD-aware decode: group g = (spin, split) runs on XCD g % 8
-    const int per_group = a.nm * a.ntn * a.ntm;
+    const int per_group = a.nmt * a.ntn * a.ntm;
     const int b = blockIdx.x;
     const int x = b & 7, j = b >> 3;
     const int g = x + 8 * (j / per_group);
     const int tile = j % per_group;
-    const int v = tile / (a.ntn * a.ntm);
+    const int vt = tile / (a.ntn * a.ntm); // mode tile
     const int tn = (tile / a.ntm) % a.ntn, tm = tile % a.ntm;
     const int is = g / a.nsplit, split = g % a.nsplit;
     if (is >= a.nspin) return;
-    const int n0 = tn * PC_TR, m0 = tm * PC_TC;
+    const int n0 = tn * (PC_TR / PC_MG), m0 = tm * PC_TC;
     const int64_t c0 = (int64_t)split * a.chunks_per_split;
     const int64_t c1 = (c0 + a.chunks_per_split < a.nchunks) ? c0 + a.chunks_per_split : a.nchunks;
     if (c0 >= c1) return; // uniform over the block
@@ -365,22 +383,27 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
     }
     if (producer)
     {
-        // split waves: rows 32 pw + lane%16 (+16) of the tile, points
-        // 8(lane/16)..+7 of every chunk; raw chunk i+3 loads while i+1 is
-        // split (three register sets, loads clamped to the last chunk so
-        // every set keeps fixed registers)
+        // split waves: A row blocks 2 pw, 2 pw + 1 (rows n0 + 32 pw + 16 h
+        // of mode vt, or rows n0 of modes 4 vt + 2 pw + h), row lane%16,
+        // points 8(lane/16)..+7 of every chunk; raw chunk i+3 loads while
+        // i+1 is split (three register sets, loads clamped to the last chunk
+        // so every set keeps fixed registers); padding modes read the last
+        // mode and padding rows the last band
         const int pw = wave - 4;
         const uint4* ap[2];
+        const uint4* dp[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h)
         {
-            const int arow = min(n0 + 32 * pw + 16 * h + (lane & 15), a.nb - 1);
+            const int arow =
+                min(n0 + (PC_MG == 1 ? 32 * pw + 16 * h : 0) + (lane & 15), a.nb - 1);
+            const int vm = min(PC_MG == 1 ? vt : PC_MG * vt + 2 * pw + h, a.nm - 1);
             ap[h] = reinterpret_cast<const uint4*>(psik + (int64_t)(is * a.nb + arow) * a.N +
                                                    8 * (lane >> 4)) + c0 * (SP_KC / 2);
+            dp[h] = reinterpret_cast<const uint4*>(
+                        dv + ((int64_t)vm * a.nmag + (a.nmag == 2 ? is : 0)) * a.N +
+                        8 * (lane >> 4)) + c0 * (SP_KC / 2);
         }
-        const uint4* dp = reinterpret_cast<const uint4*>(
-            dv + ((int64_t)v * a.nmag + (a.nmag == 2 ? is : 0)) * a.N + 8 * (lane >> 4)) +
-            c0 * (SP_KC / 2);
         PcRaw2 R0, R1, R2;
         pc_load_raw2(R0, ap, dp, 0);
         pc_produce2<NP>(R0, L, pw, lane);
@@ -397,5 +420,8 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
         return;
     }
 
-    pc_consume16<NP>(L, n, w4, lane, a, part, split, v, is, n0, m0);
+    if (PC_MG == 1)
+        pc_consume16<NP>(L, n, w4, lane, a, part, split, vt, is, n0 + 16 * w4, m0);
+    else
+        pc_consume16<NP>(L, n, w4, lane, a, part, split, PC_MG * vt + w4, is, n0, m0);
 }
