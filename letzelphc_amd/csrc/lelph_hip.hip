@@ -678,7 +678,7 @@ static int run_local_split_t(lelph_ctx* c, const double* qpt, const lelph_pair& 
     sa.nchunks = nchunks;
     // nspin * nsplit must be a multiple of 8 (XCD grouping); ~768+ blocks
     const int per_group = sa.nmt * sa.ntn * sa.ntm;
-    int64_t ns = cdiv(768, (int64_t)per_group * nspin);
+    int64_t ns = cdiv(pc ? PC_BLOCKS : 768, (int64_t)per_group * nspin);
     ns = std::max<int64_t>(8, cdiv(ns, 8) * 8);
     ns = std::min<int64_t>(ns, std::max<int64_t>(8, (nchunks / 8) * 8));
     sa.nsplit = (int)ns;

@@ -38,6 +38,9 @@
 #define PC_MG 4   // modes per block tile (1 or 4)
 #endif
 static_assert(PC_MG == 1 || PC_MG == 4, "PC_MG is 1 or 4");
+#ifndef PC_BLOCKS
+#define PC_BLOCKS 768 // target block count (sets the number of K splits)
+#endif
 #define PC_FLUSH 16 // chunks per FP32 accumulation chain (512 points)
 #define PC_TC 128 // columns of a block tile
 #ifndef PC_BRING
