@@ -263,6 +263,27 @@ __device__ __forceinline__ void pc_flush_block(flt4v& ar, flt4v& ai, double* dr,
     }
 }
 
+// partials of an empty K range (more splits than chunks): zeros, so the
+// fixed-order reduction over all splits reads defined values
+__device__ inline void pc_store_zero(const SplitArgs& a, z64* __restrict__ part, int split, int v,
+                                     int is, int row, int m0, int lane)
+{
+    if (v >= a.nm) return;
+#pragma unroll
+    for (int cb = 0; cb < 8; ++cb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const int rr = row + 4 * (lane >> 4) + e;
+            const int cc = m0 + cb * 16 + (lane & 15);
+            if (rr < a.nb && cc < a.nb)
+            {
+                const int64_t o = ((((int64_t)split * a.nm + v) * a.nspin + is) * a.nb + rr) * a.nb + cc;
+                part[o] = {0.0, 0.0};
+            }
+        }
+}
+
 // consumer: A row block w4 = mode v, rows row .. row + 15, columns m0 ..
 // m0 + 127 (8 blocks of 16); FP32 chains of PC_FLUSH chunks flushed into
 // FP64; modes v >= nm (padding of the last mode group) are not stored
@@ -342,9 +363,16 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
     const int is = g / a.nsplit, split = g % a.nsplit;
     if (is >= a.nspin) return;
     const int n0 = tn * (PC_TR / PC_MG), m0 = tm * PC_TC;
+    // mode and first row of consumer w4
+    const int vc = PC_MG == 1 ? vt : PC_MG * vt + w4;
+    const int rowc = PC_MG == 1 ? n0 + 16 * w4 : n0;
     const int64_t c0 = (int64_t)split * a.chunks_per_split;
     const int64_t c1 = (c0 + a.chunks_per_split < a.nchunks) ? c0 + a.chunks_per_split : a.nchunks;
-    if (c0 >= c1) return; // uniform over the block
+    if (c0 >= c1) // uniform over the block; no barrier is reached
+    {
+        if (wave < 4) pc_store_zero(a, part, split, vc, is, rowc, m0, lane);
+        return;
+    }
 
     const int n = (int)(c1 - c0); // chunks of this block
     if (wave >= 6)
@@ -423,8 +451,5 @@ __global__ __launch_bounds__(SP_THREADS, 2) void k_contract_split_pc(
         return;
     }
 
-    if (PC_MG == 1)
-        pc_consume16<NP>(L, n, w4, lane, a, part, split, vt, is, n0 + 16 * w4, m0);
-    else
-        pc_consume16<NP>(L, n, w4, lane, a, part, split, PC_MG * vt + w4, is, n0, m0);
+    pc_consume16<NP>(L, n, w4, lane, a, part, split, vc, is, rowc, m0);
 }
