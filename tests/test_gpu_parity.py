@@ -114,6 +114,37 @@ def test_compute_elphq_reduced(c0, c0ctx, prec, tol):
     assert err < tol
 
 
+@pytest.mark.parametrize("prec,tol", [("fp32", TOL32), ("bf16x6", TOL32), ("bf16x3", 5e-5)])
+@pytest.mark.parametrize("kind", ["spin2", "spinor_nmag1", "spinor_nmag4", "soc_nmag4"])
+def test_tiny_variants_reduced(kind, prec, tol):
+    """FP32 and split-product contractions on the spin / spinor / 2x2
+    potential branches (the split kernel's element-wise staging path for
+    spinors and nmag 4, spin-resolved dV for nmag 2), against the oracle."""
+    kw = dict(nbnds=4)
+    if kind == "spin2":
+        kw.update(nspin=2, nmag=2)
+    elif kind == "spinor_nmag1":
+        kw.update(nspinor=2, nmag=1)
+    elif kind == "spinor_nmag4":
+        kw.update(nspinor=2, nmag=4)
+    elif kind == "soc_nmag4":
+        kw.update(nspinor=2, nmag=4, soc=True, syms="cubic", q=(0.0, 0.5, 0.0))
+    s = synth.tiny(**kw)
+    o = O.OracleSystem(s)
+    say(f"case: tiny {kind} {prec}")
+    ctx = lelph.Context(s, "fp64")
+    try:
+        ctx.set_dvscf(s.qpts[0], s.eigvec(0), s.dvscf(0), kernel="dfpt")
+        ctx.set_precision(prec)
+        g = ctx.compute_elphq(s.qpts[0], non_loc=True)
+    finally:
+        ctx.close()
+    ref = o.compute_elphq(0, o.dv_total(0), non_loc=True)
+    err = rel_err(g, ref)
+    say(f"  {prec}: |dg|/|g|max = {err:.3e}")
+    assert err < tol
+
+
 @pytest.mark.parametrize("kind", ["spin2", "spinor_nmag1", "spinor_nmag4", "cubic_syms",
                                   "bare", "dfpt_local", "slab_2d", "soc", "soc_nmag4"])
 def test_tiny_variants(kind):
