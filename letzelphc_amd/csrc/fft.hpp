@@ -436,12 +436,15 @@ __global__ __launch_bounds__(256) void k_fft_lines_c(cpx<T>* __restrict__ data, 
     }
 }
 
-// lines per block: two buffers of TL*(N+1) plus the twiddles within ~60 KB
+// lines per block: two buffers of TL*(N+1) within an LDS budget of 56 KB
+// (FP64) or 28 KB (FP32: more blocks per CU; at N = 96, 16 lines of 8-byte
+// elements per block make the pass faster than 32)
 template <typename T, int N>
 constexpr int fft_tl_c()
 {
-    return ((N + 1) * (int)sizeof(cpx<T>) * 2 * 32 <= 56 * 1024) ? 32
-           : ((N + 1) * (int)sizeof(cpx<T>) * 2 * 16 <= 56 * 1024) ? 16
+    constexpr int budget = sizeof(T) == 4 ? 28 * 1024 : 56 * 1024;
+    return ((N + 1) * (int)sizeof(cpx<T>) * 2 * 32 <= budget) ? 32
+           : ((N + 1) * (int)sizeof(cpx<T>) * 2 * 16 <= budget) ? 16
            : 8;
 }
 

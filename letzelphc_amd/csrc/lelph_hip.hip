@@ -654,7 +654,7 @@ static int run_local_split_t(lelph_ctx* c, const double* qpt, const lelph_pair& 
     const int64_t nBp = presplit_elems(nspin, L.nspinor, nchunks, NP, nbp);
     LELPH_TRY(c->presplit.ensure(sizeof(__bf16) * (size_t)nBp));
     hipLaunchKernelGGL((k_presplit_B<NP>),
-                       dim3((unsigned)nchunks, (unsigned)cdiv((int64_t)nbp * 4, 256),
+                       dim3((unsigned)nchunks, (unsigned)(nbp / PS_ROWS),
                             (unsigned)(nspin * L.nspinor)),
                        dim3(256), 0, c->stream, c->box_kq.as<c32>(), nb, nbp, L.nspinor, c->nf,
                        nchunks, c->presplit.as<__bf16>());

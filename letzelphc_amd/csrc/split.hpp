@@ -77,29 +77,42 @@ static inline int64_t presplit_elems(int nspin, int nspinor, int64_t nchunks, in
     return (int64_t)nspin * nspinor * nchunks * NP * 2 * nbp * SP_KC;
 }
 
-// psi_k+q box (nspin, nb, nspinor, N) FP32 -> Bp pieces.  Thread t of
-// chunk c (grid x) and (spin, spinor) (grid z), t over grid y and the block:
-// row m = t / 4, k group kq = t % 4; it converts points c*32 + 8 kq .. +7 of
-// row m and writes one 16-byte unit per piece plane.  Rows m >= nb and
-// points >= N are zero.
+// psi_k+q box (nspin, nb, nspinor, N) FP32 -> Bp pieces.  Block (chunk c,
+// rows 64 y .. 64 y + 63, spin * nspinor + spinor): the chunk's 32 points of
+// the 64 rows are read row by row (32 threads per 256-byte row segment) into
+// LDS, rows 33 elements apart so the column reads below do not share banks;
+// then thread t takes row m = t % 64 and k group kq = t / 64, splits points
+// 8 kq .. 8 kq + 7 and writes one 16-byte unit per piece plane, consecutive
+// threads on consecutive rows (1 KB contiguous per wave and plane).  Rows
+// m >= nb and points >= N are zero.
+#define PS_ROWS 64
 template <int NP>
-__global__ void k_presplit_B(const c32* __restrict__ box, int nb, int nbp, int nspinor,
-                             int64_t N, int64_t nchunks, __bf16* __restrict__ Bp)
+__global__ __launch_bounds__(256) void k_presplit_B(const c32* __restrict__ box, int nb, int nbp,
+                                                    int nspinor, int64_t N, int64_t nchunks,
+                                                    __bf16* __restrict__ Bp)
 {
-    const int t = blockIdx.y * blockDim.x + threadIdx.x;
-    if (t >= nbp * 4) return;
-    const int m = t >> 2, kq = t & 3;
+    __shared__ c32 tile[PS_ROWS * (SP_KC + 1)];
+    const int tid = threadIdx.x;
     const int64_t c = blockIdx.x;
+    const int m0 = blockIdx.y * PS_ROWS;
     const int ss = blockIdx.z; // spin * nspinor + spinor
     const int is = ss / nspinor, sp = ss % nspinor;
-    const int64_t r0 = c * SP_KC + kq * 8;
+    const int64_t r0 = c * SP_KC;
+    for (int f = tid; f < PS_ROWS * SP_KC; f += 256)
+    {
+        const int m = f / SP_KC, j = f % SP_KC;
+        c32 v = {0, 0};
+        if (m0 + m < nb && r0 + j < N)
+            v = box[((int64_t)(is * nb + m0 + m) * nspinor + sp) * N + r0 + j];
+        tile[m * (SP_KC + 1) + j] = v;
+    }
+    __syncthreads();
+    const int m = tid % PS_ROWS, kq = tid / PS_ROWS;
     bf16x8 pr[NP], pi[NP];
-    const c32* row = box + ((int64_t)(is * nb + m) * nspinor + sp) * N;
 #pragma unroll
     for (int j = 0; j < 8; ++j)
     {
-        c32 v = {0, 0};
-        if (m < nb && r0 + j < N) v = row[r0 + j];
+        const c32 v = tile[m * (SP_KC + 1) + kq * 8 + j];
         __bf16 a[NP], b[NP];
         split_bf16<NP>(v.x, a);
         split_bf16<NP>(v.y, b);
@@ -112,7 +125,7 @@ __global__ void k_presplit_B(const c32* __restrict__ box, int nb, int nbp, int n
     }
     const int64_t plane = (int64_t)nbp * SP_KC; // bf16 per plane
     __bf16* base = Bp + ((int64_t)ss * nchunks + c) * NP * 2 * plane + (int64_t)kq * nbp * 8 +
-                   (int64_t)m * 8;
+                   (int64_t)(m0 + m) * 8;
 #pragma unroll
     for (int p = 0; p < NP; ++p)
     {
