@@ -142,8 +142,9 @@ static inline size_t plane_lds_bytes(int ny, int nz, bool fuse_y = false)
 // the plane.  Empty planes are skipped (never written).  NY > 0 (= ny): the
 // y pass too, in LDS: the z-transformed plane transposed to [z][y], FFT of
 // its NZ lines of length NY, written back in [y][z] order, so the plane
-// leaves the kernel transformed along y and z.
-#define PZ_THREADS 512
+// leaves the kernel transformed along y and z.  A plane's buffers take most
+// of the LDS (one block per CU at 96 x 96), so the block is 16 waves.
+#define PZ_THREADS 1024
 template <int NZ, int NY = 0>
 __global__ __launch_bounds__(PZ_THREADS) void k_plane_z(
     const z64* __restrict__ wfc, int nspinor, int64_t npw, const int* __restrict__ idx,
