@@ -492,12 +492,15 @@ int lelph_write_dyn_qe(const char* dyn_file, int natom, const double qpt[3], con
 int lelph_interp_timings(lelph_ctx* ctx, double* ms, int n);
 
 /* Device time of the last elph_local / nonlocal / gkq / batch call, summed
- * over its pairs (HIP events on the context stream), milliseconds:
+ * over its pairs (HIP events), milliseconds:
  *   [0] sphere scatter + backward FFTs of psi_k and psi_k+q
  *   [1] local contraction kernel (the MFMA GEMM)
  *   [2] split-K reduction and 1/N
- *   [3] non-local part
- *   [4] total device time
+ *   [3] non-local part; in a batch, where it runs on a second stream beside
+ *       the local parts, the time from the end of the pair's local part to
+ *       the end of its non-local part
+ *   [4] total device time per pair (start of its local part to the end of
+ *       both parts; overlapping pairs of a batch count their overlap twice)
  *   [5] number of (k,k+q) pairs
  * n: number of entries wanted (<= 8). */
 int lelph_last_timings(lelph_ctx* ctx, double* ms, int n);

@@ -187,6 +187,7 @@ struct lelph_ctx
     // copy-out of a batch: each pair's g leaves on its own stream into a
     // pinned ring while later pairs compute
     hipStream_t out_stream = nullptr;
+    hipStream_t nl_stream = nullptr; // batch: non-local parts beside the local ones
     void* out_ring = nullptr;
     size_t out_ring_bytes = 0;
     hipEvent_t out_ev[LELPH_OUT_SLOTS] = {};
@@ -546,7 +547,11 @@ static int run_local(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64*
 }
 
 // non-local part added onto g (device): add_elphNonLocal
-static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g)
+// non-local part of one pair on stream st; its last kernel adds into g and
+// waits for mode_after first when that is given (the local part of the same
+// pair, which writes g, runs on the context stream)
+static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t st,
+                        hipEvent_t mode_after)
 {
     const lelph_lattice& L = c->lat;
     if (!c->dv_set) return LELPH_ERR_STATE;
@@ -586,19 +591,19 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g)
     LELPH_TRY(su2mat(SK.Rmat, L.nspinor, SK.time_rev, sK.su2));
     LELPH_TRY(su2mat(SKp.Rmat, L.nspinor, SKp.time_rev, sKp.su2));
 
-    hipLaunchKernelGGL(k_nl_geom, dim3((unsigned)cdiv(wK.npw, 128)), dim3(128), 0, c->stream,
+    hipLaunchKernelGGL(k_nl_geom, dim3((unsigned)cdiv(wK.npw, 128)), dim3(128), 0, st,
                        wK.gvec, wK.npw, sK, c->lmax, wK.wfc, nsb, L.nspinor, c->nlK.as<double>(),
                        c->Y_K.as<double>(), c->wr_K.as<z64>());
-    hipLaunchKernelGGL(k_nl_geom, dim3((unsigned)cdiv(wKp.npw, 128)), dim3(128), 0, c->stream,
+    hipLaunchKernelGGL(k_nl_geom, dim3((unsigned)cdiv(wKp.npw, 128)), dim3(128), 0, st,
                        wKp.gvec, wKp.npw, sKp, c->lmax, wKp.wfc, nsb, L.nspinor,
                        c->nlKp.as<double>(), c->Y_Kp.as<double>(), c->wr_Kp.as<z64>());
     NlProj pj{natom, c->pmax, c->ntype};
     hipLaunchKernelGGL(k_nl_beta<double>, dim3((unsigned)cdiv(wK.npw, 256), natom * c->pmax),
-                       dim3(256), 0, c->stream, c->nlK.as<double>(), c->Y_K.as<double>(), wK.Fk,
+                       dim3(256), 0, st, c->nlK.as<double>(), c->Y_K.as<double>(), wK.Fk,
                        wK.npw, pj, c->d_plist, c->d_atype, c->d_apos, c->d_Fsign, -1,
                        c->beta_K.as<z64>());
     hipLaunchKernelGGL(k_nl_beta<double>, dim3((unsigned)cdiv(wKp.npw, 256), natom * c->pmax),
-                       dim3(256), 0, c->stream, c->nlKp.as<double>(), c->Y_Kp.as<double>(),
+                       dim3(256), 0, st, c->nlKp.as<double>(), c->Y_Kp.as<double>(),
                        wKp.Fk, wKp.npw, pj, c->d_plist, c->d_atype, c->d_apos, c->d_Fsign, +1,
                        c->beta_Kp.as<z64>());
     HIP_TRY(hipGetLastError());
@@ -615,23 +620,28 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g)
         int64_t cps;
         int nsplit;
         pick_splits(tiles, nchunks, (int64_t)rows * ncol, &cps, &nsplit);
-        LELPH_TRY(c->bbpart.ensure(sizeof(z64) * (size_t)nsplit * rows * ncol));
+        // the split count follows the side's npw: a larger partial buffer is
+        // allocated only once the previous pair's non-local part is done
+        const size_t need = sizeof(z64) * (size_t)nsplit * rows * ncol;
+        if (need > c->bbpart.n && st != c->stream) HIP_TRY(hipStreamSynchronize(st));
+        LELPH_TRY(c->bbpart.ensure(need));
         hipLaunchKernelGGL(k_zgemm_nt<double>, dim3((unsigned)tiles, (unsigned)nsplit), dim3(256),
-                           0, c->stream, A, B, rows, (int)ncol, w.npw, conjB, cps,
+                           0, st, A, B, rows, (int)ncol, w.npw, conjB, cps,
                            c->bbpart.as<z64>());
         hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)cdiv((int64_t)rows * ncol, 256)),
-                           dim3(256), 0, c->stream, c->bbpart.as<z64>(), nsplit,
+                           dim3(256), 0, st, c->bbpart.as<z64>(), nsplit,
                            (int64_t)rows * ncol, 1.0, out);
         HIP_TRY(hipGetLastError());
     }
     const int64_t ne = (int64_t)natom * 3 * es;
-    hipLaunchKernelGGL(k_nl_kkp, dim3((unsigned)cdiv(ne, 256)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_nl_kkp, dim3((unsigned)cdiv(ne, 256)), dim3(256), 0, st,
                        c->bb_K.as<z64>(), c->bb_Kp.as<z64>(), pj, c->d_plist, c->d_atype,
                        c->d_socfac, c->d_fco, c->has_fco ? c->d_fco_off : nullptr, L.nspin,
                        L.nbnds, L.nspinor, c->ebuf.as<z64>());
     const z64 pre = {0.0, -2 * LELPH_RPI * LELPH_E2};
+    if (mode_after) HIP_TRY(hipStreamWaitEvent(st, mode_after, 0));
     hipLaunchKernelGGL(k_nl_mode, dim3((unsigned)cdiv((int64_t)L.nmodes * es, 256)), dim3(256), 0,
-                       c->stream, c->ebuf.as<z64>(), c->eig_d.as<z64>(), L.nmodes, 3 * natom, es,
+                       st, c->ebuf.as<z64>(), c->eig_d.as<z64>(), L.nmodes, 3 * natom, es,
                        pre, g);
     HIP_TRY(hipGetLastError());
     return LELPH_OK;
@@ -838,6 +848,7 @@ static void free_ctx(lelph_ctx* c)
     for (auto& e : c->out_ev)
         if (e) (void)hipEventDestroy(e);
     if (c->out_stream) (void)hipStreamDestroy(c->out_stream);
+    if (c->nl_stream) (void)hipStreamDestroy(c->nl_stream);
     if (c->out_ring) (void)hipHostFree(c->out_ring);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1300,9 +1311,11 @@ static int ensure_events(lelph_ctx* c, int64_t npairs)
     return LELPH_OK;
 }
 
-// local (+ non-local) of one pair into device g; events ev[0..4]
+// local (+ non-local) of one pair into device g; events ev[0..4], ev[4]
+// marking the end of both parts.  overlap: the non-local part runs on
+// nl_stream, beside this pair's local part and the next pair's boxes.
 static int gkq_device(lelph_ctx* c, const double* qpt, const lelph_pair& pr, int local,
-                      int nonloc, z64* g, hipEvent_t* ev)
+                      int nonloc, z64* g, hipEvent_t* ev, bool overlap = false)
 {
     HIP_TRY(hipEventRecord(ev[0], c->stream));
     if (local)
@@ -1313,7 +1326,13 @@ static int gkq_device(lelph_ctx* c, const double* qpt, const lelph_pair& pr, int
         HIP_TRY(hipEventRecord(ev[2], c->stream));
     }
     HIP_TRY(hipEventRecord(ev[3], c->stream));
-    if (nonloc) LELPH_TRY(run_nonlocal(c, pr, g));
+    if (nonloc && overlap)
+    {
+        LELPH_TRY(run_nonlocal(c, pr, g, c->nl_stream, ev[3]));
+        HIP_TRY(hipEventRecord(ev[4], c->nl_stream));
+        return LELPH_OK;
+    }
+    if (nonloc) LELPH_TRY(run_nonlocal(c, pr, g, c->stream, nullptr));
     HIP_TRY(hipEventRecord(ev[4], c->stream));
     return LELPH_OK;
 }
@@ -1456,6 +1475,7 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     // into g_out while the GPU works on later pairs
     const size_t slice = sizeof(z64) * (size_t)gsz;
     LELPH_TRY(ensure_out_ring(c, slice));
+    if (!c->nl_stream) HIP_TRY(hipStreamCreateWithFlags(&c->nl_stream, hipStreamNonBlocking));
     z64* ring = static_cast<z64*>(c->out_ring);
     auto land = [&](int64_t m) -> int {
         const int slot = (int)(m % LELPH_OUT_SLOTS);
@@ -1471,7 +1491,7 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     {
         const int64_t i = order[n];
         st = gkq_device(c, qpt, pairs[i], 1, non_loc, c->gbatch.as<z64>() + i * gsz,
-                        c->evs.data() + 5 * i);
+                        c->evs.data() + 5 * i, true);
         if (st != LELPH_OK) break;
         if (n >= LELPH_OUT_SLOTS)
         {
@@ -1489,6 +1509,7 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     c->box_prev_valid = false;
     if (st != LELPH_OK)
     {
+        (void)hipStreamSynchronize(c->nl_stream);
         (void)hipStreamSynchronize(c->out_stream);
         (void)hipStreamSynchronize(c->stream);
         return st;
