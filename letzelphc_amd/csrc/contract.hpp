@@ -167,15 +167,23 @@ __global__ __launch_bounds__(256) void k_contract(const cpx<T>* __restrict__ psi
                 a[1] = sA[wv][16 + i][kq];
                 b[0] = sB[wm * 32 + i][kq];
                 b[1] = sB[wm * 32 + 16 + i][kq];
+                // C += a conj(b): re = ar br + ai bi, im = ai br - ar bi; the
+                // first products of all 8 accumulators go out before the
+                // second ones, so no MFMA waits on the one just issued
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
                     for (int cb = 0; cb < 2; ++cb)
                     {
-                        // C += a conj(b): re = ar br + ai bi, im = ai br - ar bi
                         accr[rb][cb] = Mfma<T>::op(a[rb].x, b[cb].x, accr[rb][cb]);
-                        accr[rb][cb] = Mfma<T>::op(a[rb].y, b[cb].y, accr[rb][cb]);
                         acci[rb][cb] = Mfma<T>::op(a[rb].y, b[cb].x, acci[rb][cb]);
+                    }
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                    {
+                        accr[rb][cb] = Mfma<T>::op(a[rb].y, b[cb].y, accr[rb][cb]);
                         acci[rb][cb] = Mfma<T>::op(-a[rb].x, b[cb].y, acci[rb][cb]);
                     }
             }
