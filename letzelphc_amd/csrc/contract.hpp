@@ -207,6 +207,154 @@ __global__ __launch_bounds__(256) void k_contract(const cpx<T>* __restrict__ psi
             }
 }
 
+// k_contract with dV and psi_k staged separately and their product formed
+// as the A fragments are read (A = dV psi_k in the same order as
+// load_dvpsi): the block's 2 modes share the psi_k rows and its 32 rows the
+// dV values, so a chunk moves 0.5 + 8 KB of A-side data instead of 32 KB
+// of element-wise loads.  Not for the 2x2 potential of spinors (nmag 4,
+// nspinor 2).  Used for FP32; in FP64 the products formed beside the FP64
+// MFMAs cost more than the loads saved.
+template <typename T>
+__global__ __launch_bounds__(256) void k_contract_sep(const cpx<T>* __restrict__ psik,
+                                                  const cpx<T>* __restrict__ psikq,
+                                                  const cpx<T>* __restrict__ dv,
+                                                  z64* __restrict__ part, int nm, int nb,
+                                                  int nspin, int nspinor, int nmag,
+                                                  int64_t N, int64_t chunks_per_split)
+{
+    typedef typename Mfma<T>::acc acc_t;
+    __shared__ cpx<T> sV[CT_VB][CT_LD]; // dV of the block's modes
+    __shared__ cpx<T> sP[CT_NB][CT_LD]; // psi_k of the block's rows
+    __shared__ cpx<T> sB[CT_MB][CT_LD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ntn = (nb + CT_NB - 1) / CT_NB, ntm = (nb + CT_MB - 1) / CT_MB;
+    int bx = blockIdx.x;
+    const int tm = bx % ntm;
+    bx /= ntm;
+    const int tn = bx % ntn;
+    bx /= ntn;
+    const int v0 = bx * CT_VB, n0 = tn * CT_NB, m0 = tm * CT_MB;
+    const int split = blockIdx.y, is = blockIdx.z;
+    const int64_t nchunks = (N + CT_KC - 1) / CT_KC;
+    const int64_t c0 = (int64_t)split * chunks_per_split;
+    const int64_t c1 = (c0 + chunks_per_split < nchunks) ? c0 + chunks_per_split : nchunks;
+    const int wv = wave >> 1, wm = wave & 1;
+
+    // FP32: the MFMA accumulators are added into FP64 every CT_FLUSH chunks,
+    // so no FP32 summation chain is longer than CT_FLUSH * CT_KC points.
+    constexpr bool flush = sizeof(T) == 4;
+    acc_t accr[2][2], acci[2][2];
+    double dr[2][2][4], di[2][2][4];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int v = 0; v < 4; ++v)
+            {
+                accr[a][b][v] = 0;
+                acci[a][b][v] = 0;
+                dr[a][b][v] = 0;
+                di[a][b][v] = 0;
+            }
+
+    int nfl = 0;
+    for (int sp = 0; sp < nspinor; ++sp)
+    {
+        for (int64_t c = c0; c < c1; ++c)
+        {
+            if (flush && ++nfl > CT_FLUSH)
+            {
+                nfl = 1;
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                        {
+                            dr[a][b][e] += (double)accr[a][b][e];
+                            di[a][b][e] += (double)acci[a][b][e];
+                            accr[a][b][e] = 0;
+                            acci[a][b][e] = 0;
+                        }
+            }
+            const int64_t r0 = c * CT_KC;
+            if (tid < CT_VB * CT_KC)
+            {
+                const int vv = tid / CT_KC, rr = tid - vv * CT_KC;
+                const int v = v0 + vv;
+                const int64_t r = r0 + rr;
+                cpx<T> x = {0, 0};
+                if (v < nm && r < N) x = dv[((int64_t)v * nmag + (nmag == 2 ? is : 0)) * N + r];
+                sV[vv][rr] = x;
+            }
+            for (int f = tid; f < CT_NB * CT_KC; f += 256)
+            {
+                const int nn = f / CT_KC, rr = f - nn * CT_KC;
+                const int n = n0 + nn;
+                const int64_t r = r0 + rr;
+                cpx<T> x = {0, 0};
+                if (n < nb && r < N) x = psik[((int64_t)(is * nb + n) * nspinor + sp) * N + r];
+                sP[nn][rr] = x;
+            }
+            for (int f = tid; f < CT_MB * CT_KC; f += 256)
+            {
+                const int mm = f / CT_KC, rr = f - mm * CT_KC;
+                const int m = m0 + mm;
+                const int64_t r = r0 + rr;
+                cpx<T> b = {0, 0};
+                if (m < nb && r < N) b = psikq[((int64_t)(is * nb + m) * nspinor + sp) * N + r];
+                sB[mm][rr] = b;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < CT_KC / 4; ++kk)
+            {
+                const int i = lane & 15, kq = kk * 4 + (lane >> 4);
+                cpx<T> a[2], b[2];
+                const cpx<T> vq = sV[wv][kq];
+                a[0] = cmul(vq, sP[i][kq]);
+                a[1] = cmul(vq, sP[16 + i][kq]);
+                b[0] = sB[wm * 32 + i][kq];
+                b[1] = sB[wm * 32 + 16 + i][kq];
+                // C += a conj(b): re = ar br + ai bi, im = ai br - ar bi; the
+                // first products of all 8 accumulators go out before the
+                // second ones, so no MFMA waits on the one just issued
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                    {
+                        accr[rb][cb] = Mfma<T>::op(a[rb].x, b[cb].x, accr[rb][cb]);
+                        acci[rb][cb] = Mfma<T>::op(a[rb].y, b[cb].x, acci[rb][cb]);
+                    }
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb)
+                    {
+                        accr[rb][cb] = Mfma<T>::op(a[rb].y, b[cb].y, accr[rb][cb]);
+                        acci[rb][cb] = Mfma<T>::op(-a[rb].x, b[cb].y, acci[rb][cb]);
+                    }
+            }
+            __syncthreads();
+        }
+    }
+    const int v = v0 + wv;
+    if (v >= nm) return;
+    for (int rb = 0; rb < 2; ++rb)
+        for (int cb = 0; cb < 2; ++cb)
+            for (int e = 0; e < 4; ++e)
+            {
+                const int row = n0 + rb * 16 + Mfma<T>::row(lane, e);
+                const int col = m0 + wm * 32 + cb * 16 + (lane & 15);
+                if (row < nb && col < nb)
+                {
+                    const int64_t o = ((((int64_t)split * nm + v) * nspin + is) * nb + row) * nb + col;
+                    part[o] = {dr[rb][cb][e] + (double)accr[rb][cb][e],
+                               di[rb][cb][e] + (double)acci[rb][cb][e]};
+                }
+            }
+}
+
 // g[i] = scale * sum_s part[s][i], fixed order.
 __global__ void k_reduce_parts(const z64* __restrict__ part, int nsplit, int64_t count,
                                double scale, z64* __restrict__ g)
