@@ -141,6 +141,13 @@ int lelph_create(const lelph_lattice* lattice, const lelph_pseudo* pseudo,
 void lelph_destroy(lelph_ctx* ctx);
 const char* lelph_strerror(int status);
 int lelph_set_precision(lelph_ctx* ctx, int precision);
+/* The precision requested with lelph_create / lelph_set_precision and the
+ * one the contractions run in on this context's shape: LELPH_BF16X3 runs as
+ * LELPH_BF16X6 unless nspinor = 1, Nx*Ny*Nz is a multiple of 32 and
+ * nbnds >= 64 (the shapes whose bf16x3 error is held to 1e-6).  No
+ * counterpart in the reference (it has one precision per build,
+ * src/elphC.h:16-32).  Either pointer may be NULL. */
+int lelph_get_precision(lelph_ctx* ctx, int* requested, int* effective);
 int lelph_get_device_count(int* count);
 
 /* iBZ wavefunction ik_ibz: wfc (nspin,nbnds,nspinor,npw) complex, gvec

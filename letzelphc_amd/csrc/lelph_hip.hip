@@ -743,10 +743,22 @@ static int run_local_split_t(lelph_ctx* c, const double* qpt, const lelph_pair& 
     return LELPH_OK;
 }
 
+// The precision a context's contractions actually run in.  bf16x3 (two
+// bf16 pieces, 16 mantissa bits per operand) holds |dg|/|g|max <= 1e-6 only
+// with the producer/consumer kernel's 512-point FP32 chains on shapes where
+// the error is measured (one spinor, N a multiple of 32, nb >= 64); every
+// other shape runs the 3-piece bf16x6 form instead (2^-25 per product).
+static int effective_precision(const lelph_ctx* c)
+{
+    if (c->precision != LELPH_BF16X3) return c->precision;
+    const bool pc = c->lat.nspinor == 1 && c->nf % SP_KC == 0 && c->lat.nbnds >= 64;
+    return pc ? LELPH_BF16X3 : LELPH_BF16X6;
+}
+
 static int run_local_split(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g,
                            hipEvent_t* ev)
 {
-    if (c->precision == LELPH_BF16X3) return run_local_split_t<2, 3>(c, qpt, pr, g, ev);
+    if (effective_precision(c) == LELPH_BF16X3) return run_local_split_t<2, 3>(c, qpt, pr, g, ev);
     return run_local_split_t<3, 6>(c, qpt, pr, g, ev);
 }
 
@@ -1015,6 +1027,14 @@ int lelph_set_precision(lelph_ctx* c, int p)
 {
     if (!c || p < LELPH_FP64 || p > LELPH_BF16X6) return LELPH_ERR_ARG;
     c->precision = p;
+    return LELPH_OK;
+}
+
+int lelph_get_precision(lelph_ctx* c, int* requested, int* effective)
+{
+    if (!c) return LELPH_ERR_ARG;
+    if (requested) *requested = c->precision;
+    if (effective) *effective = effective_precision(c);
     return LELPH_OK;
 }
 

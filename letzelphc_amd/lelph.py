@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(HERE, "liblelph_hip.so")
 
 EXPORTS = [
     "lelph_create", "lelph_destroy", "lelph_strerror", "lelph_set_precision",
+    "lelph_get_precision",
     "lelph_get_device_count", "lelph_upload_wfc", "lelph_create_vlocg_table",
     "lelph_dvlocq", "lelph_set_dvscf", "lelph_get_dvscf", "lelph_elph_local",
     "lelph_add_elph_nonlocal", "lelph_gkq", "lelph_gkq_batch", "lelph_compute_elphq",
@@ -66,6 +67,7 @@ def lib():
         L.lelph_destroy.argtypes = [C.c_void_p]
         L.lelph_destroy.restype = None
         L.lelph_set_precision.argtypes = [C.c_void_p, C.c_int]
+        L.lelph_get_precision.argtypes = [C.c_void_p, c_int_p, c_int_p]
         L.lelph_get_device_count.argtypes = [C.POINTER(C.c_int)]
         L.lelph_upload_wfc.argtypes = [C.c_void_p, C.c_int64, c_double_p, c_double_p,
                                        c_double_p, C.c_int64]
@@ -159,7 +161,8 @@ def device_count() -> int:
 
 class Context:
     """One device context for a synth.System (or any object with the same
-    fields).  Uploads every iBZ wavefunction at construction."""
+    fields).  Uploads every iBZ wavefunction the system holds (entries left
+    None are skipped) at construction."""
 
     def __init__(self, system, precision: str = "fp64", upload: bool = True):
         self.sys = system
@@ -171,7 +174,8 @@ class Context:
         self.precision = precision
         if upload:
             for ik, w in enumerate(system.wfcs):
-                self.upload_wfc(ik, w.wfc, w.gvec, w.Fk)
+                if w is not None:
+                    self.upload_wfc(ik, w.wfc, w.gvec, w.Fk)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -187,6 +191,14 @@ class Context:
     def set_precision(self, p: str):
         _check(lib().lelph_set_precision(self.h, PRECISIONS[p]), "lelph_set_precision")
         self.precision = p
+
+    def effective_precision(self) -> str:
+        """The precision the contractions run in on this shape (bf16x3 runs
+        as bf16x6 off the aligned producer/consumer path)."""
+        req, eff = C.c_int(0), C.c_int(0)
+        _check(lib().lelph_get_precision(self.h, C.byref(req), C.byref(eff)),
+               "lelph_get_precision")
+        return {v: k for k, v in PRECISIONS.items()}[eff.value]
 
     def upload_wfc(self, ik, wfc, gvec, Fk=None):
         wfc = np.ascontiguousarray(wfc, np.complex128)

@@ -357,18 +357,23 @@ def _make_wfc(sys: System, ik: int) -> Wfc:
     return Wfc(np.ascontiguousarray(c), gcart, np.ascontiguousarray(Fk), g)
 
 
-def make_wfcs(sys: System, workers: int = 0):
-    """All iBZ wavefunctions; independent per k, generated on a thread pool
+def make_wfcs(sys: System, workers: int = 0, only=None):
+    """The iBZ wavefunctions; independent per k, generated on a thread pool
     (every k has its own Philox stream, so the result does not depend on the
-    number of workers)."""
+    number of workers).  `only`: generate just these iBZ k, the other
+    entries stay None (a pair list that touches a few k of a large cell)."""
     from concurrent.futures import ThreadPoolExecutor
     nk = len(sys.kpt_ibz_crys)
+    ks = list(range(nk)) if only is None else sorted(set(int(k) for k in only))
     workers = workers or 1
-    if workers <= 1 or nk <= 1:
-        wf = [_make_wfc(sys, ik) for ik in range(nk)]
+    if workers <= 1 or len(ks) <= 1:
+        made = [_make_wfc(sys, ik) for ik in ks]
     else:
         with ThreadPoolExecutor(workers) as ex:
-            wf = list(ex.map(lambda ik: _make_wfc(sys, ik), range(nk)))
+            made = list(ex.map(lambda ik: _make_wfc(sys, ik), ks))
+    wf = [None] * nk
+    for ik, w in zip(ks, made):
+        wf[ik] = w
     sys.wfcs = wf
     return wf
 

@@ -159,6 +159,8 @@ class OracleSystem:
         self.packed = pack_system(sys)
         self.wtab = (OrcWfc * len(sys.wfcs))()
         for i, w in enumerate(sys.wfcs):
+            if w is None:  # a k the caller did not generate: never touched
+                continue
             self.wtab[i].wfc = dptr(w.wfc)
             self.wtab[i].gvec = dptr(w.gvec)
             self.wtab[i].Fk = dptr(w.Fk)

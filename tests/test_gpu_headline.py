@@ -52,6 +52,7 @@ def test_headline_bf16x3(case, kminusq):
         ctx.set_dvscf(s.qpts[0], s.eigvec(0), s.dvscf(0), kernel="dfpt")
         g64 = ctx.compute_elphq(s.qpts[0], non_loc=True, kminusq=kminusq)
         ctx.set_precision("bf16x3")
+        assert ctx.effective_precision() == "bf16x3"
         g3 = ctx.compute_elphq(s.qpts[0], non_loc=True, kminusq=kminusq)
     finally:
         ctx.close()
