@@ -513,10 +513,11 @@ def config_c3(with_wfc=True, cells=None):
 
 def tiny(nspin=1, nspinor=1, nmag=1, nbnds=4, fft=(12, 12, 12), kgrid=(2, 2, 2),
          syms="trev", ecut=6.0, seed=7, q=(0.0, 0.0, 0.5), with_wfc=True,
-         dimension="3", soc=False) -> System:
+         dimension="3", soc=False, crys_pos=None) -> System:
     """A small fcc cell for branch coverage (spin, spinors, symmetries);
     dimension "2": a hexagonal slab cell (a = 5, c = 20 bohr, 12x12x48)
-    with the 2-D Coulomb cut-off."""
+    with the 2-D Coulomb cut-off.  crys_pos replaces the two atoms'
+    crystal positions of the cubic cell."""
     a = 8.0
     I3 = np.eye(3)
     if dimension == "2":
@@ -537,7 +538,8 @@ def tiny(nspin=1, nspinor=1, nmag=1, nbnds=4, fft=(12, 12, 12), kgrid=(2, 2, 2),
               (r @ r @ r, np.zeros(3), 0), (-I3, np.zeros(3), 1), (-r, np.zeros(3), 1),
               (-(r @ r), np.zeros(3), 1), (-(r @ r @ r), np.zeros(3), 1)]
         tr = 1
-        return make_system("tiny", seed, a * I3, [[0, 0, 0], [0.5, 0.5, 0.3]], [0, 1],
+        pos = [[0, 0, 0], [0.5, 0.5, 0.3]] if crys_pos is None else crys_pos
+        return make_system("tiny", seed, a * I3, pos, [0, 1],
                            [3.0, 5.0], fft, kgrid, ecut, nbnds, sl, tr, [q],
                            nspin=nspin, nspinor=nspinor, nmag=nmag, with_wfc=with_wfc, soc=soc)
     else:

@@ -13,6 +13,12 @@ from the reference's code path:
   * the non-local operator is assembled as a dense (K, K') matrix with the
     Legendre addition theorem sum_m Y_lm(a) Y_lm(b) = (2l+1)/(4pi) P_l(a.b),
     which holds for any real-harmonic convention.
+
+pi: the reference writes every 2 pi phase, the Y_lm normalisation and the
+non-local prefactor with ELPH_PI = 3.1415927 (src/common/constants.h:11,
+src/common/numerical_func.c:120,148, src/nonloc/Vbare_nloc.c:356,387,547),
+which moves g at the 1e-8 level; the non-local formulations here take the
+same constant (synth.RPI) so that the comparison holds to rounding.
 """
 from __future__ import annotations
 
@@ -77,14 +83,18 @@ def elph_local(s, kbz_i, q, dv):
     return g
 
 
-def vnl_matrix(s, ikq, ik, qshift_tau=None):
+def vnl_matrix(s, ikq, ik, qshift_tau=None, pi=synth.RPI, kqsym=0, ksym=0):
     """Per atom: K (npwK,3), K' (npwKp,3) cartesian without 2pi, and the
     dense V_a(K, K') = sum_l sign(D_l) F_l(K) F_l(K') (2l+1)/(4pi)
-    P_l(K^.K'^) exp(-i 2pi (K - K').tau_a), identity symmetry."""
+    P_l(K^.K'^) exp(-i 2pi (K - K').tau_a).  The plane waves of the rotated
+    state psi_Sk sit at K = S (k + G) (S the cartesian matrix, -R for a
+    time-reversed operation, whose coefficients are conjugated)."""
     ps = s.pseudo
     wK, wKp = s.wfcs[ikq], s.wfcs[ik]
-    K = s.kpt_ibz_cart[ikq][None, :] + wK.gvec
-    Kp = s.kpt_ibz_cart[ik][None, :] + wKp.gvec
+    RK = np.asarray(s.syms[kqsym][0], np.float64)
+    RKp = np.asarray(s.syms[ksym][0], np.float64)
+    K = (s.kpt_ibz_cart[ikq][None, :] + wK.gvec) @ RK.T
+    Kp = (s.kpt_ibz_cart[ik][None, :] + wKp.gvec) @ RKp.T
 
     def unit(v):
         n = np.linalg.norm(v, axis=1, keepdims=True)
@@ -99,23 +109,27 @@ def vnl_matrix(s, ikq, ik, qshift_tau=None):
         for li in range(ps.nltimesj):
             l = int(round(ps.PP_table[li, t, 0] - 1))
             sg = ps.Fsign[li, t]
-            V += (sg * (2 * l + 1) / (4 * np.pi)) * np.outer(wK.Fk[li, t], wKp.Fk[li, t]) \
+            V += (sg * (2 * l + 1) / (4 * pi)) * np.outer(wK.Fk[li, t], wKp.Fk[li, t]) \
                 * eval_legendre(l, np.clip(cosg, -1, 1))
-        ph = np.exp(-2j * np.pi * (K @ pos[ia]))[:, None] * np.exp(2j * np.pi * (Kp @ pos[ia]))[None, :]
+        ph = np.exp(-2j * pi * (K @ pos[ia]))[:, None] * np.exp(2j * pi * (Kp @ pos[ia]))[None, :]
         out.append(V * ph)
     return K, Kp, out
 
 
-def elph_nonlocal(s, ikq, ik, eig, pos=None):
+def elph_nonlocal(s, ikq, ik, eig, pos=None, pi=synth.RPI, kqsym=0, ksym=0):
     """g_NL[nu, spin, n_k, m_kq] = e2 * d/dtau <psi_kq,m| V_NL |psi_k,n> along
     the mode pattern, via the analytic derivative -i 2pi (K - K')."""
-    K, Kp, Vs = vnl_matrix(s, ikq, ik, pos)
+    K, Kp, Vs = vnl_matrix(s, ikq, ik, pos, pi, kqsym, ksym)
     ck = s.wfcs[ik].wfc[:, :, 0, :]
     ckq = s.wfcs[ikq].wfc[:, :, 0, :]
+    if s.syms[ksym][2]:
+        ck = np.conj(ck)
+    if s.syms[kqsym][2]:
+        ckq = np.conj(ckq)
     g = np.zeros((s.nmodes, s.nspin, s.nbnds, s.nbnds), np.complex128)
     for ia, V in enumerate(Vs):
         for x in range(3):
-            D = (-2j * np.pi) * (K[:, x][:, None] - Kp[:, x][None, :]) * V
+            D = (-2j * pi) * (K[:, x][:, None] - Kp[:, x][None, :]) * V
             for sp in range(s.nspin):
                 M = ckq[sp].conj() @ D @ ck[sp].T  # (m, n)
                 g[:, sp] += synth.E2 * eig[:, 3 * ia + x][:, None, None] * M.T[None]
@@ -223,3 +237,48 @@ def dmat(s, R, trR, ikbz):
     for sp in range(s.nspin):
         D[sp] = np.conj(cb[sp][:, ib]) @ ca[sp][:, ia].T
     return D
+
+
+def dvlocq_direct(s, table, spline, q, eig, pi=synth.RPI):
+    """(nm, Nx, Ny, Nz) bare local dV_q as a direct sum on the box, written
+    from the physics of the displaced local potential:
+    coefficient(q+G) = sum_{a,x} eig[nu, 3a+x] (-i)(q+G)_x exp(-i(q+G).tau_a)
+    V_a(|q+G|), V_a = the tabulated V_loc(|q+G|) minus the Gaussian-screened
+    Coulomb tail 4 pi e2 Z exp(-|q+G|^2/4) c(q+G) / (|q+G|^2 Omega) for
+    |q+G| >= 1e-5 (c the slab cut-off factor in 2-D, 1 in 3-D), then the
+    unnormalised backward transform (numpy's FFT).  `table` is (g_co,
+    vlocg, vploc) and `spline(x, i, g_co, y, dy)` evaluates it at x in
+    interval i; both are the reference-pinned pieces of the oracle."""
+    g_co, vlocg, vploc = table
+    dg = g_co[1] - g_co[0]
+    dims = s.fft_dims
+    mi = [synth.miller_axis(n) for n in dims]
+    gx, gy, gz = np.meshgrid(mi[0] + q[0], mi[1] + q[1], mi[2] + q[2], indexing="ij")
+    qg = np.stack([gx, gy, gz], -1) @ s.blat.T
+    qn = np.sqrt(np.sum(qg * qg, -1))
+    cf = np.ones(qn.shape)
+    if s.dimension == "2":
+        c = s.alat[2, 2]
+        gp = np.sqrt(qg[..., 0] ** 2 + qg[..., 1] ** 2)
+        tz = qg[..., 2] * c * 0.5
+        with np.errstate(divide="ignore", invalid="ignore"):
+            cs = np.where(gp > 1e-5, np.cos(tz) - np.sin(tz) * qg[..., 2] / gp, np.cos(tz))
+        cf = 1 - np.exp(-c * gp * 0.5) * cs
+    ps = s.pseudo
+    V = np.zeros((ps.ntype,) + qn.shape)
+    flat = qn.reshape(-1)
+    for t in range(ps.ntype):
+        vals = np.array([spline(float(x), int(np.floor(x / dg)), g_co, vlocg[t], vploc[t])
+                         for x in flat]).reshape(qn.shape)
+        Z = ps.loc[t][3]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            tail = np.where(qn >= 1e-5, 4 * pi * synth.E2 * Z * np.exp(-qn * qn * 0.25) * cf
+                            / (qn * qn * s.volume), 0.0)
+        V[t] = vals - tail
+    coef = np.zeros((s.nmodes,) + tuple(dims), np.complex128)
+    for ia in range(s.natom):
+        f = (-1j) * np.exp(-1j * (qg @ s.atomic_pos[ia])) * V[int(s.atom_type[ia])]
+        for x in range(3):
+            coef += eig[:, 3 * ia + x][:, None, None, None] * (f * qg[..., x])[None]
+    N = int(np.prod(dims))
+    return np.fft.ifftn(coef, axes=(1, 2, 3)) * N

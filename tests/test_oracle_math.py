@@ -197,6 +197,32 @@ def test_dvlocq_closed_form_slab():
     assert rel_err(O.OracleSystem(s3).dvlocq(0), ref) > 1e-4  # the cut-off is resolved
 
 
+@pytest.mark.parametrize("dim", ["3", "2"])
+def test_dvlocq_direct_sum(dim):
+    """The assembly of dVlocq (long-range add-back with its cut-off, atom
+    phases, cartesian derivative, mode contraction, box placement and
+    backward transform) against a direct sum over the box that evaluates
+    the reference-pinned V_loc table at |q+G|, at 1e-12; q = 0 covers the
+    |q+G| = 0 branch.  The tail written with the exact pi misses by more
+    than the bar, so the check resolves the truncated constant."""
+    s = synth.tiny(nbnds=2) if dim == "3" else synth.tiny(nbnds=2, dimension="2")
+    o = O.OracleSystem(s)
+    tab = tuple(np.ascontiguousarray(a) for a in o.vloc_table())
+
+    def spl(x, i, gc, y, dy):
+        return O.lib().orc_spline_interpolate(x, i, dptr(np.ascontiguousarray(gc)),
+                                              dptr(np.ascontiguousarray(y)),
+                                              dptr(np.ascontiguousarray(dy)))
+    eig = s.eigvec(0)
+    say(f"case: oracle dVlocq vs direct sum, {dim}-D")
+    for q in (s.qpts[0], np.zeros(3), np.array([0.25, -0.5, 0.125])):
+        got = o.dvlocq(0, qpt=q)
+        ref = BF.dvlocq_direct(s, tab, spl, q, eig)
+        assert rel_err(got, ref) < 1e-12, q
+    q = np.array([0.25, -0.5, 0.125])
+    assert rel_err(o.dvlocq(0, qpt=q), BF.dvlocq_direct(s, tab, spl, q, eig, pi=np.pi)) > 1e-11
+
+
 def test_dvlocq_translation_covariance():
     """Atoms moved by a grid vector t: dV'_q(x) = exp(-i q.t) dV_q(x - t)."""
     s = synth.tiny(nbnds=2)
@@ -216,19 +242,33 @@ def test_dvlocq_translation_covariance():
 
 
 # ---------------------------------------------------------------- non-local
-def test_nonlocal_vs_dense_operator():
+@pytest.mark.parametrize("syms", ["id", "cubic"])
+def test_nonlocal_vs_dense_operator(syms):
     """add_elphNonLocal against the dense V_NL(K, K') assembled with the
-    Legendre addition theorem and differentiated analytically."""
-    s = synth.tiny(nbnds=3, syms="id", kgrid=(2, 1, 2))
+    Legendre addition theorem and differentiated analytically; with the
+    cubic group the pairs take rotated and time-reversed sides."""
+    if syms == "id":
+        s = synth.tiny(nbnds=3, syms="id", kgrid=(2, 1, 2))
+    else:
+        s = synth.tiny(nbnds=3, syms="cubic", q=(0.0, 0.5, 0.0))
     o = O.OracleSystem(s)
     eig = s.eigvec(0)
-    say("case: oracle non-local vs dense operator")
-    for pr in synth.kq_pairs(s, 0):
+    say(f"case: oracle non-local vs dense operator, {syms}")
+    prs = list(synth.kq_pairs(s, 0))
+    if syms == "cubic":
+        # every operation on each side of one pair, time-reversed ones too
+        ikq, ik = prs[1][0], prs[1][1]
+        prs += [(ikq, ik, a, b) for a in range(len(s.syms)) for b in (0, 5)]
+    for pr in prs:
         ikq, ik = pr[0], pr[1]
         z = np.zeros((s.nmodes, s.nspin, s.nbnds, s.nbnds), np.complex128)
         g = o.add_nonlocal(0, *pr, z)
-        ref = BF.elph_nonlocal(s, ikq, ik, eig)
-        assert rel_err(g, ref) < 1e-6, pr  # ELPH_PI in the projector phases
+        ref = BF.elph_nonlocal(s, ikq, ik, eig, kqsym=pr[2], ksym=pr[3])
+        assert rel_err(g, ref) < 1e-11, pr
+        # with the exact pi the difference is the truncated constant's
+        # 1e-8-level effect, which this check resolves
+        assert rel_err(g, BF.elph_nonlocal(s, ikq, ik, eig, pi=np.pi, kqsym=pr[2],
+                                           ksym=pr[3])) > 1e-9, pr
 
 
 def test_nonlocal_finite_difference():
