@@ -41,7 +41,8 @@ enum lelph_status
     LELPH_ERR_UNSUPPORTED = 5, /* combination not supported by this build    */
     LELPH_ERR_STATE = 6,       /* call order violated (e.g. no dV set)       */
     LELPH_ERR_NOKQ = 7,        /* k+q not on the k grid (incommensurate)     */
-    LELPH_ERR_SYM = 8          /* symmetry matrix is not orthogonal          */
+    LELPH_ERR_SYM = 8,         /* symmetry matrix is not orthogonal          */
+    LELPH_ERR_SCRATCH = 9      /* size exceeds a no-allocation scratch       */
 };
 
 /* ---------------- precision of the contractions ---------------- */
@@ -232,6 +233,21 @@ int lelph_electronic_reps(lelph_ctx* ctx, const double Rsym[9], const double tau
 int lelph_elph_q_rotate_gpu(lelph_ctx* ctx, const double* Dmats_l, const double* elph_mat_q,
                             const double* Dmats_r, int nmodes, int nspin, int nb, int tim_rev,
                             double* elph_mat_Sq);
+
+/* No counterpart in the reference: lelph_elph_q_rotate_gpu for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops).  All pointers are DEVICE pointers, laid out as in
+ * lelph_elph_q_rotate_gpu; the kernels run on `stream` (a hipStream_t, NULL
+ * for the default stream) and the call returns without waiting.  The
+ * intermediate product lives in a static per-device array of
+ * LELPH_NOALLOC_MAX_ELEMS complex128 (64 MiB), held until the process ends:
+ * nmodes * nspin * nb^2 above that returns LELPH_ERR_SCRATCH.  No hipMalloc,
+ * hipFree or allocating library call is made.  Calls that overlap on one
+ * GPU share that array and race; order them on one stream. */
+#define LELPH_NOALLOC_MAX_ELEMS (1 << 22)
+int lelph_elph_q_rotate_noalloc(const double* d_Dmats_l, const double* d_elph_mat_q,
+                                const double* d_Dmats_r, int nmodes, int nspin, int nb,
+                                int tim_rev, double* d_elph_mat_Sq, void* stream);
 
 /* get_KplusQ_idxs (src/common/numerical_func.c:471-524). */
 int lelph_get_kplusq_idxs(int64_t nbz, const double* kpoints_crys,

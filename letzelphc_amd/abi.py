@@ -19,7 +19,7 @@ LELPH_OK = 0
 STATUS_NAMES = {
     0: "LELPH_OK", 1: "LELPH_ERR_ARG", 2: "LELPH_ERR_ALLOC", 3: "LELPH_ERR_HIP",
     4: "LELPH_ERR_RANGE", 5: "LELPH_ERR_UNSUPPORTED", 6: "LELPH_ERR_STATE",
-    7: "LELPH_ERR_NOKQ", 8: "LELPH_ERR_SYM",
+    7: "LELPH_ERR_NOKQ", 8: "LELPH_ERR_SYM", 9: "LELPH_ERR_SCRATCH",
 }
 
 PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16x6": 3}

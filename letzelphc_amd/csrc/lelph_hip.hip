@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -821,6 +822,9 @@ const char* lelph_strerror(int s)
             return "K+Q point cannot be found in kgrid. The k-grid and q-grid are not "
                    "commensurate";
         case LELPH_ERR_SYM: return "Wrong symmetric matrices";
+        case LELPH_ERR_SCRATCH:
+            return "size exceeds the static scratch of the no-allocation entry point "
+                   "(nmodes * nspin * nbnds^2 <= LELPH_NOALLOC_MAX_ELEMS)";
         default: return "unknown status";
     }
 }
@@ -889,6 +893,8 @@ static int upload(X** d, const X* h, size_t n)
 }
 }
 
+static int scratch_ptr(double** out);
+
 static int create_impl(const lelph_lattice* L, const lelph_pseudo* P, int precision,
                        lelph_ctx* c)
 {
@@ -899,6 +905,10 @@ static int create_impl(const lelph_lattice* L, const lelph_pseudo* P, int precis
     if (precision < LELPH_FP64 || precision > LELPH_BF16X6) return LELPH_ERR_ARG;
     c->precision = precision;
     HIP_TRY(hipGetDevice(&c->device));
+    {
+        double* sp = nullptr;
+        LELPH_TRY(scratch_ptr(&sp));
+    }
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
     c->lat = *L;
@@ -1723,6 +1733,79 @@ int lelph_elph_q_rotate_gpu(lelph_ctx* c, const double* Dmats_l, const double* e
     HIP_TRY(hipMemcpyAsync(elph_mat_Sq, c->qr_out.p, sizeof(z64) * gsz, hipMemcpyDeviceToHost,
                            c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return LELPH_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* q-star rotation without allocation                                       */
+/* ------------------------------------------------------------------------ */
+
+// The intermediate conj(D_r) op(g) of the no-allocation q-star rotation:
+// one static array per device, held until the process ends.  Host code never
+// uses the symbol itself (on the host it is only a stand-in); it uses the
+// device address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[2 * (size_t)LELPH_NOALLOC_MAX_ELEMS];
+
+#define NOALLOC_MAX_DEVICES 64
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// device address of `scratch` on the current device, looked up once per
+// device (the first call on a device, or lelph_create on it)
+static int scratch_ptr(double** out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return LELPH_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev])
+    {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return LELPH_ERR_HIP;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<double*>(g_scratch_ptr[dev]);
+    return LELPH_OK;
+}
+
+int lelph_elph_q_rotate_noalloc(const double* d_Dmats_l, const double* d_elph_mat_q,
+                                const double* d_Dmats_r, int nmodes, int nspin, int nb,
+                                int tim_rev, double* d_elph_mat_Sq, void* stream)
+{
+    if (!d_Dmats_l || !d_elph_mat_q || !d_Dmats_r || !d_elph_mat_Sq) return LELPH_ERR_ARG;
+    if (nmodes <= 0 || nspin <= 0 || nb <= 0 || nspin > 2 || nmodes > 65535)
+        return LELPH_ERR_ARG;
+    const int64_t mat = (int64_t)nb * nb;
+    const int64_t gsz = (int64_t)nmodes * nspin * mat;
+    if (gsz > (int64_t)LELPH_NOALLOC_MAX_ELEMS) return LELPH_ERR_SCRATCH;
+    double* sp = nullptr;
+    LELPH_TRY(scratch_ptr(&sp));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const z64* Dl = reinterpret_cast<const z64*>(d_Dmats_l);
+    const z64* Dr = reinterpret_cast<const z64*>(d_Dmats_r);
+    const z64* G = reinterpret_cast<const z64*>(d_elph_mat_q);
+    z64* T = reinterpret_cast<z64*>(sp);
+    z64* O = reinterpret_cast<z64*>(d_elph_mat_Sq);
+    const int tiles = (nb + 31) / 32;
+    const dim3 grid((unsigned)(tiles * tiles), (unsigned)nmodes);
+    // the same two batched GEMMs as lelph_elph_q_rotate_gpu; every element
+    // of T is written by the first before the second reads it
+    for (int is = 0; is < nspin; ++is)
+    {
+        if (tim_rev)
+            hipLaunchKernelGGL((k_zgemm_bat<true, true, false>), grid, dim3(256), 0, st,
+                               Dr + is * mat, (int64_t)0, G + is * mat, nspin * mat, T + is * mat,
+                               nspin * mat, nb);
+        else
+            hipLaunchKernelGGL((k_zgemm_bat<true, false, false>), grid, dim3(256), 0, st,
+                               Dr + is * mat, (int64_t)0, G + is * mat, nspin * mat, T + is * mat,
+                               nspin * mat, nb);
+        hipLaunchKernelGGL((k_zgemm_bat<false, false, true>), grid, dim3(256), 0, st,
+                           T + is * mat, nspin * mat, Dl + is * mat, (int64_t)0, O + is * mat,
+                           nspin * mat, nb);
+        HIP_TRY(hipGetLastError());
+    }
     return LELPH_OK;
 }
 

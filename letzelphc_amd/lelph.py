@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(HERE, "liblelph_hip.so")
 
 EXPORTS = [
     "lelph_create", "lelph_destroy", "lelph_strerror", "lelph_set_precision",
-    "lelph_get_precision",
+    "lelph_get_precision", "lelph_elph_q_rotate_noalloc",
     "lelph_get_device_count", "lelph_upload_wfc", "lelph_create_vlocg_table",
     "lelph_dvlocq", "lelph_set_dvscf", "lelph_get_dvscf", "lelph_elph_local",
     "lelph_add_elph_nonlocal", "lelph_gkq", "lelph_gkq_batch", "lelph_compute_elphq",

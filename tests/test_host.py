@@ -53,7 +53,7 @@ def test_nc4_library_exports_header():
 def test_status_strings():
     L = lelph.lib()
     seen = set()
-    for st in range(0, 9):
+    for st in range(0, 10):
         s = lelph.strerror(st)
         assert s and s not in seen
         seen.add(s)
