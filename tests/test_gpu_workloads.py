@@ -25,7 +25,7 @@ import pytest
 from letzelphc_amd import lelph, synth
 from oracle import oracle as O
 
-from .util import rel_err, say
+from .util import rel_err, say, tr_partner
 
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
 
@@ -103,23 +103,6 @@ def test_c2_first_pair_consistent(c2):
         assert err < TOL32
 
 
-def _tr_partner(s, ik, isym):
-    """A time-reversed operation j (index >= nsym/2) with S_j k = S_isym k
-    (mod G) for iBZ k ik, or None.  The k-grid map never picks one on this
-    cell (each star is reached by a rotation first), so pairs through the
-    time-reversed branch are built here: the same full-BZ k, its
-    wavefunction obtained as the conjugate of a rotated one."""
-    half = len(s.syms) // 2
-    Sc = [synth._sym_crys(np.asarray(R), s.alat, s.blat) for (R, _, _) in s.syms]
-    k = s.kpt_ibz_crys[ik]
-    want = Sc[isym] @ k
-    for j in range(half, len(s.syms)):
-        d = Sc[j] @ k - want
-        if np.linalg.norm(d - np.rint(d)) < 1e-5:
-            return j
-    return None
-
-
 def _c1_pick(s, iq, kminusq):
     """One pair per (k+q rotated?, k rotated?) class of the pair list, plus
     the first non-trivial pair again with its k side, and then its k+q
@@ -131,7 +114,7 @@ def _c1_pick(s, iq, kminusq):
             seen.add(key)
             pick.append(pr)
     ikq, ik, kqsym, ksym = next(p for p in pick if p[2] > 0 or p[3] > 0)
-    jk, jkq = _tr_partner(s, ik, ksym), _tr_partner(s, ikq, kqsym)
+    jk, jkq = tr_partner(s, ik, ksym), tr_partner(s, ikq, kqsym)
     assert jk is not None and jkq is not None
     pick += [(ikq, ik, kqsym, jk), (ikq, ik, jkq, ksym)]
     return pick

@@ -12,3 +12,21 @@ def rel_err(a, b) -> float:
 
 def say(msg: str):
     print(msg, flush=True)
+
+
+def tr_partner(s, ik, isym):
+    """A time-reversed operation j (index >= nsym/2) with S_j k = S_isym k
+    (mod G) for iBZ k ik, or None.  A k-grid map that reaches every star by
+    a rotation first never picks one, so pairs through the
+    time-reversed branch are built with it: the same full-BZ k, its
+    wavefunction obtained as the conjugate of a rotated one."""
+    half = len(s.syms) // 2
+    from letzelphc_amd import synth
+    Sc = [synth._sym_crys(np.asarray(R), s.alat, s.blat) for (R, _, _) in s.syms]
+    k = s.kpt_ibz_crys[ik]
+    want = Sc[isym] @ k
+    for j in range(half, len(s.syms)):
+        d = Sc[j] @ k - want
+        if np.linalg.norm(d - np.rint(d)) < 1e-5:
+            return j
+    return None
