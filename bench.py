@@ -97,6 +97,11 @@ def cpu_baseline(sys_, pairs, dv, cores, modes=(2, 6)):
                        f"{sys_.nmodes} modes: {t_full:.1f} s per pair")}
 
 
+def autotune_key(ctx):
+    from letzelphc_amd import autotune
+    return autotune.shape_key(ctx)
+
+
 def aux_measurements(s, ctx):
     """Timings of the rows around the hot path, on their own workloads:
     the dV_q Fourier interpolation (c0 cell, 24^3, coarse 4x4x4 q grid with a
@@ -172,30 +177,50 @@ def aux_measurements(s, ctx):
                                 "coarse_side_s_asr_all_huang": time.perf_counter() - t0}
     finally:
         ic.close()
-    # c3: eight uneven cells, 16 pairs each, mixed FP64 / FP32, one context
-    # and stream per cell: one thread per cell against one cell after another
+    # c3: eight uneven cells, 16 pairs each, mixed FP64 / FP32: the
+    # multi-cell batch (one persistent work-queue contraction launch per
+    # precision over every cell's tiles, the whole sequence captured in a
+    # HIP graph and replayed) against one lelph_gkq_batch per cell, one cell
+    # after another, and one lelph_gkq per pair; launch shapes autotuned per
+    # grid first
     from letzelphc_amd import batch
     cells = synth.config_c3()
     ctxs, jobs = [], []
+    multi = None
     try:
         for i, cs in enumerate(cells):
             cx = lelph.Context(cs, "fp64" if i % 2 == 0 else "fp32")
             cx.set_dvscf(cs.qpts[0], cs.eigvec(0), cs.dvscf(0), kernel="dfpt")
             ctxs.append(cx)
             jobs.append((cx, cs.qpts[0], synth.kq_pairs(cs, 0)))
-        batch.run_sequential(jobs)  # warm-up (allocations, plans)
-        t_seq, g_seq = batch.run_sequential(jobs)
-        t_con, g_con = batch.run_concurrent(jobs)
-        same = all(np.array_equal(a, b) for a, b in zip(g_seq, g_con))
+        t0 = time.perf_counter()
+        tuned = batch.tune(jobs)
+        t_tune = time.perf_counter() - t0
+        batch.run_sequential(jobs)  # warm-up (allocations)
+        reps = 3
+        t_seq = min(batch.run_sequential(jobs)[0] for _ in range(reps))
+        batch.run_pairwise(jobs)
+        t_pair, g_pair = batch.run_pairwise(jobs)
+        t0 = time.perf_counter()
+        multi = lelph.MultiBatch(jobs)
+        t_build = time.perf_counter() - t0
+        t_multi, g_multi, _ = batch.run_multi(jobs, reps=reps, multi=multi)
+        same = all(np.array_equal(a, b) for a, b in zip(g_multi, g_pair))
         nel = batch.g_elements(jobs)
         out["c3_batching"] = {
             "workload": "8 cells: FFT 24^3, 32^3, 36^3, 45x45x108, 48^3, 60^3, 64^3, 72^3; "
                         "npw 400..12,000, 8..64 bands, 6..36 modes, 16 pairs each, cells "
                         "alternately FP64 and FP32",
-            "sequential_s": t_seq, "concurrent_s": t_con,
-            "sequential_g_per_s": nel / t_seq, "concurrent_g_per_s": nel / t_con,
-            "speedup": t_seq / t_con, "results_bit_identical": same}
+            "multi_replay_s": t_multi, "per_cell_batches_s": t_seq, "per_pair_s": t_pair,
+            "multi_g_per_s": nel / t_multi, "per_cell_g_per_s": nel / t_seq,
+            "speedup_vs_per_cell": t_seq / t_multi, "speedup_vs_per_pair": t_pair / t_multi,
+            "multi_bit_identical_to_per_pair": same, "multi_build_s": t_build,
+            "multi_timings_ms": {k: float(v) for k, v in multi.timings().items()},
+            "autotune_s": t_tune,
+            "tuning": {autotune_key(c): r["tuning"] for (c, _, _), r in zip(jobs, tuned)}}
     finally:
+        if multi is not None:
+            multi.close()
         for cx in ctxs:
             cx.close()
     # the driver on real-format inputs: a Yambo SAVE and a QE ph_save of the

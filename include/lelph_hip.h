@@ -142,6 +142,24 @@ int lelph_create(const lelph_lattice* lattice, const lelph_pseudo* pseudo,
 void lelph_destroy(lelph_ctx* ctx);
 const char* lelph_strerror(int status);
 int lelph_set_precision(lelph_ctx* ctx, int precision);
+/* Launch shapes of a context (no counterpart in the reference, whose FFTW
+ * plans are measured per call, src/fft/planner.c:226-312): chosen per grid
+ * by the project's autotuner (letzelphc_amd/autotune.py).  fft_lines and
+ * fp32_form give bit-identical g; split_target moves the boundaries of the
+ * fixed-order FP64 sum (g changes at the rounding level).  Zero fields
+ * take the defaults. */
+typedef struct lelph_tuning
+{
+    int split_target; /* blocks aimed at by the FP64/FP32 contraction's K
+                         splits (default 2048)                               */
+    int fft_lines;    /* lines per block of the compiled line FFTs: 8, 16,
+                         32 where they fit in LDS, 0 = per-length default    */
+    int fp32_form;    /* FP32 contraction: 0 dV and psi_k staged separately,
+                         1 element-wise A = dV psi_k                          */
+} lelph_tuning;
+int lelph_set_tuning(lelph_ctx* ctx, const lelph_tuning* tuning);
+int lelph_get_tuning(lelph_ctx* ctx, lelph_tuning* tuning);
+
 /* The precision requested with lelph_create / lelph_set_precision and the
  * one the contractions run in on this context's shape: LELPH_BF16X3 runs as
  * LELPH_BF16X6 unless nspinor = 1, Nx*Ny*Nz is a multiple of 32 and
@@ -204,6 +222,30 @@ int lelph_gkq(lelph_ctx* ctx, const double qpt[3], int ikq, int ik, int kqsym,
  * nbnds, nbnds).  Device-resident between pairs, one copy back at the end. */
 int lelph_gkq_batch(lelph_ctx* ctx, const double qpt[3], int64_t npairs,
                     const lelph_pair* pairs, int non_loc, double* g_out);
+
+/* Uneven batches over several contexts (cells of different grids, plane
+ * wave, band and mode counts, FP64 or FP32 each): the k loops of
+ * compute_and_write_elphq (src/elph/compute_elph_matq.c:116-223) of every
+ * cell as one unit.  lelph_multi_create plans it: every pair gets its own
+ * psi boxes and K-split partials, the contractions of all pairs of one
+ * precision become the tile list of one persistent work-queue launch
+ * (largest tiles first), and the whole sequence (boxes, contractions,
+ * fixed-order reductions, non-local parts, copy-out) runs once eagerly and
+ * is captured in a HIP graph.  lelph_multi_run replays the graph; g_out[i]
+ * receives cell i's (npairs[i], nmodes, nspin, nbnds, nbnds).  g is
+ * bit-identical to lelph_gkq on each pair.  The contexts' dV and precision
+ * must stay as they were at creation; bf16 split precisions are not
+ * supported here (LELPH_ERR_UNSUPPORTED).  No counterpart in the
+ * reference.  Timings (ms): boxes, contractions, reductions + non-local +
+ * copy of the eager pass, the last replay in total, then the item counts
+ * and grids of the FP64 and FP32 queues. */
+typedef struct lelph_multi lelph_multi;
+int lelph_multi_create(int ncells, lelph_ctx* const* ctxs, const double* qpts,
+                       const int64_t* npairs, const lelph_pair* const* pairs, int non_loc,
+                       lelph_multi** out);
+int lelph_multi_run(lelph_multi* multi, double* const* g_out);
+int lelph_multi_timings(lelph_multi* multi, double* ms /* 8 */);
+void lelph_multi_destroy(lelph_multi* multi);
 
 /* The k loop of compute_and_write_elphq (src/elph/compute_elph_matq.c:20-234)
  * for the iBZ q point qpt (crystal): k+q lookup on the full-BZ grid,

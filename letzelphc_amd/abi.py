@@ -59,6 +59,12 @@ class Pseudo(C.Structure):
                 ("loc_pseudo", C.POINTER(LocalPseudo)), ("qmax_abs", C.c_double)]
 
 
+class Tuning(C.Structure):
+    """lelph_tuning: launch shapes of a context (fft_lines and fp32_form
+    give bit-identical g, split_target changes it at the rounding level)."""
+    _fields_ = [("split_target", C.c_int), ("fft_lines", C.c_int), ("fp32_form", C.c_int)]
+
+
 class Pair(C.Structure):
     _fields_ = [("ikq", C.c_int), ("ik", C.c_int), ("kqsym", C.c_int),
                 ("ksym", C.c_int)]

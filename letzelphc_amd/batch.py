@@ -1,18 +1,36 @@
-"""Uneven cells side by side on one GPU.
+"""Uneven batches of cells on one GPU.
 
-Several (k, q) workloads of different cells (FFT grids, plane-wave counts,
-bands, modes, precisions) run at once, one context per cell: each context
-owns its own non-blocking HIP stream, and a host thread per cell issues its
-batch (`lelph_gkq_batch`; ctypes releases the interpreter lock for the
-call), so small cells' kernels overlap instead of queueing one cell after
-another.  `run_sequential` is the per-cell baseline.
+A job is (Context, qpt, pairs): one cell (its own FFT grid, plane-wave,
+band and mode counts, FP64 or FP32) and the (k, k+q) pairs of one q.
+
+- `run_multi`: every job as one unit through lelph_multi - per-pair boxes,
+  one persistent work-queue contraction launch per precision over the
+  tiles of all pairs of all cells (largest first), the reductions and
+  non-local parts - captured in a HIP graph once and replayed
+  (letzelphc_amd.lelph.MultiBatch).  Each context first gets the launch
+  shapes the autotuner picked for its grid (letzelphc_amd.autotune).
+- `run_sequential`: the per-cell baseline, one lelph_gkq_batch per cell,
+  one cell after another.
+- `run_pairwise`: one lelph_gkq per pair, the launches the multi batch
+  reproduces bit for bit.
 """
 from __future__ import annotations
 
-import threading
 import time
 
 import numpy as np
+
+from . import autotune
+from .lelph import MultiBatch
+
+
+def tune(jobs, path=None, force=False):
+    """Autotunes (or applies the cached tuning of) every job's context;
+    returns the per-cell results."""
+    cache = autotune.load_cache(path)
+    out = [autotune.tune(ctx, pairs, q, cache=cache, force=force) for ctx, q, pairs in jobs]
+    autotune.save_cache(cache, path)
+    return out
 
 
 def run_sequential(jobs):
@@ -22,31 +40,22 @@ def run_sequential(jobs):
     return time.perf_counter() - t0, out
 
 
-def run_concurrent(jobs):
-    """the same jobs, one host thread (and so one stream) per cell"""
-    out = [None] * len(jobs)
-    err = [None] * len(jobs)
-    start = threading.Barrier(len(jobs) + 1)
-
-    def work(i, ctx, q, pairs):
-        start.wait()
-        try:
-            out[i] = ctx.gkq_batch(pairs, qpt=q)
-        except Exception as e:  # re-raised in the caller
-            err[i] = e
-
-    th = [threading.Thread(target=work, args=(i, *j)) for i, j in enumerate(jobs)]
-    for t in th:
-        t.start()
-    start.wait()
+def run_pairwise(jobs):
+    """one lelph_gkq per pair; returns (seconds, [g])"""
     t0 = time.perf_counter()
-    for t in th:
-        t.join()
-    dt = time.perf_counter() - t0
-    for e in err:
-        if e is not None:
-            raise e
-    return dt, out
+    out = [np.stack([ctx.gkq(*p, qpt=q) for p in pairs]) for ctx, q, pairs in jobs]
+    return time.perf_counter() - t0, out
+
+
+def run_multi(jobs, reps=1, multi=None):
+    """the jobs as one captured multi-cell batch (built unless given);
+    returns (seconds per replay, [g], the MultiBatch)"""
+    m = multi or MultiBatch(jobs)
+    out = m.run()  # first replay
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = m.run()
+    return (time.perf_counter() - t0) / reps, [o.copy() for o in out], m
 
 
 def g_elements(jobs):

@@ -74,26 +74,27 @@ __device__ inline cpx<T> load_dvpsi(const cpx<T>* __restrict__ psik,
     return cmul(V[r], pk[(int64_t)sp * N + r]);
 }
 
+// One tile of the contraction: tile index bx (mode group, row tile, column
+// tile), K split `split`, spin `is`.  The kernel k_contract runs one tile
+// per block; the work-queue kernel k_contract_queue runs tiles of many
+// (k, k+q) pairs and cells from a list.
 template <typename T>
-__global__ __launch_bounds__(256) void k_contract(const cpx<T>* __restrict__ psik,
-                                                  const cpx<T>* __restrict__ psikq,
-                                                  const cpx<T>* __restrict__ dv,
-                                                  z64* __restrict__ part, int nm, int nb,
-                                                  int nspin, int nspinor, int nmag,
-                                                  int64_t N, int64_t chunks_per_split)
+__device__ inline void contract_tile(const cpx<T>* __restrict__ psik,
+                                     const cpx<T>* __restrict__ psikq,
+                                     const cpx<T>* __restrict__ dv, z64* __restrict__ part,
+                                     int nm, int nb, int nspin, int nspinor, int nmag, int64_t N,
+                                     int64_t chunks_per_split, int bx, int split, int is)
 {
     typedef typename Mfma<T>::acc acc_t;
     __shared__ cpx<T> sA[CT_VB][CT_NB][CT_LD];
     __shared__ cpx<T> sB[CT_MB][CT_LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ntn = (nb + CT_NB - 1) / CT_NB, ntm = (nb + CT_MB - 1) / CT_MB;
-    int bx = blockIdx.x;
     const int tm = bx % ntm;
     bx /= ntm;
     const int tn = bx % ntn;
     bx /= ntn;
     const int v0 = bx * CT_VB, n0 = tn * CT_NB, m0 = tm * CT_MB;
-    const int split = blockIdx.y, is = blockIdx.z;
     const int64_t nchunks = (N + CT_KC - 1) / CT_KC;
     const int64_t c0 = (int64_t)split * chunks_per_split;
     const int64_t c1 = (c0 + chunks_per_split < nchunks) ? c0 + chunks_per_split : nchunks;
@@ -215,12 +216,12 @@ __global__ __launch_bounds__(256) void k_contract(const cpx<T>* __restrict__ psi
 // nspinor 2).  Used for FP32; in FP64 the products formed beside the FP64
 // MFMAs cost more than the loads saved.
 template <typename T>
-__global__ __launch_bounds__(256) void k_contract_sep(const cpx<T>* __restrict__ psik,
-                                                  const cpx<T>* __restrict__ psikq,
-                                                  const cpx<T>* __restrict__ dv,
-                                                  z64* __restrict__ part, int nm, int nb,
-                                                  int nspin, int nspinor, int nmag,
-                                                  int64_t N, int64_t chunks_per_split)
+__device__ inline void contract_sep_tile(const cpx<T>* __restrict__ psik,
+                                         const cpx<T>* __restrict__ psikq,
+                                         const cpx<T>* __restrict__ dv, z64* __restrict__ part,
+                                         int nm, int nb, int nspin, int nspinor, int nmag,
+                                         int64_t N, int64_t chunks_per_split, int bx, int split,
+                                         int is)
 {
     typedef typename Mfma<T>::acc acc_t;
     __shared__ cpx<T> sV[CT_VB][CT_LD]; // dV of the block's modes
@@ -228,13 +229,11 @@ __global__ __launch_bounds__(256) void k_contract_sep(const cpx<T>* __restrict__
     __shared__ cpx<T> sB[CT_MB][CT_LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ntn = (nb + CT_NB - 1) / CT_NB, ntm = (nb + CT_MB - 1) / CT_MB;
-    int bx = blockIdx.x;
     const int tm = bx % ntm;
     bx /= ntm;
     const int tn = bx % ntn;
     bx /= ntn;
     const int v0 = bx * CT_VB, n0 = tn * CT_NB, m0 = tm * CT_MB;
-    const int split = blockIdx.y, is = blockIdx.z;
     const int64_t nchunks = (N + CT_KC - 1) / CT_KC;
     const int64_t c0 = (int64_t)split * chunks_per_split;
     const int64_t c1 = (c0 + chunks_per_split < nchunks) ? c0 + chunks_per_split : nchunks;
@@ -353,6 +352,100 @@ __global__ __launch_bounds__(256) void k_contract_sep(const cpx<T>* __restrict__
                                di[rb][cb][e] + (double)acci[rb][cb][e]};
                 }
             }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_contract(const cpx<T>* __restrict__ psik,
+                                                  const cpx<T>* __restrict__ psikq,
+                                                  const cpx<T>* __restrict__ dv,
+                                                  z64* __restrict__ part, int nm, int nb,
+                                                  int nspin, int nspinor, int nmag,
+                                                  int64_t N, int64_t chunks_per_split)
+{
+    contract_tile<T>(psik, psikq, dv, part, nm, nb, nspin, nspinor, nmag, N, chunks_per_split,
+                     blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_contract_sep(const cpx<T>* __restrict__ psik,
+                                                      const cpx<T>* __restrict__ psikq,
+                                                      const cpx<T>* __restrict__ dv,
+                                                      z64* __restrict__ part, int nm, int nb,
+                                                      int nspin, int nspinor, int nmag,
+                                                      int64_t N, int64_t chunks_per_split)
+{
+    contract_sep_tile<T>(psik, psikq, dv, part, nm, nb, nspin, nspinor, nmag, N,
+                         chunks_per_split, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// A (k, k+q) pair of a work-queue launch: its boxes, potential, partials
+// and shape (the arguments of k_contract / k_contract_sep).
+template <typename T>
+struct QueuePair
+{
+    const cpx<T>* psik;
+    const cpx<T>* psikq;
+    const cpx<T>* dv;
+    z64* part;
+    int nm, nb, nspin, nspinor, nmag, sep;
+    int64_t N, chunks_per_split;
+};
+
+// One work item: a tile of one pair (pair index, tile, K split, spin).
+struct QueueItem
+{
+    int pair, tile, split, spin;
+};
+
+// Persistent work-queue contraction over the tiles of many pairs (of one
+// precision, possibly of different cells, grids, band and mode counts):
+// each block takes the next item from an atomic counter (zeroed by the
+// caller with a kernel before the launch) until the list is exhausted, so
+// every block reaches the exit.  The items are listed largest first.  A
+// tile computes exactly what k_contract / k_contract_sep computes for it,
+// so the partials, and g after the fixed-order reduction, are bit-identical
+// to one launch per pair.  SEP: the launch may contain separate-staging
+// pairs (FP32 without a 2x2 spinor potential).
+template <typename T, bool SEP>
+__global__ __launch_bounds__(256) void k_contract_queue(const QueuePair<T>* __restrict__ pairs,
+                                                        const QueueItem* __restrict__ items,
+                                                        int nitems, int* __restrict__ counter)
+{
+    __shared__ int s_next;
+    for (;;)
+    {
+        if (threadIdx.x == 0) s_next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int w = s_next;
+        __syncthreads(); // every thread has read s_next before it changes
+        if (w >= nitems) break;
+        const QueueItem it = items[w];
+        const QueuePair<T> p = pairs[it.pair];
+        if (SEP && p.sep)
+            contract_sep_tile<T>(p.psik, p.psikq, p.dv, p.part, p.nm, p.nb, p.nspin, p.nspinor,
+                                 p.nmag, p.N, p.chunks_per_split, it.tile, it.split, it.spin);
+        else
+            contract_tile<T>(p.psik, p.psikq, p.dv, p.part, p.nm, p.nb, p.nspin, p.nspinor,
+                             p.nmag, p.N, p.chunks_per_split, it.tile, it.split, it.spin);
+    }
+}
+
+// zeroes n 16-byte units (a kernel rather than a memset node, so that it
+// can sit in a captured graph); grid-stride
+__global__ void k_zero_u4(uint4* __restrict__ p, int64_t n)
+{
+    const uint4 z = {0u, 0u, 0u, 0u};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = z;
+}
+
+// zeroes n 32-bit words
+__global__ void k_zero_words(unsigned* __restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
 }
 
 // g[i] = scale * sum_s part[s][i], fixed order.

@@ -448,23 +448,57 @@ constexpr int fft_tl_c()
            : 8;
 }
 
+// a line tile TL fits when the twiddles and both line buffers stay within
+// 64 KB of static LDS
+template <typename T, int N, int TL>
+constexpr bool fft_tl_fits()
+{
+    return (N + 2 * TL * (N + 1)) * (int)sizeof(cpx<T>) <= 64 * 1024;
+}
+
+template <typename T, int N, int TL>
+static bool fft_launch_tl(cpx<T>* data, int64_t n_outer, int64_t n_inner, int64_t ostride,
+                          int64_t lstride, int64_t estride, int sign, const cpx<T>* W,
+                          hipStream_t stream, const int* ocnt, int omod, const int* emask)
+{
+    if constexpr (!fft_tl_fits<T, N, TL>())
+        return false;
+    else
+    {
+        const int64_t nblk = cdiv(n_inner, TL) * n_outer;
+        if (nblk > 0x7fffffffLL) return false;
+        if (sign > 0)
+            hipLaunchKernelGGL((k_fft_lines_c<T, 1, N, TL>), dim3((unsigned)nblk), dim3(256), 0,
+                               stream, data, n_inner, ostride, lstride, estride, W, ocnt, omod,
+                               emask);
+        else
+            hipLaunchKernelGGL((k_fft_lines_c<T, -1, N, TL>), dim3((unsigned)nblk), dim3(256), 0,
+                               stream, data, n_inner, ostride, lstride, estride, W, ocnt, omod,
+                               emask);
+        return true;
+    }
+}
+
+// tl: lines per block (8, 16 or 32 where it fits; anything else takes the
+// default of fft_tl_c).  The transform is the same for every tile size:
+// the lines are independent, so the results are bit-identical.
 template <typename T, int N>
 static bool fft_launch_c(cpx<T>* data, int64_t n_outer, int64_t n_inner, int64_t ostride,
                          int64_t lstride, int64_t estride, int sign, const cpx<T>* W,
-                         hipStream_t stream, const int* ocnt, int omod, const int* emask)
+                         hipStream_t stream, const int* ocnt, int omod, const int* emask,
+                         int tl = 0)
 {
-    constexpr int TL = fft_tl_c<T, N>();
-    const int64_t nblk = cdiv(n_inner, TL) * n_outer;
-    if (nblk > 0x7fffffffLL) return false;
-    if (sign > 0)
-        hipLaunchKernelGGL((k_fft_lines_c<T, 1, N, TL>), dim3((unsigned)nblk), dim3(256), 0,
-                           stream, data, n_inner, ostride, lstride, estride, W, ocnt, omod,
-                           emask);
-    else
-        hipLaunchKernelGGL((k_fft_lines_c<T, -1, N, TL>), dim3((unsigned)nblk), dim3(256), 0,
-                           stream, data, n_inner, ostride, lstride, estride, W, ocnt, omod,
-                           emask);
-    return true;
+    if (tl == 8 && fft_launch_tl<T, N, 8>(data, n_outer, n_inner, ostride, lstride, estride, sign,
+                                          W, stream, ocnt, omod, emask))
+        return true;
+    if (tl == 16 && fft_launch_tl<T, N, 16>(data, n_outer, n_inner, ostride, lstride, estride,
+                                            sign, W, stream, ocnt, omod, emask))
+        return true;
+    if (tl == 32 && fft_launch_tl<T, N, 32>(data, n_outer, n_inner, ostride, lstride, estride,
+                                            sign, W, stream, ocnt, omod, emask))
+        return true;
+    return fft_launch_tl<T, N, fft_tl_c<T, N>()>(data, n_outer, n_inner, ostride, lstride,
+                                                 estride, sign, W, stream, ocnt, omod, emask);
 }
 
 // true if n has a compile-time line kernel
@@ -486,10 +520,12 @@ static inline const cpx<T>* axis_tw(const FftAxis& a);
 template <typename T>
 static bool fft_try_c(int n, cpx<T>* data, int64_t no, int64_t ni, int64_t os, int64_t ls,
                       int64_t es, int sign, const cpx<T>* W, hipStream_t st,
-                      const int* ocnt = nullptr, int omod = 1, const int* emask = nullptr)
+                      const int* ocnt = nullptr, int omod = 1, const int* emask = nullptr,
+                      int tl = 0)
 {
 #define FFT_C_CASE(NV)                                                                        \
-    case NV: return fft_launch_c<T, NV>(data, no, ni, os, ls, es, sign, W, st, ocnt, omod, emask);
+    case NV:                                                                                  \
+        return fft_launch_c<T, NV>(data, no, ni, os, ls, es, sign, W, st, ocnt, omod, emask, tl);
     switch (n)
     {
         FFT_C_CASE(12)
@@ -523,11 +559,11 @@ inline const c32* axis_tw<float>(const FftAxis& a)
 template <typename T>
 static int fft_pass(const FftAxis& ax, cpx<T>* data, int64_t n_outer, int64_t n_inner,
                     int64_t ostride, int64_t lstride, int64_t estride, int sign,
-                    hipStream_t stream)
+                    hipStream_t stream, int lines = 0)
 {
     const int n = ax.n;
     if (fft_try_c<T>(n, data, n_outer, n_inner, ostride, lstride, estride, sign, axis_tw<T>(ax),
-                     stream))
+                     stream, nullptr, 1, nullptr, lines))
     {
         HIP_TRY(hipGetLastError());
         return LELPH_OK;
@@ -556,15 +592,15 @@ static int fft_pass(const FftAxis& ax, cpx<T>* data, int64_t n_outer, int64_t n_
 // 3-D transform of nsets boxes (Nx,Ny,Nz), in place.
 template <typename T>
 static int fft3d_batch(const FftAxis* ax, cpx<T>* data, int64_t nsets, int sign,
-                       hipStream_t stream)
+                       hipStream_t stream, int tl = 0)
 {
     const int64_t Nx = ax[0].n, Ny = ax[1].n, Nz = ax[2].n;
     const int64_t nf = Nx * Ny * Nz;
     // z: contiguous lines
-    LELPH_TRY(fft_pass<T>(ax[2], data, 1, nsets * Nx * Ny, 0, Nz, 1, sign, stream));
+    LELPH_TRY(fft_pass<T>(ax[2], data, 1, nsets * Nx * Ny, 0, Nz, 1, sign, stream, tl));
     // y: outer (set, x), inner z
-    LELPH_TRY(fft_pass<T>(ax[1], data, nsets * Nx, Nz, Ny * Nz, 1, Nz, sign, stream));
+    LELPH_TRY(fft_pass<T>(ax[1], data, nsets * Nx, Nz, Ny * Nz, 1, Nz, sign, stream, tl));
     // x: outer set, inner (y, z)
-    LELPH_TRY(fft_pass<T>(ax[0], data, nsets, Ny * Nz, nf, 1, Ny * Nz, sign, stream));
+    LELPH_TRY(fft_pass<T>(ax[0], data, nsets, Ny * Nz, nf, 1, Ny * Nz, sign, stream, tl));
     return LELPH_OK;
 }

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <array>
 #include <mutex>
 #include <vector>
 
@@ -82,6 +83,7 @@ struct lelph_ctx
     int device = 0;
     hipStream_t stream = nullptr;
     int precision = LELPH_FP64;
+    lelph_tuning tune{0, 0, 0};
 
     // lattice (host copy)
     lelph_lattice lat{};
@@ -359,6 +361,8 @@ static int ensure_dv32(lelph_ctx* c)
 // plane, gather + z transform per occupied plane (and its y transform in
 // LDS when the y-z plane is square), else a y pass on occupied planes; the
 // x pass reads occupied planes only
+static int zero_async(void* p, size_t bytes, hipStream_t st);
+
 static int pruned_side(lelph_ctx* c, const DevWfc& w, const SideGeom& geo, const int* idx,
                        const z64* ph, int side, c32* box)
 {
@@ -371,7 +375,7 @@ static int pruned_side(lelph_ctx* c, const DevWfc& w, const SideGeom& geo, const
     LELPH_TRY(c->pl_cur[side].ensure(sizeof(int) * Nx));
     LELPH_TRY(c->pl_perm[side].ensure(sizeof(int) * std::max<int64_t>(1, w.npw)));
     int* cnt = c->pl_cnt[side].as<int>();
-    HIP_TRY(hipMemsetAsync(cnt, 0, sizeof(int) * Nx, c->stream));
+    LELPH_TRY(zero_async(cnt, sizeof(int) * Nx, c->stream));
     const unsigned gb = (unsigned)cdiv(w.npw, 4096);
     hipLaunchKernelGGL(k_plane_count, dim3(gb), dim3(256), 0, c->stream, idx, w.npw, nyz, Nx, cnt);
     hipLaunchKernelGGL(k_plane_scan, dim3(1), dim3(1024), 0, c->stream, cnt, Nx,
@@ -392,12 +396,12 @@ static int pruned_side(lelph_ctx* c, const DevWfc& w, const SideGeom& geo, const
             return LELPH_ERR_UNSUPPORTED;
         HIP_TRY(hipGetLastError());
         if (!fft_try_c<float>(Ny, box, (int64_t)nbox * Nx, Nz, nyz, 1, Nz, +1, c->ax[1].w32,
-                              c->stream, cnt, Nx, nullptr))
+                              c->stream, cnt, Nx, nullptr, c->tune.fft_lines))
             return LELPH_ERR_UNSUPPORTED;
     }
     HIP_TRY(hipGetLastError());
     if (!fft_try_c<float>(Nx, box, nbox, nyz, c->nf, 1, nyz, +1, c->ax[0].w32, c->stream, nullptr,
-                          1, cnt))
+                          1, cnt, c->tune.fft_lines))
         return LELPH_ERR_UNSUPPORTED;
     HIP_TRY(hipGetLastError());
     return LELPH_OK;
@@ -450,14 +454,14 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
                                pg[2], -1);
             HIP_TRY(hipGetLastError());
         }
-        if (!pruned) HIP_TRY(hipMemsetAsync(c->box_kq.p, 0, bb, c->stream));
+        if (!pruned) LELPH_TRY(zero_async(c->box_kq.p, bb, c->stream));
     }
     else
     {
         if (!pruned)
         {
-            HIP_TRY(hipMemsetAsync(c->box_k.p, 0, bb, c->stream));
-            HIP_TRY(hipMemsetAsync(c->box_kq.p, 0, bb, c->stream));
+            LELPH_TRY(zero_async(c->box_k.p, bb, c->stream));
+            LELPH_TRY(zero_async(c->box_kq.p, bb, c->stream));
         }
         hipLaunchKernelGGL(k_gindex, dim3((unsigned)cdiv(wk.npw, 256)), dim3(256), 0, c->stream,
                            wk.gvec, wk.npw, gk, Nx, Ny, Nz, c->idx_k.as<int>(),
@@ -487,8 +491,11 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
                            c->stream, wkq.wfc, nsb, L.nspinor, wkq.npw, c->idx_kq.as<int>(),
                            c->ph_kq.as<z64>(), gkq, c->nf, c->box_kq.as<cpx<T>>());
         HIP_TRY(hipGetLastError());
-        if (!reuse) LELPH_TRY(fft3d_batch<T>(c->ax, c->box_k.as<cpx<T>>(), nbox, +1, c->stream));
-        LELPH_TRY(fft3d_batch<T>(c->ax, c->box_kq.as<cpx<T>>(), nbox, +1, c->stream));
+        if (!reuse)
+            LELPH_TRY(fft3d_batch<T>(c->ax, c->box_k.as<cpx<T>>(), nbox, +1, c->stream,
+                                     c->tune.fft_lines));
+        LELPH_TRY(fft3d_batch<T>(c->ax, c->box_kq.as<cpx<T>>(), nbox, +1, c->stream,
+                                 c->tune.fft_lines));
     }
     if (c->box_reuse)
     {
@@ -501,10 +508,31 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
     return LELPH_OK;
 }
 
-static int pick_splits(int64_t base_blocks, int64_t nchunks, int64_t out_elems, int64_t* cps,
-                       int* nsplit)
+// zeroes device memory with a kernel (graph-capturable, unlike a memset
+// node); bytes a multiple of 4, 16-byte units when aligned
+static int zero_async(void* p, size_t bytes, hipStream_t st)
 {
-    int64_t ns = cdiv(2048, base_blocks > 0 ? base_blocks : 1);
+    if (!bytes) return LELPH_OK;
+    if (((uintptr_t)p % 16) == 0 && bytes % 16 == 0)
+    {
+        const int64_t n = (int64_t)(bytes / 16);
+        const unsigned nb = (unsigned)std::min<int64_t>(cdiv(n, 256), 65536);
+        hipLaunchKernelGGL(k_zero_u4, dim3(nb), dim3(256), 0, st, static_cast<uint4*>(p), n);
+    }
+    else
+    {
+        const int64_t n = (int64_t)(bytes / 4);
+        const unsigned nb = (unsigned)std::min<int64_t>(cdiv(n, 256), 65536);
+        hipLaunchKernelGGL(k_zero_words, dim3(nb), dim3(256), 0, st, static_cast<unsigned*>(p), n);
+    }
+    HIP_TRY(hipGetLastError());
+    return LELPH_OK;
+}
+
+static int pick_splits(int64_t base_blocks, int64_t nchunks, int64_t out_elems, int64_t* cps,
+                       int* nsplit, int64_t target = 2048)
+{
+    int64_t ns = cdiv(target > 0 ? target : 2048, base_blocks > 0 ? base_blocks : 1);
     if (ns < 1) ns = 1;
     // keep the partial buffer below ~512 MB
     const int64_t cap = std::max<int64_t>(1, (512ll << 20) / (16 * std::max<int64_t>(out_elems, 1)));
@@ -514,6 +542,15 @@ static int pick_splits(int64_t base_blocks, int64_t nchunks, int64_t out_elems, 
     *cps = cdiv(nchunks, ns);
     *nsplit = (int)cdiv(nchunks, *cps);
     return LELPH_OK;
+}
+
+// true when the contraction of this context runs k_contract_sep
+template <typename T>
+static bool contract_sep_form(const lelph_ctx* c)
+{
+    const int nmag_eff = (c->lat.nspinor == 1 && c->lat.nmag == 4) ? 1 : c->lat.nmag;
+    return std::is_same<T, float>::value && !(nmag_eff == 4 && c->lat.nspinor == 2) &&
+           c->tune.fp32_form == 0;
 }
 
 // local part into g (device): elphLocal
@@ -533,13 +570,13 @@ static int run_local(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64*
     const int64_t gsz = (int64_t)nm * nspin * nb * nb;
     int64_t cps;
     int nsplit;
-    pick_splits(tiles * nspin, nchunks, gsz, &cps, &nsplit);
+    pick_splits(tiles * nspin, nchunks, gsz, &cps, &nsplit, c->tune.split_target);
     LELPH_TRY(c->part.ensure(sizeof(z64) * (size_t)(nsplit * gsz)));
     // FP32: dV and psi_k staged separately unless the potential is a 2x2
-    // spinor one (87.0 -> 75.5 ms per c2 pair); FP64 forms A element by
-    // element (its FP64 products beside the FP64 MFMAs made the separate
-    // form 21 % slower)
-    if (!std::is_same<T, float>::value || (nmag_eff == 4 && L.nspinor == 2))
+    // spinor one (87.0 -> 75.5 ms per c2 pair) or the tuning asks for the
+    // element-wise form; FP64 forms A element by element (its FP64 products
+    // beside the FP64 MFMAs made the separate form 21 % slower)
+    if (!contract_sep_form<T>(c))
         hipLaunchKernelGGL(k_contract<T>, dim3((unsigned)tiles, (unsigned)nsplit, (unsigned)nspin),
                            dim3(256), 0, c->stream, c->box_k.as<cpx<T>>(),
                            c->box_kq.as<cpx<T>>(), dv_ptr<T>(c), c->part.as<z64>(), nm, nb,
@@ -1037,6 +1074,24 @@ int lelph_set_precision(lelph_ctx* c, int p)
 {
     if (!c || p < LELPH_FP64 || p > LELPH_BF16X6) return LELPH_ERR_ARG;
     c->precision = p;
+    return LELPH_OK;
+}
+
+int lelph_set_tuning(lelph_ctx* c, const lelph_tuning* t)
+{
+    if (!c || !t) return LELPH_ERR_ARG;
+    if (t->split_target < 0 || t->split_target > (1 << 20)) return LELPH_ERR_ARG;
+    if (t->fft_lines != 0 && t->fft_lines != 8 && t->fft_lines != 16 && t->fft_lines != 32)
+        return LELPH_ERR_ARG;
+    if (t->fp32_form != 0 && t->fp32_form != 1) return LELPH_ERR_ARG;
+    c->tune = *t;
+    return LELPH_OK;
+}
+
+int lelph_get_tuning(lelph_ctx* c, lelph_tuning* t)
+{
+    if (!c || !t) return LELPH_ERR_ARG;
+    *t = c->tune;
     return LELPH_OK;
 }
 
@@ -1559,6 +1614,392 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     LELPH_TRY(check_err(c));
     return finish_timings(c, npairs);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Uneven batches over several contexts (cells): every pair's boxes, one     */
+/* work-queue contraction launch per precision over the tiles of all pairs, */
+/* the reductions and non-local parts, the copy-out, captured in one HIP     */
+/* graph and replayed                                                        */
+/* ------------------------------------------------------------------------ */
+} // extern "C"
+
+namespace {
+struct MultiPair
+{
+    int cell = 0;
+    int64_t index = 0; // position in its cell's output
+    lelph_pair pr{};
+    int prec = LELPH_FP64;
+    int sep = 0;
+    int nsplit = 1;
+    int64_t cps = 1, gsz = 0, tiles = 0;
+    DBuf box_k, box_kq, part;
+};
+} // namespace
+
+struct lelph_multi
+{
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<lelph_ctx*> ctx;
+    std::vector<std::array<double, 3>> qpt;
+    std::vector<int64_t> npairs;
+    std::vector<MultiPair> pairs;
+    int non_loc = 1;
+    std::vector<DBuf> gdev;    // per cell: (npairs, nmodes, nspin, nb, nb) complex128
+    std::vector<void*> ghost;  // per cell: pinned copy of gdev
+    std::vector<size_t> gbytes;
+    DBuf qp64, qi64, qp32, qi32, counter;
+    int n64 = 0, n32 = 0, grid64 = 0, grid32 = 0;
+    bool sep32 = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double ms[4] = {0, 0, 0, 0};
+};
+
+namespace {
+// issues the whole batch on m->stream (eagerly with events, or under capture)
+static int multi_enqueue(lelph_multi* m, bool timed)
+{
+    struct Saved
+    {
+        hipStream_t st;
+        DBuf bk, bkq;
+        bool reuse, prev;
+    };
+    std::vector<Saved> saved(m->ctx.size());
+    for (size_t i = 0; i < m->ctx.size(); ++i)
+    {
+        lelph_ctx* c = m->ctx[i];
+        saved[i] = {c->stream, c->box_k, c->box_kq, c->box_reuse, c->box_prev_valid};
+        c->stream = m->stream;
+        c->box_reuse = false;
+        c->box_prev_valid = false;
+    }
+    int st = LELPH_OK;
+    auto body = [&]() -> int {
+        if (timed) HIP_TRY(hipEventRecord(m->ev[0], m->stream));
+        // every pair's psi_k and psi_k+q boxes into its own slots
+        for (auto& p : m->pairs)
+        {
+            lelph_ctx* c = m->ctx[p.cell];
+            c->box_k = p.box_k;
+            c->box_kq = p.box_kq;
+            const double* q = m->qpt[p.cell].data();
+            if (p.prec == LELPH_FP64)
+                LELPH_TRY(build_boxes<double>(c, q, p.pr));
+            else
+                LELPH_TRY(build_boxes<float>(c, q, p.pr));
+        }
+        if (timed) HIP_TRY(hipEventRecord(m->ev[1], m->stream));
+        // the work-queue contractions
+        LELPH_TRY(zero_async(m->counter.p, 16, m->stream));
+        if (m->n64)
+            hipLaunchKernelGGL((k_contract_queue<double, false>), dim3((unsigned)m->grid64),
+                               dim3(256), 0, m->stream, m->qp64.as<QueuePair<double>>(),
+                               m->qi64.as<QueueItem>(), m->n64, m->counter.as<int>());
+        if (m->n32)
+        {
+            if (m->sep32)
+                hipLaunchKernelGGL((k_contract_queue<float, true>), dim3((unsigned)m->grid32),
+                                   dim3(256), 0, m->stream, m->qp32.as<QueuePair<float>>(),
+                                   m->qi32.as<QueueItem>(), m->n32, m->counter.as<int>() + 1);
+            else
+                hipLaunchKernelGGL((k_contract_queue<float, false>), dim3((unsigned)m->grid32),
+                                   dim3(256), 0, m->stream, m->qp32.as<QueuePair<float>>(),
+                                   m->qi32.as<QueueItem>(), m->n32, m->counter.as<int>() + 1);
+        }
+        HIP_TRY(hipGetLastError());
+        if (timed) HIP_TRY(hipEventRecord(m->ev[2], m->stream));
+        // fixed-order reductions, the non-local parts, the copy-out
+        for (auto& p : m->pairs)
+        {
+            lelph_ctx* c = m->ctx[p.cell];
+            z64* g = m->gdev[p.cell].as<z64>() + p.index * p.gsz;
+            hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)cdiv(p.gsz, 256)), dim3(256), 0,
+                               m->stream, p.part.as<z64>(), p.nsplit, p.gsz, 1.0 / (double)c->nf,
+                               g);
+            HIP_TRY(hipGetLastError());
+            if (m->non_loc) LELPH_TRY(run_nonlocal(c, p.pr, g, m->stream, nullptr));
+        }
+        for (size_t i = 0; i < m->ctx.size(); ++i)
+            if (m->gbytes[i])
+                HIP_TRY(hipMemcpyAsync(m->ghost[i], m->gdev[i].p, m->gbytes[i],
+                                       hipMemcpyDeviceToHost, m->stream));
+        if (timed) HIP_TRY(hipEventRecord(m->ev[3], m->stream));
+        return LELPH_OK;
+    };
+    st = body();
+    for (size_t i = 0; i < m->ctx.size(); ++i)
+    {
+        lelph_ctx* c = m->ctx[i];
+        c->stream = saved[i].st;
+        c->box_k = saved[i].bk;
+        c->box_kq = saved[i].bkq;
+        c->box_reuse = saved[i].reuse;
+        c->box_prev_valid = false;
+    }
+    return st;
+}
+
+template <typename T, bool SEP>
+static int queue_grid(int nitems, int dev, int* grid)
+{
+    int occ = 0, ncu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_contract_queue<T, SEP>, 256, 0));
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    *grid = std::max(1, std::min(nitems, std::max(1, occ) * std::max(1, ncu)));
+    return LELPH_OK;
+}
+
+// the pair and item lists of one precision's queue, items largest first
+template <typename T>
+static int build_queue(lelph_multi* m, int prec, DBuf& qp, DBuf& qi, int* nitems, bool* any_sep)
+{
+    std::vector<QueuePair<T>> hp;
+    std::vector<std::pair<int64_t, QueueItem>> items;
+    *any_sep = false;
+    for (auto& p : m->pairs)
+    {
+        if (p.prec != prec) continue;
+        lelph_ctx* c = m->ctx[p.cell];
+        const lelph_lattice& L = c->lat;
+        const int nmag_eff = (L.nspinor == 1 && L.nmag == 4) ? 1 : L.nmag;
+        QueuePair<T> d{};
+        d.psik = p.box_k.as<cpx<T>>();
+        d.psikq = p.box_kq.as<cpx<T>>();
+        d.dv = dv_ptr<T>(c);
+        d.part = p.part.as<z64>();
+        d.nm = L.nmodes;
+        d.nb = L.nbnds;
+        d.nspin = L.nspin;
+        d.nspinor = L.nspinor;
+        d.nmag = nmag_eff;
+        d.sep = p.sep;
+        d.N = c->nf;
+        d.chunks_per_split = p.cps;
+        *any_sep = *any_sep || p.sep;
+        const int pi = (int)hp.size();
+        hp.push_back(d);
+        const int64_t nchunks = cdiv(c->nf, CT_KC);
+        for (int64_t t = 0; t < p.tiles; ++t)
+            for (int sp = 0; sp < p.nsplit; ++sp)
+                for (int is = 0; is < L.nspin; ++is)
+                {
+                    const int64_t c0 = (int64_t)sp * p.cps;
+                    const int64_t cost = std::max<int64_t>(0, std::min(p.cps, nchunks - c0)) *
+                                         L.nspinor;
+                    items.push_back({cost, QueueItem{pi, (int)t, sp, is}});
+                }
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const std::pair<int64_t, QueueItem>& a,
+                        const std::pair<int64_t, QueueItem>& b) { return a.first > b.first; });
+    if (items.size() > 0x7fffffff) return LELPH_ERR_UNSUPPORTED;
+    *nitems = (int)items.size();
+    if (!*nitems) return LELPH_OK;
+    std::vector<QueueItem> hi(items.size());
+    for (size_t i = 0; i < items.size(); ++i) hi[i] = items[i].second;
+    LELPH_TRY(qp.ensure(sizeof(QueuePair<T>) * hp.size()));
+    LELPH_TRY(qi.ensure(sizeof(QueueItem) * hi.size()));
+    HIP_TRY(hipMemcpy(qp.p, hp.data(), sizeof(QueuePair<T>) * hp.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(qi.p, hi.data(), sizeof(QueueItem) * hi.size(), hipMemcpyHostToDevice));
+    return LELPH_OK;
+}
+
+static void multi_free(lelph_multi* m)
+{
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    if (m->exec) (void)hipGraphExecDestroy(m->exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    for (auto& p : m->pairs)
+    {
+        p.box_k.release();
+        p.box_kq.release();
+        p.part.release();
+    }
+    for (auto& g : m->gdev) g.release();
+    for (void* h : m->ghost)
+        if (h) (void)hipHostFree(h);
+    m->qp64.release();
+    m->qi64.release();
+    m->qp32.release();
+    m->qi32.release();
+    m->counter.release();
+    for (auto& e : m->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+} // namespace
+
+extern "C" {
+
+int lelph_multi_create(int ncells, lelph_ctx* const* ctxs, const double* qpts,
+                       const int64_t* npairs, const lelph_pair* const* pairs, int non_loc,
+                       lelph_multi** out)
+{
+    if (!out || ncells < 1 || !ctxs || !qpts || !npairs || !pairs) return LELPH_ERR_ARG;
+    *out = nullptr;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    for (int i = 0; i < ncells; ++i)
+    {
+        lelph_ctx* c = ctxs[i];
+        if (!c || npairs[i] < 0 || (npairs[i] > 0 && !pairs[i])) return LELPH_ERR_ARG;
+        if (c->device != dev) return LELPH_ERR_ARG;
+        if (!c->dv_set) return LELPH_ERR_STATE;
+        const int pe = effective_precision(c);
+        if (pe != LELPH_FP64 && pe != LELPH_FP32) return LELPH_ERR_UNSUPPORTED;
+        if (c->lat.nmag == 2 && (c->lat.nspin != 2 || c->lat.nspinor != 1)) return LELPH_ERR_ARG;
+        for (int64_t j = 0; j < npairs[i]; ++j)
+        {
+            const lelph_pair& p = pairs[i][j];
+            if (p.ik < 0 || p.ikq < 0 || p.ik >= (int)c->wf.size() || p.ikq >= (int)c->wf.size())
+                return LELPH_ERR_ARG;
+            if (p.ksym < 0 || p.kqsym < 0 || p.ksym >= c->lat.nsym || p.kqsym >= c->lat.nsym)
+                return LELPH_ERR_ARG;
+            if (!c->wf[p.ik].wfc || !c->wf[p.ikq].wfc) return LELPH_ERR_STATE;
+        }
+    }
+    lelph_multi* m = new (std::nothrow) lelph_multi();
+    if (!m) return LELPH_ERR_ALLOC;
+    auto fail = [&](int st) {
+        multi_free(m);
+        return st;
+    };
+    m->non_loc = non_loc ? 1 : 0;
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(LELPH_ERR_HIP);
+    for (auto& e : m->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(LELPH_ERR_HIP);
+    for (int i = 0; i < ncells; ++i)
+    {
+        lelph_ctx* c = ctxs[i];
+        m->ctx.push_back(c);
+        m->qpt.push_back({qpts[3 * i], qpts[3 * i + 1], qpts[3 * i + 2]});
+        m->npairs.push_back(npairs[i]);
+        const int prec = effective_precision(c);
+        if (prec == LELPH_FP32)
+        {
+            int st = ensure_dv32(c);
+            if (st != LELPH_OK) return fail(st);
+        }
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(LELPH_ERR_HIP);
+        const lelph_lattice& L = c->lat;
+        const int64_t gsz = (int64_t)L.nmodes * L.nspin * L.nbnds * L.nbnds;
+        const int64_t nbox = (int64_t)L.nspin * L.nbnds * L.nspinor;
+        const size_t bb = (prec == LELPH_FP64 ? sizeof(z64) : sizeof(c32)) * (size_t)(nbox * c->nf);
+        const int64_t nchunks = cdiv(c->nf, CT_KC);
+        const int64_t tiles =
+            (int64_t)cdiv(L.nmodes, CT_VB) * cdiv(L.nbnds, CT_NB) * cdiv(L.nbnds, CT_MB);
+        for (int64_t j = 0; j < npairs[i]; ++j)
+        {
+            MultiPair p;
+            p.cell = i;
+            p.index = j;
+            p.pr = pairs[i][j];
+            p.prec = prec;
+            p.sep = prec == LELPH_FP32 ? (contract_sep_form<float>(c) ? 1 : 0) : 0;
+            p.gsz = gsz;
+            p.tiles = tiles;
+            pick_splits(tiles * L.nspin, nchunks, gsz, &p.cps, &p.nsplit, c->tune.split_target);
+            if (p.box_k.ensure(bb) || p.box_kq.ensure(bb) ||
+                p.part.ensure(sizeof(z64) * (size_t)(p.nsplit * gsz)))
+            {
+                m->pairs.push_back(p);
+                return fail(LELPH_ERR_ALLOC);
+            }
+            m->pairs.push_back(p);
+        }
+        m->gdev.emplace_back();
+        m->gbytes.push_back(sizeof(z64) * (size_t)(npairs[i] * gsz));
+        m->ghost.push_back(nullptr);
+        if (m->gbytes[i])
+        {
+            if (m->gdev[i].ensure(m->gbytes[i])) return fail(LELPH_ERR_ALLOC);
+            if (hipHostMalloc(&m->ghost[i], m->gbytes[i]) != hipSuccess)
+            {
+                m->ghost[i] = nullptr;
+                return fail(LELPH_ERR_ALLOC);
+            }
+        }
+    }
+    bool sep64 = false;
+    int st = build_queue<double>(m, LELPH_FP64, m->qp64, m->qi64, &m->n64, &sep64);
+    if (st == LELPH_OK) st = build_queue<float>(m, LELPH_FP32, m->qp32, m->qi32, &m->n32, &m->sep32);
+    if (st == LELPH_OK && m->n64) st = queue_grid<double, false>(m->n64, dev, &m->grid64);
+    if (st == LELPH_OK && m->n32)
+        st = m->sep32 ? queue_grid<float, true>(m->n32, dev, &m->grid32)
+                      : queue_grid<float, false>(m->n32, dev, &m->grid32);
+    if (st == LELPH_OK && m->counter.ensure(16)) st = LELPH_ERR_ALLOC;
+    if (st != LELPH_OK) return fail(st);
+    // eager pass: every workspace of the contexts reaches its size here,
+    // so the captured pass allocates nothing
+    st = multi_enqueue(m, true);
+    if (st == LELPH_OK && hipStreamSynchronize(m->stream) != hipSuccess) st = LELPH_ERR_HIP;
+    if (st != LELPH_OK) return fail(st);
+    float a = 0;
+    for (int k = 0; k < 3; ++k)
+    {
+        if (hipEventElapsedTime(&a, m->ev[k], m->ev[k + 1]) != hipSuccess) return fail(LELPH_ERR_HIP);
+        m->ms[k] = a;
+    }
+    for (lelph_ctx* c : m->ctx)
+    {
+        st = check_err(c);
+        if (st != LELPH_OK) return fail(st);
+    }
+    if (hipStreamBeginCapture(m->stream, hipStreamCaptureModeRelaxed) != hipSuccess)
+        return fail(LELPH_ERR_HIP);
+    st = multi_enqueue(m, false);
+    hipGraph_t gr = nullptr;
+    const hipError_t ec = hipStreamEndCapture(m->stream, &gr);
+    m->graph = gr;
+    if (st != LELPH_OK) return fail(st);
+    if (ec != hipSuccess || !gr) return fail(LELPH_ERR_HIP);
+    if (hipGraphInstantiate(&m->exec, m->graph, nullptr, nullptr, 0) != hipSuccess)
+    {
+        m->exec = nullptr;
+        return fail(LELPH_ERR_HIP);
+    }
+    *out = m;
+    return LELPH_OK;
+}
+
+int lelph_multi_run(lelph_multi* m, double* const* g_out)
+{
+    if (!m || !g_out) return LELPH_ERR_ARG;
+    for (size_t i = 0; i < m->ctx.size(); ++i)
+        if (m->gbytes[i] && !g_out[i]) return LELPH_ERR_ARG;
+    HIP_TRY(hipEventRecord(m->ev[0], m->stream));
+    HIP_TRY(hipGraphLaunch(m->exec, m->stream));
+    HIP_TRY(hipEventRecord(m->ev[3], m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    float a = 0;
+    HIP_TRY(hipEventElapsedTime(&a, m->ev[0], m->ev[3]));
+    m->ms[3] = a;
+    for (size_t i = 0; i < m->ctx.size(); ++i)
+    {
+        if (m->gbytes[i]) std::memcpy(g_out[i], m->ghost[i], m->gbytes[i]);
+        LELPH_TRY(check_err(m->ctx[i]));
+    }
+    return LELPH_OK;
+}
+
+int lelph_multi_timings(lelph_multi* m, double* ms)
+{
+    if (!m || !ms) return LELPH_ERR_ARG;
+    for (int k = 0; k < 4; ++k) ms[k] = m->ms[k];
+    ms[4] = (double)m->n64;
+    ms[5] = (double)m->n32;
+    ms[6] = (double)m->grid64;
+    ms[7] = (double)m->grid32;
+    return LELPH_OK;
+}
+
+void lelph_multi_destroy(lelph_multi* m) { multi_free(m); }
 
 // electronic_reps (src/symmetries/elec_reps.c:16-413)
 int lelph_electronic_reps(lelph_ctx* c, const double Rsym[9], const double tauR[3], int tim_revR,

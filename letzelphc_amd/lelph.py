@@ -20,14 +20,15 @@ import os
 import numpy as np
 
 from .abi import (KERNELS, PRECISIONS, Lattice, LongRange, LrParams, Pair, Pseudo, Symmetry,
-                  c_double_p, c_int_p, dptr, i64ptr, pack_system)
+                  Tuning, c_double_p, c_int_p, dptr, i64ptr, pack_system)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblelph_hip.so")
 
 EXPORTS = [
     "lelph_create", "lelph_destroy", "lelph_strerror", "lelph_set_precision",
-    "lelph_get_precision", "lelph_elph_q_rotate_noalloc",
+    "lelph_get_precision", "lelph_elph_q_rotate_noalloc", "lelph_set_tuning", "lelph_get_tuning",
+    "lelph_multi_create", "lelph_multi_run", "lelph_multi_timings", "lelph_multi_destroy",
     "lelph_get_device_count", "lelph_upload_wfc", "lelph_create_vlocg_table",
     "lelph_dvlocq", "lelph_set_dvscf", "lelph_get_dvscf", "lelph_elph_local",
     "lelph_add_elph_nonlocal", "lelph_gkq", "lelph_gkq_batch", "lelph_compute_elphq",
@@ -68,6 +69,15 @@ def lib():
         L.lelph_destroy.restype = None
         L.lelph_set_precision.argtypes = [C.c_void_p, C.c_int]
         L.lelph_get_precision.argtypes = [C.c_void_p, c_int_p, c_int_p]
+        L.lelph_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.lelph_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.lelph_multi_create.argtypes = [C.c_int, C.POINTER(C.c_void_p), c_double_p,
+                                         C.POINTER(C.c_int64), C.POINTER(C.POINTER(Pair)),
+                                         C.c_int, C.POINTER(C.c_void_p)]
+        L.lelph_multi_run.argtypes = [C.c_void_p, C.POINTER(c_double_p)]
+        L.lelph_multi_timings.argtypes = [C.c_void_p, c_double_p]
+        L.lelph_multi_destroy.argtypes = [C.c_void_p]
+        L.lelph_multi_destroy.restype = None
         L.lelph_get_device_count.argtypes = [C.POINTER(C.c_int)]
         L.lelph_upload_wfc.argtypes = [C.c_void_p, C.c_int64, c_double_p, c_double_p,
                                        c_double_p, C.c_int64]
@@ -199,6 +209,16 @@ class Context:
         _check(lib().lelph_get_precision(self.h, C.byref(req), C.byref(eff)),
                "lelph_get_precision")
         return {v: k for k, v in PRECISIONS.items()}[eff.value]
+
+    def set_tuning(self, split_target=0, fft_lines=0, fp32_form=0):
+        t = Tuning(int(split_target), int(fft_lines), int(fp32_form))
+        _check(lib().lelph_set_tuning(self.h, C.byref(t)), "lelph_set_tuning")
+
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().lelph_get_tuning(self.h, C.byref(t)), "lelph_get_tuning")
+        return {"split_target": t.split_target, "fft_lines": t.fft_lines,
+                "fp32_form": t.fp32_form}
 
     def upload_wfc(self, ik, wfc, gvec, Fk=None):
         wfc = np.ascontiguousarray(wfc, np.complex128)
@@ -475,6 +495,57 @@ class Context:
         _check(lib().lelph_last_timings(self.h, dptr(t), 6), "lelph_last_timings")
         return {"boxes_ms": t[0], "contract_ms": t[1], "reduce_ms": t[2],
                 "nonlocal_ms": t[3], "total_ms": t[4], "pairs": int(t[5])}
+
+
+class MultiBatch:
+    """lelph_multi: the (k, k+q) pairs of several contexts (cells) as one
+    unit - per-pair boxes, one persistent work-queue contraction launch per
+    precision, reductions and non-local parts - captured in a HIP graph at
+    construction and replayed by run().  jobs: [(Context, qpt, pairs)]."""
+
+    def __init__(self, jobs, non_loc=True):
+        self.jobs = jobs
+        n = len(jobs)
+        ctxs = (C.c_void_p * n)(*[j[0].h.value for j in jobs])
+        q = np.ascontiguousarray(np.array([np.asarray(j[1], np.float64) for j in jobs]))
+        npairs = (C.c_int64 * n)(*[len(j[2]) for j in jobs])
+        self._arrs = []
+        parr = (C.POINTER(Pair) * n)()
+        for i, (_, _, pairs) in enumerate(jobs):
+            a = (Pair * max(len(pairs), 1))()
+            for k, (ikq, ik, kqsym, ksym) in enumerate(pairs):
+                a[k].ikq, a[k].ik, a[k].kqsym, a[k].ksym = ikq, ik, kqsym, ksym
+            self._arrs.append(a)
+            parr[i] = C.cast(a, C.POINTER(Pair))
+        self._q = q
+        h = C.c_void_p()
+        _check(lib().lelph_multi_create(n, ctxs, dptr(q), npairs, parr, int(non_loc),
+                                        C.byref(h)), "lelph_multi_create")
+        self.h = h
+        self.out = [np.zeros((len(p),) + ctx._gshape(), np.complex128) for ctx, _, p in jobs]
+
+    def run(self):
+        ptrs = (c_double_p * len(self.out))(*[dptr(o) for o in self.out])
+        _check(lib().lelph_multi_run(self.h, ptrs), "lelph_multi_run")
+        return self.out
+
+    def timings(self) -> dict:
+        t = np.zeros(8)
+        _check(lib().lelph_multi_timings(self.h, dptr(t)), "lelph_multi_timings")
+        return {"boxes_ms": t[0], "contract_ms": t[1], "rest_ms": t[2], "replay_ms": t[3],
+                "items_fp64": int(t[4]), "items_fp32": int(t[5]), "grid_fp64": int(t[6]),
+                "grid_fp32": int(t[7])}
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().lelph_multi_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def elph_q_rotate(Dl, g, Dr, tim_rev: bool) -> np.ndarray:
