@@ -97,6 +97,73 @@ def cpu_baseline(sys_, pairs, dv, cores, modes=(2, 6)):
                        f"{sys_.nmodes} modes: {t_full:.1f} s per pair")}
 
 
+def learned_c4_measure(c0, ic, lrp):
+    """The learned stand-in on the c4 workload: ic is the c0 cell's context
+    with the coarse 4x4x4 interpolation (with long range) set up."""
+    from letzelphc_amd import learned, lelph, synth
+    li = learned.LearnedInterpolation(ic, (4, 4, 4), lrp, depth=5, seed=7)
+    try:
+        fine = learned.fine_grid(12)
+        on = np.all(np.abs(fine * 4 - np.rint(fine * 4)) < 1e-9, axis=1)
+        cand = np.flatnonzero(~on)
+        np.random.default_rng(5).shuffle(cand)
+        tq, vq = fine[cand[:512]], fine[cand[512:520]]
+        t0 = time.perf_counter()
+        hist = li.train(tq, epochs=8, batch=8, lr=3e-4, seed=3)
+        t_train = time.perf_counter() - t0
+        ref = li.periodic(vq)
+        t0 = time.perf_counter()
+        net = li.predict_periodic(vq)
+        t_pred = time.perf_counter() - t0
+        tri = li.trilinear_periodic(vq)
+
+        def rms(a, b):
+            return float(np.sqrt(np.mean(np.abs(a - b) ** 2) / np.mean(np.abs(b) ** 2)))
+        gs = synth.make_system(
+            "c0k12", 1000, synth.fcc_cell(10.26), [[0, 0, 0], [0.25, 0.25, 0.25]], [0, 0],
+            [4.0], (24, 24, 24), (12, 12, 12), 20.0, 8, [(np.eye(3), np.zeros(3), 0)], 0,
+            [q for q in vq], with_wfc=False)
+        kb = gs.kpt_bz_crys
+        pairs_q, need = [], set()
+        for q in vq:
+            prs = []
+            for i in (0, 137, 611):
+                d = kb - (kb[i] + q)[None, :]
+                d -= np.rint(d)
+                j = int(np.argmin(np.linalg.norm(d, axis=1)))
+                prs.append((j, i, 0, 0))
+                need |= {i, j}
+            pairs_q.append(prs)
+        synth.make_wfcs(gs, workers=4, only=sorted(need))
+        gctx = lelph.Context(gs, "fp64")
+        try:
+            eig = gs.eigvec(0)
+            dv_ref, dv_net = li.interpolated(vq), li.dv(vq, net)
+            dv_tri = li.dv(vq, tri)
+            g_net, g_tri, acc = [], [], 0
+            for i, q in enumerate(vq):
+                e, ok = learned.judge(gctx, gs, q, eig, dv_net[i], dv_ref[i], pairs_q[i])
+                g_net.append(e)
+                g_tri.append(learned.judge(gctx, gs, q, eig, dv_tri[i], dv_ref[i],
+                                           pairs_q[i])[0])
+                acc += ok
+        finally:
+            gctx.close()
+        return {
+            "workload": "c0 cell 24^3, coarse 4x4x4 q with Born-charge long range (eps 12); "
+                        "network trained on 512 fine 12x12x12 q (interpolation outputs), "
+                        "judged on 8 held-out fine q, 3 (k, k+q) pairs each on a 12^3 k grid",
+            "params": li.net.effective_params(), "train_s": t_train,
+            "loss_by_epoch": [float(x) for x in hist],
+            "predict_ms_per_q": t_pred / len(vq) * 1e3,
+            "heldout_rms_network": rms(net, ref), "heldout_rms_trilinear": rms(tri, ref),
+            "gain_over_trilinear": rms(tri, ref) / rms(net, ref),
+            "g_rel_err_network": g_net, "g_rel_err_trilinear": g_tri,
+            "accepted_at_1e-3": int(acc), "fallback_to_interpolation": len(vq) - int(acc)}
+    finally:
+        li.net.close()
+
+
 def autotune_key(ctx):
     from letzelphc_amd import autotune
     return autotune.shape_key(ctx)
@@ -175,6 +242,12 @@ def aux_measurements(s, ctx):
         out["dyn_interp_c4"] = {"coarse_side_s": t_dc, "fine_wall_s": t_df,
                                 "nq_fine": len(fine),
                                 "coarse_side_s_asr_all_huang": time.perf_counter() - t0}
+        # c4 learned form: the network trained on the device on 512 fine q
+        # the interpolation above produces, judged on held-out fine q
+        try:
+            out["learned_c4"] = learned_c4_measure(c0, ic, lr)
+        except Exception as e:  # never hides the rest
+            log(f"learned c4 failed: {e}")
     finally:
         ic.close()
     # c3: eight uneven cells, 16 pairs each, mixed FP64 / FP32: the

@@ -570,6 +570,53 @@ int lelph_interp_timings(lelph_ctx* ctx, double* ms, int n);
  * n: number of entries wanted (<= 8). */
 int lelph_last_timings(lelph_ctx* ctx, double* ms, int n);
 
+/* ------------------------------------------------------------------------
+ * Learned stand-in for the Fourier interpolation of dV_q onto fine q
+ * (src/interpolation/interpolation_driver.c:813-856, fft_R2q_dvscf
+ * src/interpolation/interpolation_utilities.c:165-238); no counterpart
+ * among the reference's functions.  A 3-D convolutional network on the FFT
+ * grid (3x3x3 taps, periodic; 128 input channels, width 64, depth + 2
+ * layers, SiLU), BF16 operands with FP32 accumulation on MFMA, FP32 master
+ * weights, Adam, trained on the device.  It predicts the short-range
+ * periodic part u_q(r) = e^{-2 pi i q.x} (dV_q(r) - dV_q^LR(r)) of nfields
+ * complex fields (cartesian basis) at fine q as a correction to the
+ * trilinear blend of the 8 coarse q around it; fields are laid out
+ * (q, point, 2 nfields) (channel fastest, re/im interleaved per field) on
+ * the host side of every call.  Nx*Ny*Nz must be a multiple of 64.
+ * ---------------------------------------------------------------------- */
+typedef struct lelph_nn lelph_nn;
+int lelph_nn_create(const int64_t grid[3], int nfields, int depth, uint64_t seed,
+                    lelph_nn** out);
+void lelph_nn_destroy(lelph_nn* nn);
+/* number of parameters (padding included), and the FP32 masters as
+ * [layer][W (27 cin, 64) then b (64)] */
+int lelph_nn_nparams(lelph_nn* nn, int64_t* n);
+int lelph_nn_get_params(lelph_nn* nn, float* params);
+int lelph_nn_set_params(lelph_nn* nn, const float* params);
+/* the coarse grid's short-range fields (nqc, nfields, N) complex128 in the
+ * cartesian basis (NOT multiplied by e^{-iqr}; the features do that per
+ * neighbour), and the scale every field is divided by inside the network */
+int lelph_nn_set_coarse(lelph_nn* nn, const int64_t qgrid[3], const double* fields,
+                        double scale);
+/* the network's input features X0 (S, N, 128) and trilinear base
+ * (S, N, 2 nfields) of S fine q (crystal), scaled; either output may be NULL */
+int lelph_nn_features(lelph_nn* nn, int S, const double* qpts, float* X0, float* base);
+/* forward only on given inputs (X0 rounded to BF16): out = base + net(X0),
+ * (S, N, 2 nfields), in the scaled units of the inputs */
+int lelph_nn_forward(lelph_nn* nn, int S, const float* X0, const float* base, float* out);
+/* mean-square loss of base + net(X0) against target (scaled units) and the
+ * gradient of every parameter (layout of lelph_nn_get_params; may be NULL) */
+int lelph_nn_loss_grad(lelph_nn* nn, int S, const float* X0, const float* base,
+                       const float* target, double* loss, float* grad);
+/* training data: (nq, N, 2 nfields) targets u_q, unscaled */
+int lelph_nn_set_targets(lelph_nn* nn, int64_t nq, const float* targets);
+/* Adam over the targets' q (crystal, nq rows matching lelph_nn_set_targets)
+ * in shuffled batches; loss_hist (epochs) receives each epoch's mean loss */
+int lelph_nn_train(lelph_nn* nn, int64_t nq, const double* qpts, int epochs, int batch,
+                   double lr, uint64_t seed, double* loss_hist);
+/* u_q of nq fine q, (nq, N, 2 nfields), unscaled */
+int lelph_nn_predict(lelph_nn* nn, int64_t nq, const double* qpts, float* out);
+
 #ifdef __cplusplus
 }
 #endif

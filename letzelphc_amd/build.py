@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblelph_hip.so")
 ARCH = os.environ.get("LELPH_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["lelph_hip.hip"]
+HIP_SOURCES = ["lelph_hip.hip", "learned.hip"]
 C_SOURCES = ["driver.c", "qeread.c", "asr.c", "phonon.c"]
 HIPCC_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -185,6 +185,10 @@ def _mp_grid(n):
 
 
 def _reduce_kgrid(kbz, syms, alat, blat):
+    if len(syms) == 1:  # the identity alone: every k is its own star
+        kmap = np.zeros((len(kbz), 2), np.int32)
+        kmap[:, 0] = np.arange(len(kbz))
+        return np.array(kbz), kmap
     ibz, kmap = [], np.zeros((len(kbz), 2), np.int32)
     Sc = [_sym_crys(np.asarray(R), alat, blat) for (R, _, _) in syms]
     for ik, k in enumerate(kbz):
