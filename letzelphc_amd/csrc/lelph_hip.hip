@@ -172,6 +172,8 @@ struct lelph_ctx
     hipEvent_t ip_ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double ip_ms[5] = {0, 0, 0, 0, 0};
     DBuf nlK, nlKp, Y_K, Y_Kp, wr_K, wr_Kp, beta_K, beta_Kp, bb_K, bb_Kp, bbpart, ebuf;
+    // reduced-precision projector operands: FP32 copies, or bf16 piece planes
+    DBuf beta32_K, beta32_Kp, wr32_K, wr32_Kp, nlsplitA, nlsplitB;
 
     // psi box reuse along a batch: the k+q box of the previous pair is the k
     // box of the next when their (iBZ k, symmetry) agree, up to the umklapp
@@ -599,6 +601,41 @@ static int run_local(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64*
 // non-local part of one pair on stream st; its last kernel adds into g and
 // waits for mode_after first when that is given (the local part of the same
 // pair, which writes g, runs on the context stream)
+static int effective_precision(const lelph_ctx* c);
+
+// one side's <beta|psi> projector GEMM into out (rows, ncol) from the beta /
+// psi pieces of k_nl_presplit on BF16 MFMA (NP pieces, NPROD products),
+// FP64 split-K partials and a fixed-order FP64 reduction
+template <int NP, int NPROD>
+static int nl_gemm_split(lelph_ctx* c, const z64* A, const z64* B, int rows, int64_t ncol,
+                         int64_t npw, int conjB, z64* out, hipStream_t st)
+{
+    const int Mp = (int)(cdiv(rows, 64) * 64), Np = (int)(cdiv(ncol, 64) * 64);
+    const int64_t Kp = cdiv(npw, 32) * 32;
+    LELPH_TRY(c->nlsplitA.ensure(sizeof(__bf16) * (size_t)NP * 2 * Mp * Kp));
+    LELPH_TRY(c->nlsplitB.ensure(sizeof(__bf16) * (size_t)NP * 2 * Np * Kp));
+    hipLaunchKernelGGL((k_nl_presplit<NP>), dim3((unsigned)cdiv(Kp, 256), (unsigned)Mp), dim3(256),
+                       0, st, A, rows, npw, Mp, Kp, 0, c->nlsplitA.as<__bf16>());
+    hipLaunchKernelGGL((k_nl_presplit<NP>), dim3((unsigned)cdiv(Kp, 256), (unsigned)Np), dim3(256),
+                       0, st, B, (int)ncol, npw, Np, Kp, conjB, c->nlsplitB.as<__bf16>());
+    HIP_TRY(hipGetLastError());
+    const int64_t nchunks = Kp / 32;
+    const int64_t tiles = (int64_t)(Mp / 64) * (Np / 64);
+    int64_t cps;
+    int nsplit;
+    pick_splits(tiles, nchunks, (int64_t)rows * ncol, &cps, &nsplit);
+    const size_t need = sizeof(z64) * (size_t)nsplit * rows * ncol;
+    if (need > c->bbpart.n && st != c->stream) HIP_TRY(hipStreamSynchronize(st));
+    LELPH_TRY(c->bbpart.ensure(need));
+    hipLaunchKernelGGL((k_zgemm_nt_bf<NP, NPROD>), dim3((unsigned)tiles, (unsigned)nsplit),
+                       dim3(256), 0, st, c->nlsplitA.as<__bf16>(), c->nlsplitB.as<__bf16>(), rows,
+                       (int)ncol, Mp, Np, Kp, cps, c->bbpart.as<z64>());
+    hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)cdiv((int64_t)rows * ncol, 256)), dim3(256),
+                       0, st, c->bbpart.as<z64>(), nsplit, (int64_t)rows * ncol, 1.0, out);
+    HIP_TRY(hipGetLastError());
+    return LELPH_OK;
+}
+
 static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t st,
                         hipEvent_t mode_after)
 {
@@ -657,7 +694,54 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
                        c->beta_Kp.as<z64>());
     HIP_TRY(hipGetLastError());
     // projector GEMMs
-    for (int side = 0; side < 2; ++side)
+    const int prec = effective_precision(c);
+    for (int side = 0; side < 2 && prec != LELPH_FP64; ++side)
+    {
+        const DevWfc& w = side == 0 ? wK : wKp;
+        const z64* A = side == 0 ? c->beta_K.as<z64>() : c->beta_Kp.as<z64>();
+        const z64* B = side == 0 ? c->wr_K.as<z64>() : c->wr_Kp.as<z64>();
+        z64* out = side == 0 ? c->bb_K.as<z64>() : c->bb_Kp.as<z64>();
+        const int conjB = side == 0 ? sK.conj_w : sKp.conj_w;
+        // both split precisions take the 3-piece, 6-product form here: the
+        // 2-piece form left 8.5e-6 of |g_NL|max on the non-local part alone
+        // (80-band cell), against 2.5e-7 for 3 pieces, at a cost that is
+        // small next to the local contraction
+        if (prec == LELPH_BF16X3 || prec == LELPH_BF16X6)
+            LELPH_TRY((nl_gemm_split<3, 6>(c, A, B, rows, ncol, w.npw, conjB, out, st)));
+        else
+        {
+            // FP32: beta again from the FP64 tables, stored in FP32; psi in FP32
+            DBuf& b32 = side == 0 ? c->beta32_K : c->beta32_Kp;
+            DBuf& w32 = side == 0 ? c->wr32_K : c->wr32_Kp;
+            LELPH_TRY(b32.ensure(sizeof(c32) * (size_t)rows * npm));
+            LELPH_TRY(w32.ensure(sizeof(c32) * (size_t)ncol * npm));
+            hipLaunchKernelGGL(k_nl_beta<float>,
+                               dim3((unsigned)cdiv(w.npw, 256), natom * c->pmax), dim3(256), 0, st,
+                               side == 0 ? c->nlK.as<double>() : c->nlKp.as<double>(),
+                               side == 0 ? c->Y_K.as<double>() : c->Y_Kp.as<double>(), w.Fk,
+                               w.npw, pj, c->d_plist, c->d_atype, c->d_apos, c->d_Fsign,
+                               side == 0 ? -1 : +1, b32.as<c32>());
+            hipLaunchKernelGGL(k_z64_to_c32, dim3((unsigned)cdiv(ncol * w.npw, 256)), dim3(256), 0,
+                               st, B, w32.as<c32>(), ncol * w.npw);
+            HIP_TRY(hipGetLastError());
+            const int64_t nchunks = cdiv(w.npw, CT_KC);
+            const int64_t tiles = cdiv(rows, GZ_MB) * cdiv(ncol, GZ_NB);
+            int64_t cps;
+            int nsplit;
+            pick_splits(tiles, nchunks, (int64_t)rows * ncol, &cps, &nsplit);
+            const size_t need = sizeof(z64) * (size_t)nsplit * rows * ncol;
+            if (need > c->bbpart.n && st != c->stream) HIP_TRY(hipStreamSynchronize(st));
+            LELPH_TRY(c->bbpart.ensure(need));
+            hipLaunchKernelGGL(k_zgemm_nt<float>, dim3((unsigned)tiles, (unsigned)nsplit),
+                               dim3(256), 0, st, b32.as<c32>(), w32.as<c32>(), rows, (int)ncol,
+                               w.npw, conjB, cps, c->bbpart.as<z64>());
+            hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)cdiv((int64_t)rows * ncol, 256)),
+                               dim3(256), 0, st, c->bbpart.as<z64>(), nsplit,
+                               (int64_t)rows * ncol, 1.0, out);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    for (int side = 0; side < 2 && prec == LELPH_FP64; ++side)
     {
         const DevWfc& w = side == 0 ? wK : wKp;
         const z64* A = side == 0 ? c->beta_K.as<z64>() : c->beta_Kp.as<z64>();

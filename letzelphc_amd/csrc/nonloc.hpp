@@ -242,6 +242,136 @@ __global__ __launch_bounds__(256) void k_zgemm_nt(const cpx<T>* __restrict__ A,
             }
 }
 
+// Reduced-precision projector GEMMs (the FP32 and BF16-split forms of the
+// <beta|psi> products, src/nonloc/Vbare_nloc.c:409-418).  beta (phases and
+// radial / Y_lm tables in FP64) and the rotated wavefunctions are split into
+// NP bf16 pieces per real component, x = x0 + x1 (+ x2), each the
+// round-to-nearest of the remaining FP32 residual, stored as planes
+// P[piece][re/im][row][k] with k contiguous and zero padded to Kp (a
+// multiple of 32) and rows to Rp (a multiple of 64); negim flips the sign of
+// the imaginary planes (the conjugate operand).
+template <int NP>
+__global__ void k_nl_presplit(const z64* __restrict__ X, int R, int64_t K, int Rp, int64_t Kp,
+                              int negim, __bf16* __restrict__ out)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (k >= Kp || r >= Rp) return;
+    z64 v = {0, 0};
+    if (r < R && k < K) v = X[(int64_t)r * K + k];
+    float re = (float)v.x, im = (float)(negim ? -v.y : v.y);
+    const int64_t plane = (int64_t)Rp * Kp;
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+    {
+        const __bf16 a = (__bf16)re, b = (__bf16)im;
+        out[(2 * p) * plane + (int64_t)r * Kp + k] = a;
+        out[(2 * p + 1) * plane + (int64_t)r * Kp + k] = b;
+        re -= (float)a;
+        im -= (float)b;
+    }
+}
+
+typedef __bf16 nl_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float nl_flt4 __attribute__((ext_vector_type(4)));
+
+__device__ inline nl_bf16x8 nl_neg(nl_bf16x8 x)
+{
+    typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+    u16x8 u = __builtin_bit_cast(u16x8, x);
+    u ^= (unsigned short)0x8000;
+    return __builtin_bit_cast(nl_bf16x8, u);
+}
+
+// C[M][N] (partial of split s) = sum_k A[m][k] B[n][k] over the split's k
+// range, complex, from the piece planes of k_nl_presplit (B's imaginary
+// planes already carry the conjugation sign): Cr = Ar Br - Ai Bi, Ci = Ar Bi
+// + Ai Br, each real product as the piece products p1 + p2 < NP (NPROD 3:
+// p1 + p2 <= 1) on v_mfma_f32_16x16x32_bf16, FP32 accumulation within the
+// split, partials in FP64.  Block 64 x 64, 4 waves of 32 x 32; fragments are
+// read straight from global memory (16 bytes per lane: A lane l holds row
+// l % 16, k 8 (l / 16) .. + 7; B the same for column l % 16).
+template <int NP, int NPROD>
+__global__ __launch_bounds__(256) void k_zgemm_nt_bf(const __bf16* __restrict__ Ap,
+                                                     const __bf16* __restrict__ Bp, int M,
+                                                     int Nc, int Mp, int Np, int64_t Kp,
+                                                     int64_t chunks_per_split,
+                                                     z64* __restrict__ part)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ntn = Np / 64;
+    const int tm = blockIdx.x / ntn, tn = blockIdx.x % ntn;
+    const int m0 = tm * 64 + (wave >> 1) * 32, n0 = tn * 64 + (wave & 1) * 32;
+    const int split = blockIdx.y;
+    const int64_t nch = Kp / 32;
+    const int64_t c0 = (int64_t)split * chunks_per_split;
+    const int64_t c1 = (c0 + chunks_per_split < nch) ? c0 + chunks_per_split : nch;
+    const int64_t pa = (int64_t)Mp * Kp, pb = (int64_t)Np * Kp;
+    nl_flt4 accr[2][2], acci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+        {
+            accr[a][b] = (nl_flt4){0.f, 0.f, 0.f, 0.f};
+            acci[a][b] = (nl_flt4){0.f, 0.f, 0.f, 0.f};
+        }
+    const int r = lane & 15, kq = 8 * (lane >> 4);
+    for (int64_t c = c0; c < c1; ++c)
+    {
+        const int64_t k = c * 32 + kq;
+        nl_bf16x8 ar[NP][2], ai[NP][2], nai[NP][2], br[NP][2], bi[NP][2];
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+            {
+                const int64_t ra = (int64_t)(m0 + 16 * h + r) * Kp + k;
+                const int64_t rb = (int64_t)(n0 + 16 * h + r) * Kp + k;
+                ar[p][h] = *reinterpret_cast<const nl_bf16x8*>(Ap + (2 * p) * pa + ra);
+                ai[p][h] = *reinterpret_cast<const nl_bf16x8*>(Ap + (2 * p + 1) * pa + ra);
+                nai[p][h] = nl_neg(ai[p][h]);
+                br[p][h] = *reinterpret_cast<const nl_bf16x8*>(Bp + (2 * p) * pb + rb);
+                bi[p][h] = *reinterpret_cast<const nl_bf16x8*>(Bp + (2 * p + 1) * pb + rb);
+            }
+#pragma unroll
+        for (int p1 = 0; p1 < NP; ++p1)
+#pragma unroll
+            for (int p2 = 0; p2 < NP; ++p2)
+            {
+                if (p1 + p2 >= NP) continue;
+                if (NPROD == 3 && p1 + p2 > 1) continue;
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+                    {
+                        accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[p1][a], br[p2][b],
+                                                                            accr[a][b], 0, 0, 0);
+                        accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nai[p1][a], bi[p2][b],
+                                                                            accr[a][b], 0, 0, 0);
+                        acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar[p1][a], bi[p2][b],
+                                                                            acci[a][b], 0, 0, 0);
+                        acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai[p1][a], br[p2][b],
+                                                                            acci[a][b], 0, 0, 0);
+                    }
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+                const int row = m0 + 16 * a + 4 * (lane >> 4) + v;
+                const int col = n0 + 16 * b + r;
+                if (row < M && col < Nc)
+                    part[((int64_t)split * M + row) * Nc + col] = {(double)accr[a][b][v],
+                                                                   (double)acci[a][b][v]};
+            }
+}
+
 // ebuf[a][x][s][n][m] = sum f (K3_x[m] Kp0[n] + K0[m] Kp3_x[n])
 // bbK/bbKp rows (a*pmax+p)*4+x, columns (s*nb + b)*nspinor + sigma.
 __global__ void k_nl_kkp(const z64* __restrict__ bbK, const z64* __restrict__ bbKp,

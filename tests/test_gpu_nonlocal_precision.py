@@ -1,0 +1,49 @@
+"""The Kleinman-Bylander part in the reduced precisions: the <beta|psi>
+projector GEMMs in FP32 (FP32 MFMA) and BF16-split (bf16x3 / bf16x6 pieces on
+BF16 MFMA; both split precisions use 3 pieces and 6 products here), beta's
+phases and radial / Y_lm tables and the K/K' sums and mode rotation in
+FP64.  The non-local part alone against the oracle's,
+and the proof that the reduced path ran (its error is above the FP64
+path's).  Cells: 80 bands (bf16x3 runs as bf16x3), 8 bands (bf16x3 runs as
+bf16x6), spinors."""
+import numpy as np
+import pytest
+
+from letzelphc_amd import lelph, synth
+from oracle import oracle as O
+
+from .util import rel_err, say
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "nb80": dict(nbnds=80, ecut=14.0, fft=(16, 16, 16)),
+    "nb8": dict(nbnds=8),
+    "spinor": dict(nbnds=4, nspinor=2, nmag=1),
+    "cubic_syms": dict(nbnds=6, syms="cubic", q=(0.0, 0.5, 0.0)),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_nonlocal_reduced(case):
+    s = synth.tiny(**CASES[case])
+    o = O.OracleSystem(s)
+    pairs = synth.kq_pairs(s, 0)[:4]
+    z = np.zeros((s.nmodes, s.nspin, s.nbnds, s.nbnds), np.complex128)
+    refs = [o.add_nonlocal(0, *p, z) for p in pairs]
+    ctx = lelph.Context(s, "fp64")
+    try:
+        ctx.set_dvscf(s.qpts[0], s.eigvec(0), s.dvscf(0), kernel="dfpt")
+        errs = {}
+        for prec in ("fp64", "fp32", "bf16x6", "bf16x3"):
+            ctx.set_precision(prec)
+            e = max(rel_err(ctx.add_nonlocal(*p, z), r) for p, r in zip(pairs, refs))
+            errs[prec] = (ctx.effective_precision(), e)
+        say(f"case: non-local part alone, {case}: " +
+            ", ".join(f"{k} (as {v[0]}) {v[1]:.2e}" for k, v in errs.items()))
+        assert errs["fp64"][1] < 1e-10
+        for prec in ("fp32", "bf16x6", "bf16x3"):
+            assert errs[prec][1] < 1e-6, prec
+            assert errs[prec][1] > 1e-13, prec  # the reduced path ran
+    finally:
+        ctx.close()
