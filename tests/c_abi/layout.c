@@ -59,6 +59,10 @@ int main(void)
     F(lelph_pair, ik);
     F(lelph_pair, kqsym);
     F(lelph_pair, ksym);
+    Z(lelph_tuning);
+    F(lelph_tuning, split_target);
+    F(lelph_tuning, fft_lines);
+    F(lelph_tuning, fp32_form);
     Z(lelph_longrange);
     F(lelph_longrange, epsilon);
     F(lelph_longrange, Zborn);

@@ -22,6 +22,7 @@ MIRRORS = {
     "lelph_pair": abi.Pair,
     "lelph_longrange": abi.LongRange,
     "lelph_qexml_info": lelph.QexmlInfo,
+    "lelph_tuning": abi.Tuning,
 }
 
 
