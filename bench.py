@@ -271,6 +271,16 @@ def aux_measurements(s, ctx):
         t_multi, g_multi, _ = batch.run_multi(jobs, reps=reps, multi=multi)
         same = all(np.array_equal(a, b) for a, b in zip(g_multi, g_pair))
         nel = batch.g_elements(jobs)
+        # the four small cells alone (24^3 .. 45x45x108), where launches
+        # rather than arithmetic set the pace
+        small = jobs[:4]
+        t_seq_s = min(batch.run_sequential(small)[0] for _ in range(reps))
+        ms_ = lelph.MultiBatch(small)
+        try:
+            t_multi_s, g_ms, _ = batch.run_multi(small, reps=reps, multi=ms_)
+        finally:
+            ms_.close()
+        nel_s = batch.g_elements(small)
         out["c3_batching"] = {
             "workload": "8 cells: FFT 24^3, 32^3, 36^3, 45x45x108, 48^3, 60^3, 64^3, 72^3; "
                         "npw 400..12,000, 8..64 bands, 6..36 modes, 16 pairs each, cells "
@@ -280,6 +290,9 @@ def aux_measurements(s, ctx):
             "speedup_vs_per_cell": t_seq / t_multi, "speedup_vs_per_pair": t_pair / t_multi,
             "multi_bit_identical_to_per_pair": same, "multi_build_s": t_build,
             "multi_timings_ms": {k: float(v) for k, v in multi.timings().items()},
+            "small_cells_0_3": {"multi_replay_s": t_multi_s, "per_cell_batches_s": t_seq_s,
+                                "speedup_vs_per_cell": t_seq_s / t_multi_s,
+                                "multi_g_per_s": nel_s / t_multi_s},
             "autotune_s": t_tune,
             "tuning": {autotune_key(c): r["tuning"] for (c, _, _), r in zip(jobs, tuned)}}
     finally:

@@ -636,8 +636,24 @@ static int nl_gemm_split(lelph_ctx* c, const z64* A, const z64* B, int rows, int
     return LELPH_OK;
 }
 
+// the cart -> mode rotation of the non-local part, accumulated into g
+// (src/nonloc/Vbare_nloc.c:541-558): g += -2 pi i e^2 eig ebuf
+static int nl_mode(lelph_ctx* c, const z64* ebuf, z64* g, hipStream_t st)
+{
+    const lelph_lattice& L = c->lat;
+    const int64_t es = (int64_t)L.nspin * L.nbnds * L.nbnds;
+    const z64 pre = {0.0, -2 * LELPH_RPI * LELPH_E2};
+    hipLaunchKernelGGL(k_nl_mode, dim3((unsigned)cdiv((int64_t)L.nmodes * es, 256)), dim3(256), 0,
+                       st, ebuf, c->eig_d.as<z64>(), L.nmodes, 3 * L.natom, es, pre, g);
+    HIP_TRY(hipGetLastError());
+    return LELPH_OK;
+}
+
+// add_elphNonLocal into g on stream st; the final mode rotation waits for
+// mode_after when given.  With ebuf_out the per-atom K/K' sums go there and
+// the rotation is left to the caller (nl_mode), so g is not touched.
 static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t st,
-                        hipEvent_t mode_after)
+                        hipEvent_t mode_after, z64* ebuf_out = nullptr)
 {
     const lelph_lattice& L = c->lat;
     if (!c->dv_set) return LELPH_ERR_STATE;
@@ -767,17 +783,15 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
         HIP_TRY(hipGetLastError());
     }
     const int64_t ne = (int64_t)natom * 3 * es;
+    z64* eb = ebuf_out ? ebuf_out : c->ebuf.as<z64>();
     hipLaunchKernelGGL(k_nl_kkp, dim3((unsigned)cdiv(ne, 256)), dim3(256), 0, st,
                        c->bb_K.as<z64>(), c->bb_Kp.as<z64>(), pj, c->d_plist, c->d_atype,
                        c->d_socfac, c->d_fco, c->has_fco ? c->d_fco_off : nullptr, L.nspin,
-                       L.nbnds, L.nspinor, c->ebuf.as<z64>());
-    const z64 pre = {0.0, -2 * LELPH_RPI * LELPH_E2};
-    if (mode_after) HIP_TRY(hipStreamWaitEvent(st, mode_after, 0));
-    hipLaunchKernelGGL(k_nl_mode, dim3((unsigned)cdiv((int64_t)L.nmodes * es, 256)), dim3(256), 0,
-                       st, c->ebuf.as<z64>(), c->eig_d.as<z64>(), L.nmodes, 3 * natom, es,
-                       pre, g);
+                       L.nbnds, L.nspinor, eb);
     HIP_TRY(hipGetLastError());
-    return LELPH_OK;
+    if (ebuf_out) return LELPH_OK;
+    if (mode_after) HIP_TRY(hipStreamWaitEvent(st, mode_after, 0));
+    return nl_mode(c, eb, g, st);
 }
 
 // local part with split-product bf16 MFMA (split.hpp)
@@ -1717,7 +1731,7 @@ struct MultiPair
     int sep = 0;
     int nsplit = 1;
     int64_t cps = 1, gsz = 0, tiles = 0;
-    DBuf box_k, box_kq, part;
+    DBuf box_k, box_kq, part, ebuf;
 };
 } // namespace
 
@@ -1738,6 +1752,10 @@ struct lelph_multi
     int n64 = 0, n32 = 0, grid64 = 0, grid32 = 0;
     bool sep32 = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // the non-local parts run on a second stream: forked at the start,
+    // their mode rotations wait for the reductions, joined before the copy
+    hipStream_t nl = nullptr;
+    hipEvent_t fork = nullptr, reduced = nullptr, joined = nullptr;
     double ms[4] = {0, 0, 0, 0};
 };
 
@@ -1763,6 +1781,17 @@ static int multi_enqueue(lelph_multi* m, bool timed)
     int st = LELPH_OK;
     auto body = [&]() -> int {
         if (timed) HIP_TRY(hipEventRecord(m->ev[0], m->stream));
+        // the non-local G-space work of every pair (its own buffers and the
+        // device wavefunctions) on the second stream, beside the boxes and
+        // contractions
+        if (m->non_loc)
+        {
+            HIP_TRY(hipEventRecord(m->fork, m->stream));
+            HIP_TRY(hipStreamWaitEvent(m->nl, m->fork, 0));
+            for (auto& p : m->pairs)
+                LELPH_TRY(run_nonlocal(m->ctx[p.cell], p.pr, nullptr, m->nl, nullptr,
+                                       p.ebuf.as<z64>()));
+        }
         // every pair's psi_k and psi_k+q boxes into its own slots
         for (auto& p : m->pairs)
         {
@@ -1795,7 +1824,8 @@ static int multi_enqueue(lelph_multi* m, bool timed)
         }
         HIP_TRY(hipGetLastError());
         if (timed) HIP_TRY(hipEventRecord(m->ev[2], m->stream));
-        // fixed-order reductions, the non-local parts, the copy-out
+        // fixed-order reductions, then the non-local mode rotations (second
+        // stream, after the reductions), joined, then the copy-out
         for (auto& p : m->pairs)
         {
             lelph_ctx* c = m->ctx[p.cell];
@@ -1804,7 +1834,16 @@ static int multi_enqueue(lelph_multi* m, bool timed)
                                m->stream, p.part.as<z64>(), p.nsplit, p.gsz, 1.0 / (double)c->nf,
                                g);
             HIP_TRY(hipGetLastError());
-            if (m->non_loc) LELPH_TRY(run_nonlocal(c, p.pr, g, m->stream, nullptr));
+        }
+        if (m->non_loc)
+        {
+            HIP_TRY(hipEventRecord(m->reduced, m->stream));
+            HIP_TRY(hipStreamWaitEvent(m->nl, m->reduced, 0));
+            for (auto& p : m->pairs)
+                LELPH_TRY(nl_mode(m->ctx[p.cell], p.ebuf.as<z64>(),
+                                  m->gdev[p.cell].as<z64>() + p.index * p.gsz, m->nl));
+            HIP_TRY(hipEventRecord(m->joined, m->nl));
+            HIP_TRY(hipStreamWaitEvent(m->stream, m->joined, 0));
         }
         for (size_t i = 0; i < m->ctx.size(); ++i)
             if (m->gbytes[i])
@@ -1897,11 +1936,13 @@ static void multi_free(lelph_multi* m)
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     if (m->exec) (void)hipGraphExecDestroy(m->exec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
+    if (m->nl) (void)hipStreamSynchronize(m->nl);
     for (auto& p : m->pairs)
     {
         p.box_k.release();
         p.box_kq.release();
         p.part.release();
+        p.ebuf.release();
     }
     for (auto& g : m->gdev) g.release();
     for (void* h : m->ghost)
@@ -1913,6 +1954,9 @@ static void multi_free(lelph_multi* m)
     m->counter.release();
     for (auto& e : m->ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {m->fork, m->reduced, m->joined})
+        if (e) (void)hipEventDestroy(e);
+    if (m->nl) (void)hipStreamDestroy(m->nl);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
@@ -1958,6 +2002,11 @@ int lelph_multi_create(int ncells, lelph_ctx* const* ctxs, const double* qpts,
         return fail(LELPH_ERR_HIP);
     for (auto& e : m->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(LELPH_ERR_HIP);
+    if (hipStreamCreateWithFlags(&m->nl, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&m->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&m->reduced, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&m->joined, hipEventDisableTiming) != hipSuccess)
+        return fail(LELPH_ERR_HIP);
     for (int i = 0; i < ncells; ++i)
     {
         lelph_ctx* c = ctxs[i];
@@ -1989,8 +2038,10 @@ int lelph_multi_create(int ncells, lelph_ctx* const* ctxs, const double* qpts,
             p.gsz = gsz;
             p.tiles = tiles;
             pick_splits(tiles * L.nspin, nchunks, gsz, &p.cps, &p.nsplit, c->tune.split_target);
+            const int64_t es = (int64_t)L.nspin * L.nbnds * L.nbnds;
             if (p.box_k.ensure(bb) || p.box_kq.ensure(bb) ||
-                p.part.ensure(sizeof(z64) * (size_t)(p.nsplit * gsz)))
+                p.part.ensure(sizeof(z64) * (size_t)(p.nsplit * gsz)) ||
+                (non_loc && p.ebuf.ensure(sizeof(z64) * (size_t)(3 * L.natom * es))))
             {
                 m->pairs.push_back(p);
                 return fail(LELPH_ERR_ALLOC);
