@@ -158,3 +158,93 @@ def run_elph(save_dir, ph_save_dir, out_dir=".", kernel="dfpt", kminusq=False,
     finally:
         ctx.close()
     return p_elph, p_dmat
+
+
+# ---------------------------------------------------------------------------
+# the input file (elph.in) and the command line
+# ---------------------------------------------------------------------------
+KERNELS = ("dfpt", "dfpt_local", "bare", "bare_local")
+
+
+def _strip_quotes(v: str) -> str:
+    """strip_quotes_in_string (src/common/string_func.c:329): a value in
+    matching single or double quotes loses them and the blanks inside"""
+    if len(v) > 1 and v[0] in "\"'" and v[-1] == v[0]:
+        return v[1:-1].strip()
+    return v
+
+
+def read_elph_input(path) -> dict:
+    """read_elph_input_file (src/parser/elph_input_parser.c:22-161): INI
+    lines `name = value` (';' and '#' comments, sections ignored), the keys
+    and defaults of the reference: nkpool 1, nqpool 1, start_bnd 0,
+    end_bnd 0 (0: every band), save_dir SAVE, ph_save_dir ph_save, kernel
+    dfpt (lower-cased), convention standard (k+q) or yambo (k-q).  An empty
+    value, an unknown key or another convention is an error, as there."""
+    out = {"nkpool": 1, "nqpool": 1, "start_bnd": 0, "end_bnd": 0, "save_dir": "SAVE",
+           "ph_save_dir": "ph_save", "kernel": "dfpt", "kminusq": False}
+    with open(path) as f:
+        for ln, raw in enumerate(f, 1):
+            line = raw.strip()
+            if not line or line[0] in ";#" or (line[0] == "[" and line.endswith("]")):
+                continue
+            for c in (" ;", " #"):  # inline comments
+                if c in line:
+                    line = line[:line.index(c)].rstrip()
+            if "=" not in line and ":" not in line:
+                raise ValueError(f"{path}:{ln}: not a `name = value` line")
+            sep = "=" if "=" in line else ":"
+            name, value = (x.strip() for x in line.split(sep, 1))
+            if not value.replace(" ", ""):
+                raise ValueError(f"Invalid input for {name}")
+            if name in ("nkpool", "nqpool", "start_bnd", "end_bnd"):
+                out[name] = int(value)
+            elif name in ("save_dir", "ph_save_dir"):
+                out[name] = _strip_quotes(value)
+            elif name == "kernel":
+                out["kernel"] = _strip_quotes(value).lower()
+            elif name == "convention":
+                if "yambo" in value:
+                    out["kminusq"] = True
+                elif "standard" in value:
+                    out["kminusq"] = False
+                else:
+                    raise ValueError("Invalid value for convention in the input file. "
+                                     "Only yambo or standard are accepted.")
+            else:
+                raise ValueError("Invalid variable in the elph input file.")
+    if out["kernel"] not in KERNELS:
+        raise ValueError(f"unknown kernel {out['kernel']!r}; one of {KERNELS}")
+    return out
+
+
+def run_from_input(path, out_dir=".", precision="fp64"):
+    """elph_driver from an elph.in: paths relative to the input file's
+    directory are taken as given (the reference reads them relative to the
+    working directory); nkpool / nqpool are pools of MPI ranks there and map
+    to GPUs here (one GPU per process), so a single process ignores them."""
+    inp = read_elph_input(path)
+    sb = inp["start_bnd"] if inp["start_bnd"] > 0 else -1
+    eb = inp["end_bnd"] if inp["end_bnd"] > 0 else -1
+    return run_elph(inp["save_dir"], inp["ph_save_dir"], out_dir, kernel=inp["kernel"],
+                    kminusq=inp["kminusq"], start_band=sb, end_band=eb, precision=precision)
+
+
+def main(argv=None):
+    """`python -m letzelphc_amd.driver -F elph.in` (the reference's `lelphc
+    -F elph.in`, src/preprocessor/cli.c:14-80); --precision picks the
+    contraction precision of this library."""
+    import argparse
+    ap = argparse.ArgumentParser(prog="letzelphc_amd.driver")
+    ap.add_argument("-F", dest="input", required=True, help="elph input file")
+    ap.add_argument("--out", default=".", help="directory of ndb.elph and ndb.Dmats")
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32", "bf16x3", "bf16x6"))
+    args = ap.parse_args(argv)
+    p_elph, p_dmat = run_from_input(args.input, args.out, args.precision)
+    print(p_elph)
+    print(p_dmat)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

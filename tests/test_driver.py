@@ -7,6 +7,7 @@ import pytest
 from letzelphc_amd import driver, save, synth
 
 from .test_save import _write_both
+from .util import say
 
 
 @pytest.fixture(scope="module")
@@ -72,3 +73,24 @@ def test_unfold_star_indices(cubic):
             assert np.allclose(d, np.rint(d), atol=1e-9)
         assert len(calls) == (int(ph.nqstar[iq]) - 1) * nk
     assert len(seen) == (len(ph.qpts_bz) - len(ph.qpts_ibz)) * nk
+
+
+def test_read_elph_input(tmp_path):
+    """the reference's elph.in keys, defaults, quotes and errors
+    (src/parser/elph_input_parser.c:22-161)"""
+    from letzelphc_amd import driver
+    p = tmp_path / "elph.in"
+    p.write_text("# comment\n[elph]\nnkpool = 2\nstart_bnd = 3\nend_bnd=10\n"
+                 "save_dir = \" ../my SAVE \"\nph_save_dir='phs'\nkernel = DFPT_Local\n"
+                 "convention = yambo\n")
+    d = driver.read_elph_input(p)
+    say("case: elph.in parsing")
+    assert d == {"nkpool": 2, "nqpool": 1, "start_bnd": 3, "end_bnd": 10,
+                 "save_dir": "../my SAVE", "ph_save_dir": "phs", "kernel": "dfpt_local",
+                 "kminusq": True}
+    p.write_text("")
+    assert driver.read_elph_input(p)["save_dir"] == "SAVE"
+    for bad in ("convention = other\n", "unknown = 1\n", "kernel =   \n"):
+        p.write_text(bad)
+        with pytest.raises(ValueError):
+            driver.read_elph_input(p)

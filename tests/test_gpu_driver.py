@@ -66,3 +66,26 @@ def test_run_elph_end_to_end(tmp_path):
                 Sk = save.find_kidx_in_list(r.kpt_bz_crys, r.alat.T @ (R @ kc))
                 want = lelph.elph_q_rotate(Do[isym, kq], gq[k], Do[isym, k], tr)
                 assert rel_err(G[qs, Sk], want) < 1e-10, (iq, istar, k)
+
+
+def test_cli_from_input_file(tmp_path):
+    """`python -m letzelphc_amd.driver -F elph.in` (the reference's lelphc
+    -F elph.in) with the yambo convention and a band window gives the same
+    ndb.elph as run_elph called with those arguments"""
+    s = synth.tiny(syms="cubic", q=(0.0, 0.5, 0.0), nbnds=4)
+    masses = np.array([28.0855, 15.999])[np.asarray(s.atom_type)]
+    SF.write_inputs(tmp_path, s, (2, 2, 2), masses)
+    inp = tmp_path / "elph.in"
+    inp.write_text(f"save_dir = '{tmp_path / 'SAVE'}'\nph_save_dir = {tmp_path / 'ph_save'}\n"
+                   "kernel = dfpt\nconvention = yambo\nstart_bnd = 2\nend_bnd = 4\n")
+    say("case: driver command line from elph.in")
+    assert driver.main(["-F", str(inp), "--out", str(tmp_path / "cli")]) == 0
+    p2, _ = driver.run_elph(tmp_path / "SAVE", tmp_path / "ph_save", tmp_path / "direct",
+                            kminusq=True, start_band=2, end_band=4)
+    with ndb.NC4File(tmp_path / "cli" / "ndb.elph", "r") as f:
+        a = f.get("elph_mat")
+    with ndb.NC4File(p2, "r") as f:
+        b = f.get("elph_mat")
+    assert a.shape == b.shape and a.shape[-3] == 3
+    assert np.array_equal(a, b)
+    say(f"  ndb.elph {a.shape} identical to run_elph's")
