@@ -263,13 +263,15 @@ def aux_measurements(s, ctx):
         batch.run_sequential(jobs)  # warm-up (allocations)
         reps = 3
         t_seq = min(batch.run_sequential(jobs)[0] for _ in range(reps))
+        g_seq = batch.run_sequential(jobs)[1]
         batch.run_pairwise(jobs)
         t_pair, g_pair = batch.run_pairwise(jobs)
         t0 = time.perf_counter()
         multi = lelph.MultiBatch(jobs)
         t_build = time.perf_counter() - t0
         t_multi, g_multi, _ = batch.run_multi(jobs, reps=reps, multi=multi)
-        same = all(np.array_equal(a, b) for a, b in zip(g_multi, g_pair))
+        same = all(np.array_equal(a, b) for a, b in zip(g_multi, g_seq))
+        err_pair = max(rel_err(a, b) for a, b in zip(g_multi, g_pair))
         nel = batch.g_elements(jobs)
         # the four small cells alone (24^3 .. 45x45x108), where launches
         # rather than arithmetic set the pace
@@ -288,7 +290,8 @@ def aux_measurements(s, ctx):
             "multi_replay_s": t_multi, "per_cell_batches_s": t_seq, "per_pair_s": t_pair,
             "multi_g_per_s": nel / t_multi, "per_cell_g_per_s": nel / t_seq,
             "speedup_vs_per_cell": t_seq / t_multi, "speedup_vs_per_pair": t_pair / t_multi,
-            "multi_bit_identical_to_per_pair": same, "multi_build_s": t_build,
+            "multi_bit_identical_to_per_cell": same, "multi_rel_err_vs_per_pair": err_pair,
+            "multi_build_s": t_build,
             "multi_timings_ms": {k: float(v) for k, v in multi.timings().items()},
             "small_cells_0_3": {"multi_replay_s": t_multi_s, "per_cell_batches_s": t_seq_s,
                                 "speedup_vs_per_cell": t_seq_s / t_multi_s,

@@ -409,8 +409,51 @@ static int pruned_side(lelph_ctx* c, const DevWfc& w, const SideGeom& geo, const
     return LELPH_OK;
 }
 
+// orders pairs along k -> k+q chains: a pair follows the one whose k+q side
+// (iBZ k, symmetry) is its own k side; chain heads first, then what is left
+// of cycles
+static void chain_order(const lelph_ctx* c, const lelph_pair* pairs, int64_t npairs,
+                        std::vector<int64_t>& order)
+{
+    order.clear();
+    order.reserve((size_t)npairs);
+    std::vector<int64_t> next((size_t)npairs, -1);
+    std::vector<char> has_prev((size_t)npairs, 0), used((size_t)npairs, 0);
+    std::vector<std::pair<int64_t, int64_t>> keys((size_t)npairs);
+    for (int64_t i = 0; i < npairs; ++i)
+        keys[i] = {(int64_t)pairs[i].ik * c->lat.nsym + pairs[i].ksym, i};
+    std::sort(keys.begin(), keys.end());
+    for (int64_t i = 0; i < npairs; ++i)
+    {
+        const int64_t want = (int64_t)pairs[i].ikq * c->lat.nsym + pairs[i].kqsym;
+        auto it = std::lower_bound(keys.begin(), keys.end(), std::make_pair(want, (int64_t)-1));
+        if (it != keys.end() && it->first == want && it->second != i && !has_prev[it->second])
+        {
+            next[i] = it->second;
+            has_prev[it->second] = 1;
+        }
+    }
+    for (int pass = 0; pass < 2; ++pass)
+        for (int64_t i = 0; i < npairs; ++i)
+        {
+            if (used[i] || (pass == 0 && has_prev[i])) continue;
+            for (int64_t j = i; j >= 0 && !used[j]; j = next[j])
+            {
+                used[j] = 1;
+                order.push_back(j);
+            }
+        }
+}
+
+// psi_k and psi_k+q boxes of a pair into c->box_k, c->box_kq.  k_from: the
+// k side is the k+q box of the pair before it in a k -> k+q chain (its
+// umklapp vector k_from_g0): taken over with the shift theorem (as is, when
+// that vector is zero and k_from is c->box_k itself) instead of
+// transformed.  g0_out: this pair's k+q umklapp vector.
 template <typename T>
-static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
+static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
+                       const void* k_from = nullptr, const int* k_from_g0 = nullptr,
+                       int* g0_out = nullptr)
 {
     const lelph_lattice& L = c->lat;
     if (pr.ik < 0 || pr.ikq < 0 || pr.ik >= (int)c->wf.size() || pr.ikq >= (int)c->wf.size())
@@ -443,7 +486,9 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
                        c->box_prev_sym == pr.ksym && c->box_prev_prec == prec_tag;
     int g0[3];
     for (int i = 0; i < 3; ++i) g0[i] = (int)std::lrint(gkq.G0c[i]);
+    if (g0_out) std::memcpy(g0_out, g0, sizeof(g0));
     c->box_prev_valid = false;
+    const bool linked = k_from != nullptr && !reuse;
     if (reuse)
     {
         std::swap(c->box_k.p, c->box_kq.p);
@@ -452,10 +497,24 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
         if (pg[0] || pg[1] || pg[2])
         {
             hipLaunchKernelGGL(k_box_shift<T>, dim3((unsigned)cdiv(c->nf, 256)), dim3(256), 0,
-                               c->stream, c->box_k.as<cpx<T>>(), nbox, Nx, Ny, Nz, pg[0], pg[1],
-                               pg[2], -1);
+                               c->stream, c->box_k.as<cpx<T>>(), c->box_k.as<cpx<T>>(), nbox, Nx,
+                               Ny, Nz, pg[0], pg[1], pg[2], -1);
             HIP_TRY(hipGetLastError());
         }
+        if (!pruned) LELPH_TRY(zero_async(c->box_kq.p, bb, c->stream));
+    }
+    else if (linked)
+    {
+        const int* pg = k_from_g0;
+        if (pg[0] || pg[1] || pg[2])
+        {
+            hipLaunchKernelGGL(k_box_shift<T>, dim3((unsigned)cdiv(c->nf, 256)), dim3(256), 0,
+                               c->stream, static_cast<const cpx<T>*>(k_from),
+                               c->box_k.as<cpx<T>>(), nbox, Nx, Ny, Nz, pg[0], pg[1], pg[2], -1);
+            HIP_TRY(hipGetLastError());
+        }
+        else if (k_from != c->box_k.p)
+            HIP_TRY(hipMemcpyAsync(c->box_k.p, k_from, bb, hipMemcpyDeviceToDevice, c->stream));
         if (!pruned) LELPH_TRY(zero_async(c->box_kq.p, bb, c->stream));
     }
     else
@@ -476,7 +535,7 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
     if constexpr (std::is_same<T, float>::value)
         if (pruned)
         {
-            if (!reuse)
+            if (!reuse && !linked)
                 LELPH_TRY(pruned_side(c, wk, gk, c->idx_k.as<int>(), c->ph_k.as<z64>(), 0,
                                       c->box_k.as<c32>()));
             LELPH_TRY(pruned_side(c, wkq, gkq, c->idx_kq.as<int>(), c->ph_kq.as<z64>(), 1,
@@ -485,7 +544,7 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
         }
     if (!done)
     {
-        if (!reuse)
+        if (!reuse && !linked)
             hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)cdiv(nsb * wk.npw, 256)), dim3(256),
                                0, c->stream, wk.wfc, nsb, L.nspinor, wk.npw, c->idx_k.as<int>(),
                                c->ph_k.as<z64>(), gk, c->nf, c->box_k.as<cpx<T>>());
@@ -493,7 +552,7 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr)
                            c->stream, wkq.wfc, nsb, L.nspinor, wkq.npw, c->idx_kq.as<int>(),
                            c->ph_kq.as<z64>(), gkq, c->nf, c->box_kq.as<cpx<T>>());
         HIP_TRY(hipGetLastError());
-        if (!reuse)
+        if (!reuse && !linked)
             LELPH_TRY(fft3d_batch<T>(c->ax, c->box_k.as<cpx<T>>(), nbox, +1, c->stream,
                                      c->tune.fft_lines));
         LELPH_TRY(fft3d_batch<T>(c->ax, c->box_kq.as<cpx<T>>(), nbox, +1, c->stream,
@@ -1630,40 +1689,10 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
         if (p.ksym < 0 || p.kqsym < 0 || p.ksym >= c->lat.nsym || p.kqsym >= c->lat.nsym)
             return LELPH_ERR_ARG;
     }
-    // order the pairs along k -> k+q chains: a pair follows the one whose
-    // k+q side (iBZ k, symmetry) is its own k side, so every second psi box
-    // is reused (build_boxes); results keep the caller's order
+    // order the pairs along k -> k+q chains, so every second psi box is
+    // reused (build_boxes); results keep the caller's order
     std::vector<int64_t> order;
-    order.reserve((size_t)npairs);
-    {
-        std::vector<int64_t> next((size_t)npairs, -1);
-        std::vector<char> has_prev((size_t)npairs, 0), used((size_t)npairs, 0);
-        std::vector<std::pair<int64_t, int64_t>> keys((size_t)npairs);
-        for (int64_t i = 0; i < npairs; ++i)
-            keys[i] = {(int64_t)pairs[i].ik * c->lat.nsym + pairs[i].ksym, i};
-        std::sort(keys.begin(), keys.end());
-        for (int64_t i = 0; i < npairs; ++i)
-        {
-            const int64_t want = (int64_t)pairs[i].ikq * c->lat.nsym + pairs[i].kqsym;
-            auto it = std::lower_bound(keys.begin(), keys.end(), std::make_pair(want, (int64_t)-1));
-            if (it != keys.end() && it->first == want && it->second != i && !has_prev[it->second])
-            {
-                next[i] = it->second;
-                has_prev[it->second] = 1;
-            }
-        }
-        for (int pass = 0; pass < 2; ++pass)
-            for (int64_t i = 0; i < npairs; ++i)
-            {
-                // chain heads first, then what is left of cycles
-                if (used[i] || (pass == 0 && has_prev[i])) continue;
-                for (int64_t j = i; j >= 0 && !used[j]; j = next[j])
-                {
-                    used[j] = 1;
-                    order.push_back(j);
-                }
-            }
-    }
+    chain_order(c, pairs, npairs, order);
     // copy-out: pair n's g goes device -> pinned slot n % LELPH_OUT_SLOTS on
     // the copy stream once its last event has fired, and the host moves it
     // into g_out while the GPU works on later pairs
@@ -1731,6 +1760,10 @@ struct MultiPair
     int sep = 0;
     int nsplit = 1;
     int64_t cps = 1, gsz = 0, tiles = 0;
+    size_t bb = 0;     // bytes of one psi box
+    int64_t link = -1; // the pair before it in its k -> k+q chain (index in pairs)
+    int g0[3] = {0, 0, 0}; // umklapp vector of its k+q side
+    void* kbox = nullptr;  // its psi_k box: box_k, or the linked pair's box_kq
     DBuf box_k, box_kq, part, ebuf;
 };
 } // namespace
@@ -1744,6 +1777,7 @@ struct lelph_multi
     std::vector<std::array<double, 3>> qpt;
     std::vector<int64_t> npairs;
     std::vector<MultiPair> pairs;
+    std::vector<int64_t> border; // box build order: each cell's pairs along chains
     int non_loc = 1;
     std::vector<DBuf> gdev;    // per cell: (npairs, nmodes, nspin, nb, nb) complex128
     std::vector<void*> ghost;  // per cell: pinned copy of gdev
@@ -1792,17 +1826,26 @@ static int multi_enqueue(lelph_multi* m, bool timed)
                 LELPH_TRY(run_nonlocal(m->ctx[p.cell], p.pr, nullptr, m->nl, nullptr,
                                        p.ebuf.as<z64>()));
         }
-        // every pair's psi_k and psi_k+q boxes into its own slots
-        for (auto& p : m->pairs)
+        // every pair's psi_k+q box into its own slot, along each cell's
+        // k -> k+q chains: a linked pair's psi_k box is the k+q box of the
+        // pair before it (the same box, or a shifted copy where that pair
+        // has an umklapp vector), as in lelph_gkq_batch; a chain head
+        // transforms both sides
+        for (int64_t bi : m->border)
         {
+            MultiPair& p = m->pairs[(size_t)bi];
             lelph_ctx* c = m->ctx[p.cell];
-            c->box_k = p.box_k;
+            c->box_k.p = p.kbox;
+            c->box_k.n = p.bb;
             c->box_kq = p.box_kq;
             const double* q = m->qpt[p.cell].data();
+            const MultiPair* lp = p.link >= 0 ? &m->pairs[(size_t)p.link] : nullptr;
+            const void* kf = lp ? lp->box_kq.p : nullptr;
+            const int* kg = lp ? lp->g0 : nullptr;
             if (p.prec == LELPH_FP64)
-                LELPH_TRY(build_boxes<double>(c, q, p.pr));
+                LELPH_TRY(build_boxes<double>(c, q, p.pr, kf, kg));
             else
-                LELPH_TRY(build_boxes<float>(c, q, p.pr));
+                LELPH_TRY(build_boxes<float>(c, q, p.pr, kf, kg));
         }
         if (timed) HIP_TRY(hipEventRecord(m->ev[1], m->stream));
         // the work-queue contractions
@@ -1889,7 +1932,7 @@ static int build_queue(lelph_multi* m, int prec, DBuf& qp, DBuf& qi, int* nitems
         const lelph_lattice& L = c->lat;
         const int nmag_eff = (L.nspinor == 1 && L.nmag == 4) ? 1 : L.nmag;
         QueuePair<T> d{};
-        d.psik = p.box_k.as<cpx<T>>();
+        d.psik = static_cast<cpx<T>*>(p.kbox);
         d.psikq = p.box_kq.as<cpx<T>>();
         d.dv = dv_ptr<T>(c);
         d.part = p.part.as<z64>();
@@ -2038,15 +2081,53 @@ int lelph_multi_create(int ncells, lelph_ctx* const* ctxs, const double* qpts,
             p.gsz = gsz;
             p.tiles = tiles;
             pick_splits(tiles * L.nspin, nchunks, gsz, &p.cps, &p.nsplit, c->tune.split_target);
+            p.bb = bb;
             const int64_t es = (int64_t)L.nspin * L.nbnds * L.nbnds;
-            if (p.box_k.ensure(bb) || p.box_kq.ensure(bb) ||
-                p.part.ensure(sizeof(z64) * (size_t)(p.nsplit * gsz)) ||
+            // the k+q side's umklapp vector (it shifts the box a linked
+            // pair takes over)
+            SideGeom gkq, gk;
+            int gs = local_geometry(c, m->qpt.back().data(), p.pr.ikq, p.pr.ik, p.pr.kqsym,
+                                    p.pr.ksym, &gkq, &gk);
+            if (gs != LELPH_OK)
+            {
+                m->pairs.push_back(p);
+                return fail(gs);
+            }
+            for (int d = 0; d < 3; ++d) p.g0[d] = (int)std::lrint(gkq.G0c[d]);
+            if (p.box_kq.ensure(bb) || p.part.ensure(sizeof(z64) * (size_t)(p.nsplit * gsz)) ||
                 (non_loc && p.ebuf.ensure(sizeof(z64) * (size_t)(3 * L.natom * es))))
             {
                 m->pairs.push_back(p);
                 return fail(LELPH_ERR_ALLOC);
             }
             m->pairs.push_back(p);
+        }
+        // chains: a pair whose k side is the k+q side of the pair before it
+        // takes that box over (aliased when the umklapp vector is zero);
+        // every other pair gets a psi_k box of its own
+        {
+            const int64_t first = (int64_t)m->pairs.size() - npairs[i];
+            std::vector<int64_t> order;
+            chain_order(c, pairs[i], npairs[i], order);
+            for (size_t n = 0; n < order.size(); ++n)
+            {
+                MultiPair& p = m->pairs[(size_t)(first + order[n])];
+                m->border.push_back(first + order[n]);
+                if (n > 0)
+                {
+                    const MultiPair& q = m->pairs[(size_t)(first + order[n - 1])];
+                    if (p.pr.ik == q.pr.ikq && p.pr.ksym == q.pr.kqsym)
+                        p.link = first + order[n - 1];
+                }
+                const MultiPair* lp = p.link >= 0 ? &m->pairs[(size_t)p.link] : nullptr;
+                if (lp && !lp->g0[0] && !lp->g0[1] && !lp->g0[2])
+                    p.kbox = lp->box_kq.p;
+                else
+                {
+                    if (p.box_k.ensure(bb)) return fail(LELPH_ERR_ALLOC);
+                    p.kbox = p.box_k.p;
+                }
+            }
         }
         m->gdev.emplace_back();
         m->gbytes.push_back(sizeof(z64) * (size_t)(npairs[i] * gsz));

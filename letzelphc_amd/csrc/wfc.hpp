@@ -105,13 +105,14 @@ __global__ void k_scatter(const z64* __restrict__ wfc, int64_t nsets, int nspino
     }
 }
 
-// box[b][p] *= e^{sgn 2 pi i G0 . r_p} for every box b: the backward
-// transform of coefficients shifted by the integer G0 (shift theorem), so a
-// psi(r) box built for one umklapp vector serves another.  Exact integer
-// reduction of G0 . r before the FP64 sincospi.
+// dst[b][p] = src[b][p] e^{sgn 2 pi i G0 . r_p} for every box b (src == dst
+// in place): the backward transform of coefficients shifted by the integer
+// G0 (shift theorem), so a psi(r) box built for one umklapp vector serves
+// another.  Exact integer reduction of G0 . r before the FP64 sincospi.
 template <typename T>
-__global__ __launch_bounds__(256) void k_box_shift(cpx<T>* __restrict__ box, int64_t nbox, int Nx,
-                                                   int Ny, int Nz, int g0, int g1, int g2, int sgn)
+__global__ __launch_bounds__(256) void k_box_shift(const cpx<T>* src, cpx<T>* dst, int64_t nbox,
+                                                   int Nx, int Ny, int Nz, int g0, int g1, int g2,
+                                                   int sgn)
 {
     const int64_t nf = (int64_t)Nx * Ny * Nz;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -127,9 +128,5 @@ __global__ __launch_bounds__(256) void k_box_shift(cpx<T>* __restrict__ box, int
     double s, c;
     sincospi(2.0 * f * sgn, &s, &c);
     const cpx<T> e = {(T)c, (T)s};
-    for (int64_t b = 0; b < nbox; ++b)
-    {
-        cpx<T>& v = box[b * nf + p];
-        v = cmul(v, e);
-    }
+    for (int64_t b = 0; b < nbox; ++b) dst[b * nf + p] = cmul(src[b * nf + p], e);
 }

@@ -4,8 +4,11 @@
 batch: per-pair boxes, one persistent work-queue contraction launch per
 precision over all tiles, the rest, captured in a HIP graph and replayed.
 
-- every cell's g bit-identical to one lelph_gkq per pair (same tiles, same
-  fixed-order reductions), after the autotuner has set each grid's shapes;
+- every cell's g bit-identical to its own lelph_gkq_batch (same tiles, same
+  fixed-order reductions, the same psi boxes taken over along k -> k+q
+  chains), after the autotuner has set each grid's shapes, and within
+  rounding (1e-12 FP64, 1e-6 FP32) of one lelph_gkq per pair, which
+  transforms every box;
 - graph replays, with other work and synchronisations between them, each
   equal to the eager pass;
 - the first pair of every cell against the FP64 oracle (FP64 cells 1e-10,
@@ -46,15 +49,20 @@ def cells(tmp_path_factory):
         ctx.close()
 
 
-def test_multi_bit_identical_to_pairwise(cells):
+def test_multi_bit_identical_to_cell_batches(cells):
     cs, jobs, m, _ = cells
-    say("case: multi-cell batch vs one lelph_gkq per pair, 8 cells x 16 pairs")
+    say("case: multi-cell batch vs per-cell batches and one lelph_gkq per pair, "
+        "8 cells x 16 pairs")
     gm = [g.copy() for g in m.run()]
+    _, gs = batch.run_sequential(jobs)
     _, gp = batch.run_pairwise(jobs)
     say(f"  timings {m.timings()}")
-    for i, (a, b) in enumerate(zip(gm, gp)):
-        say(f"  cell {i} {cs[i].fft_dims}: max |d| = {float(np.max(np.abs(a - b))):.3e}")
+    for i, (a, b, c) in enumerate(zip(gm, gs, gp)):
+        e = rel_err(a, c)
+        say(f"  cell {i} {cs[i].fft_dims}: vs per-cell batch max |d| = "
+            f"{float(np.max(np.abs(a - b))):.3e}, vs per pair {e:.3e}")
         assert np.array_equal(a, b), i
+        assert e < (1e-12 if i % 2 == 0 else 1e-6), i
 
 
 def test_multi_replays(cells):
