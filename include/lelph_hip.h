@@ -227,13 +227,16 @@ int lelph_gkq_batch(lelph_ctx* ctx, const double qpt[3], int64_t npairs,
  * wave, band and mode counts, FP64 or FP32 each): the k loops of
  * compute_and_write_elphq (src/elph/compute_elph_matq.c:116-223) of every
  * cell as one unit.  lelph_multi_create plans it: every pair gets its own
- * psi boxes and K-split partials, the contractions of all pairs of one
- * precision become the tile list of one persistent work-queue launch
+ * psi_k+q box and K-split partials, and its psi_k box is the psi_k+q box of
+ * the pair before it on a k -> k+q chain (shifted where that side has an
+ * umklapp vector) as in lelph_gkq_batch, the contractions of all pairs of
+ * one precision become the tile list of one persistent work-queue launch
  * (largest tiles first), and the whole sequence (boxes, contractions,
  * fixed-order reductions, non-local parts, copy-out) runs once eagerly and
  * is captured in a HIP graph.  lelph_multi_run replays the graph; g_out[i]
  * receives cell i's (npairs[i], nmodes, nspin, nbnds, nbnds).  g is
- * bit-identical to lelph_gkq on each pair.  The contexts' dV and precision
+ * bit-identical to lelph_gkq_batch on each cell's pairs (and within
+ * rounding of lelph_gkq per pair).  The contexts' dV and precision
  * must stay as they were at creation; bf16 split precisions are not
  * supported here (LELPH_ERR_UNSUPPORTED).  No counterpart in the
  * reference.  Timings (ms): boxes, contractions, reductions + non-local +
