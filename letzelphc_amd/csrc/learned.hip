@@ -110,21 +110,43 @@ __global__ void k_nn_im2col(const __bf16* __restrict__ A, int C, NnGrid g, int64
         *reinterpret_cast<const uint4*>(A + q * C + 8 * u);
 }
 
-// colT[t * C + c][p] = A[nbr(p, t)][c]: 8 consecutive points per thread
-__global__ void k_nn_im2col_t(const __bf16* __restrict__ A, int C, NnGrid g, int64_t P,
-                              __bf16* __restrict__ colT)
+// colT[t * C + c][p] = A[nbr(p, t)][c] (C = 64 or 128).  Block: 64 points
+// x the C channels of one tap through LDS: the neighbours' channel rows are
+// read as 16-byte units (a point's channels are contiguous), stored with a
+// row stride of C + 2 bf16 (65 or 33 words, so the column reads below fall
+// on distinct banks), then every thread writes 8 consecutive points of one
+// channel (16 bytes; 8 threads cover a channel's 128-byte row segment).
+// P is a multiple of 64 (the host requires grids of 64k points).
+#define NN_T_PTS 64
+__global__ __launch_bounds__(256) void k_nn_im2col_t(const __bf16* __restrict__ A, int C,
+                                                     NnGrid g, int64_t P,
+                                                     __bf16* __restrict__ colT)
 {
-    const int64_t pu = P / 8;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t K = (int64_t)NN_TAPS * C;
-    if (i >= K * pu) return;
-    const int64_t p0 = (i % pu) * 8;
-    const int64_t k = i / pu;
-    const int t = (int)(k / C), c = (int)(k % C);
-    nn_bf16x8 v;
+    __shared__ uint32_t tile[NN_T_PTS * (NN_CIN + 2) / 2];
+    const int64_t p0 = (int64_t)blockIdx.x * NN_T_PTS;
+    const int t = blockIdx.y;
+    const int cu = C / 8, rw = (C + 2) / 2; // 16-byte units per point, words per LDS row
+    for (int u = threadIdx.x; u < NN_T_PTS * cu; u += blockDim.x)
+    {
+        const int r = u / cu, k = u % cu;
+        const int64_t q = nn_nbr(g, p0 + r, t, 1);
+        const uint4 v = *reinterpret_cast<const uint4*>(A + q * C + 8 * k);
+        uint32_t* d = tile + r * rw + 4 * k;
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+        d[3] = v.w;
+    }
+    __syncthreads();
+    const __bf16* t16 = reinterpret_cast<const __bf16*>(tile);
+    for (int v = threadIdx.x; v < C * (NN_T_PTS / 8); v += blockDim.x)
+    {
+        const int c = v / (NN_T_PTS / 8), j = v % (NN_T_PTS / 8);
+        nn_bf16x8 o;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = A[nn_nbr(g, p0 + j, t, 1) * C + c];
-    *reinterpret_cast<nn_bf16x8*>(colT + k * P + p0) = v;
+        for (int i = 0; i < 8; ++i) o[i] = t16[(8 * j + i) * (2 * rw) + c];
+        *reinterpret_cast<nn_bf16x8*>(colT + ((int64_t)t * C + c) * P + p0 + 8 * j) = o;
+    }
 }
 
 // C[M][N] (or partial split s of it) = sum_k A[m][k] B[n][k], A and B BF16
@@ -534,8 +556,7 @@ static int nn_backward(lelph_nn* m, int S, double* loss)
         HIP_TRY(hipGetLastError());
         const __bf16* in = l == 0 ? m->X0.as<__bf16>() : m->A[l - 1].as<__bf16>();
         const int C = m->cin[l];
-        const int64_t totT = (int64_t)NN_TAPS * C * (P / 8);
-        hipLaunchKernelGGL(k_nn_im2col_t, dim3((unsigned)cdiv(totT, 256)), dim3(256), 0,
+        hipLaunchKernelGGL(k_nn_im2col_t, dim3((unsigned)(P / NN_T_PTS), NN_TAPS), dim3(256), 0,
                            m->stream, in, C, m->g, P, m->colT.as<__bf16>());
         HIP_TRY(hipGetLastError());
         // dW[k][c] = sum_p colT[k][p] dZT[c][p], split over the points

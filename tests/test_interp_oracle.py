@@ -430,3 +430,91 @@ def test_soc_nonlocal_reduces_to_scalar():
         gb = ob.add_nonlocal(0, *pr, z)
         assert np.abs(gb).max() > 0
         assert rel_err(ga, gb) < 1e-6, pr
+
+
+def _dd_short_and_self(s, lr, q_crys, eta, rmax=36.0):
+    """Real-space part and self term of the Ewald sum of the dipole-dipole
+    dynamical matrix (Gonze & Lee, PRB 55, 10355 (1997), sec. IV), derived
+    here from the split 1/(sqrt(det eps) D) = [erf(L D) + erfc(L D)] /
+    (sqrt(det eps) D), D^2 = d.eps^-1.d, L^2 = eta, e^2 = 2 (Rydberg), on the
+    lattice whose reciprocal vectors are the system's blat (which carries the
+    reference's truncated pi): returns sum'_R F_short(tau_a - tau_b + R)
+    e^{-i q.R} - delta_ab F_long(0), F = -Z_a^T (Hessian of the potential) Z_b,
+    with the index order of the oracle's dynamical matrix [(a, i), (b, j)]."""
+    from scipy.special import erfc
+    e2 = 2.0
+    B = np.asarray(s.blat, np.float64)
+    A = 2 * np.pi * np.linalg.inv(B).T  # columns a_j with a_j . b_i = 2 pi delta_ij
+    eps = np.asarray(lr.epsilon, np.float64)
+    ei = np.linalg.inv(eps)
+    sq = np.sqrt(np.linalg.det(eps))
+    L = np.sqrt(eta)
+    Z = np.asarray(lr.Zborn, np.float64)  # Z[a, k, i]: (Z^T K)_i = sum_k Z[a, k, i] K_k
+    tau = np.asarray(s.atomic_pos, np.float64)
+    na = s.natom
+    nmax = int(np.ceil(rmax / np.min(np.linalg.norm(A, axis=0)))) + 1
+    rng_ = np.arange(-nmax, nmax + 1)
+    n = np.stack(np.meshgrid(rng_, rng_, rng_, indexing="ij"), -1).reshape(-1, 3)
+    R = n @ A.T
+    phase = np.exp(-2j * np.pi * (n @ np.asarray(q_crys, np.float64)))
+    out = np.zeros((na, 3, na, 3), np.complex128)
+    for a in range(na):
+        for b in range(na):
+            d = tau[a] - tau[b] + R
+            keep = np.linalg.norm(d, axis=1) > 1e-9
+            d, ph = d[keep], phase[keep]
+            Dl = d @ ei  # Delta = eps^-1 d
+            D = np.sqrt(np.einsum("ni,ni->n", d, Dl))
+            ec = erfc(L * D)
+            ex = (2 * L / np.sqrt(np.pi)) * np.exp(-(L * D) ** 2)
+            f1 = -ec / D ** 2 - ex / D
+            f2 = 2 * ec / D ** 3 + ex * (2 / D ** 2 + 2 * L ** 2)
+            H = (np.einsum("nk,nl->nkl", Dl, Dl) * ((f2 - f1 / D) / D ** 2)[:, None, None]
+                 + ei[None] * (f1 / D)[:, None, None]) / sq
+            Hs = np.einsum("nkl,n->kl", H, ph)
+            out[a, :, b, :] = -np.einsum("ki,kl,lj->ij", Z[a], Hs, Z[b])
+        self_long = (4 * L ** 3 / (3 * np.sqrt(np.pi) * sq)) * np.einsum("ki,kl,lj->ij", Z[a], ei,
+                                                                         Z[a])
+        out[a, :, a, :] -= self_long
+    return e2 * out.reshape(3 * na, 3 * na)
+
+
+def test_dyn_longrange_ewald_invariance():
+    """The oracle's long-range dynamical matrix (the reciprocal sum over
+    q + G != 0 with the Gaussian e^{-(q+G).eps.(q+G)/(4 eta)} of
+    add_ph_dyn_long_range, src/phonon/dyn_long_range.c:125-404) is the
+    reciprocal half of an Ewald sum: adding the real-space erfc half and the
+    self term derived independently above gives the full dipole-dipole
+    lattice sum, which cannot depend on eta.  Checked for three eta at a
+    general q, with anisotropic eps and non-symmetric Born charges; the
+    reciprocal parts alone differ by O(1) between the eta, and a reciprocal
+    part off by 0.1 % breaks the invariance.  The reference's 4 pi and
+    volume use ELPH_PI and the true cell while its reciprocal vectors carry
+    ELPH_PI: the exact Ewald normalisation of that reciprocal lattice is
+    (pi / ELPH_PI) (Omega / Omega'), applied to the oracle's sum."""
+    s = synth.tiny(with_wfc=False)
+    lr = _lr_params(s, quad=False)
+    lr.epsilon = np.array([[5.6, 0.3, -0.1], [0.3, 6.2, 0.2], [-0.1, 0.2, 5.9]])
+    q = np.array([0.23, -0.11, 0.17])
+    nm = s.nmodes
+    z = np.zeros((nm, nm), np.complex128)
+    B = np.asarray(s.blat, np.float64)
+    vol_p = abs(np.linalg.det(2 * np.pi * np.linalg.inv(B).T))
+    norm = (np.pi / synth.RPI) * (s.volume / vol_p)
+    say("case: Ewald invariance of the long-range dynamical matrix")
+    tot, rec = [], []
+    for eta in (0.6, 1.0, 1.8):
+        od = O.OracleDynInterp(s, (1, 1, 1), masses=None, lr=lr, eta_ph=eta, ecut=200.0,
+                               asr="none")
+        r = od.lr_term(q, z, 1, masses=False, asr=False) * norm
+        rec.append(r)
+        tot.append(r + _dd_short_and_self(s, lr, q, eta))
+    scale = np.abs(tot[0]).max()
+    spread = max(np.abs(t - tot[0]).max() for t in tot[1:]) / scale
+    rec_change = np.abs(rec[2] - rec[0]).max() / scale
+    planted = np.abs((1.001 * rec[2] + tot[2] - rec[2]) - tot[0]).max() / scale
+    say(f"  total spread over eta {spread:.2e}; reciprocal parts differ by {rec_change:.2e}; "
+        f"reciprocal x 1.001: {planted:.2e}")
+    assert rec_change > 1e-1
+    assert spread < 1e-10
+    assert planted > 1e-4
