@@ -156,6 +156,10 @@ typedef struct lelph_tuning
                          32 where they fit in LDS, 0 = per-length default    */
     int fp32_form;    /* FP32 contraction: 0 dV and psi_k staged separately,
                          1 element-wise A = dV psi_k                          */
+    int ctile;        /* FP64 / FP32 contraction block tile: 0 chosen from
+                         the mode and band counts, 1 two modes x 32 x 64,
+                         2 four modes x 32 x 32, 3 four modes x 16 x 48,
+                         4 four modes x 16 x 16 (bit-identical g)             */
 } lelph_tuning;
 int lelph_set_tuning(lelph_ctx* ctx, const lelph_tuning* tuning);
 int lelph_get_tuning(lelph_ctx* ctx, lelph_tuning* tuning);

@@ -60,9 +60,11 @@ class Pseudo(C.Structure):
 
 
 class Tuning(C.Structure):
-    """lelph_tuning: launch shapes of a context (fft_lines and fp32_form
-    give bit-identical g, split_target changes it at the rounding level)."""
-    _fields_ = [("split_target", C.c_int), ("fft_lines", C.c_int), ("fp32_form", C.c_int)]
+    """lelph_tuning: launch shapes of a context (fft_lines, fp32_form and
+    ctile give bit-identical g, split_target changes it at the rounding
+    level)."""
+    _fields_ = [("split_target", C.c_int), ("fft_lines", C.c_int), ("fp32_form", C.c_int),
+                ("ctile", C.c_int)]
 
 
 class Pair(C.Structure):

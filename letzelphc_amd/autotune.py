@@ -14,7 +14,11 @@ spinors and potential components, and the plane-wave count (in steps of
 - fft_lines: lines per block of the compiled line FFTs of the box build
   (bit-identical g);
 - fp32_form: FP32 contraction with dV and psi_k staged separately, or
-  element-wise (bit-identical g).
+  element-wise (bit-identical g);
+- ctile: block tile of the FP64 / FP32 contraction (2 modes x 32 x 64, or
+  4 modes x 32 x 32, 16 x 48, 16 x 16; 0 picks by the mode and band counts):
+  the padding a tile wastes when the band count is not a multiple of it,
+  against the fragment reuse of larger tiles (bit-identical g).
 
 The search is coordinate descent from the defaults, one knob at a time,
 each candidate timed on a few pairs of the actual workload with the
@@ -37,6 +41,7 @@ KNOBS = [
     ("split_target", (512, 1024, 2048, 4096)),
     ("fft_lines", (0, 8, 16, 32)),
     ("fp32_form", (0, 1)),
+    ("ctile", (0, 1, 2, 3, 4)),
 ]
 
 
@@ -98,7 +103,7 @@ def tune(ctx, pairs, qpt, sample: int = 3, reps: int = 3, cache: dict | None = N
         ctx.set_tuning(**cache[key]["tuning"])
         return dict(cache[key], cached=True)
     sub = list(pairs[:sample])
-    best = {"split_target": 0, "fft_lines": 0, "fp32_form": 0}
+    best = {"split_target": 0, "fft_lines": 0, "fp32_form": 0, "ctile": 0}
     ctx.set_tuning(**best)
     base_ms, ref = _time(ctx, sub, qpt, reps)
     best_ms, best_g = base_ms, ref

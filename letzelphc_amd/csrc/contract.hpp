@@ -50,10 +50,31 @@ struct Mfma<float>
 
 #define CT_KC 16
 #define CT_LD (CT_KC + 1)
-#define CT_VB 2  // modes per block
+#define CT_VB 2  // modes per block (wide tile; its tile count sets the K splits)
 #define CT_NB 32 // k bands (rows) per block
 #define CT_MB 64 // k+q bands (columns) per block
 #define CT_FLUSH 4
+
+// Block tile shapes: VB modes x NB rows x MB columns over 4 waves, wave w
+// taking mode w / NCG and column group w % NCG, each wave RB x CB blocks of
+// 16 x 16.  Every shape issues, for each g element, the same MFMAs on the
+// same operands in the same order (per 4-point step the first products of
+// all the wave's blocks, then the second ones), so with the same K splits
+// all shapes give bit-identical partials; they differ only in how much of a
+// tile is padding when nb is not a multiple of the tile.
+template <int VB_, int RB_, int CB_>
+struct CtShape
+{
+    static constexpr int VB = VB_, RB = RB_, CB = CB_;
+    static constexpr int NCG = 4 / VB_;
+    static constexpr int NB = 16 * RB_;
+    static constexpr int MB = 16 * CB_ * NCG;
+    static_assert(VB_ == 1 || VB_ == 2 || VB_ == 4, "4 waves per block");
+};
+typedef CtShape<2, 2, 2> CtWide; // 2 modes x 32 x 64
+typedef CtShape<4, 2, 2> Ct32;   // 4 modes x 32 x 32
+typedef CtShape<4, 1, 3> Ct16x48;   // 4 modes x 16 x 48
+typedef CtShape<4, 1, 1> Ct16;   // 4 modes x 16 x 16
 
 // Loads A[n][r] = (dV_nu psi_k,n)_sigma(r) for the block's modes.
 template <typename T>
@@ -78,7 +99,7 @@ __device__ inline cpx<T> load_dvpsi(const cpx<T>* __restrict__ psik,
 // tile), K split `split`, spin `is`.  The kernel k_contract runs one tile
 // per block; the work-queue kernel k_contract_queue runs tiles of many
 // (k, k+q) pairs and cells from a list.
-template <typename T>
+template <typename T, typename S>
 __device__ inline void contract_tile(const cpx<T>* __restrict__ psik,
                                      const cpx<T>* __restrict__ psikq,
                                      const cpx<T>* __restrict__ dv, z64* __restrict__ part,
@@ -86,27 +107,31 @@ __device__ inline void contract_tile(const cpx<T>* __restrict__ psik,
                                      int64_t chunks_per_split, int bx, int split, int is)
 {
     typedef typename Mfma<T>::acc acc_t;
-    __shared__ cpx<T> sA[CT_VB][CT_NB][CT_LD];
-    __shared__ cpx<T> sB[CT_MB][CT_LD];
+    constexpr int VB = S::VB, NB = S::NB, MB = S::MB, RB = S::RB, CB = S::CB;
+    __shared__ cpx<T> sA[VB][NB][CT_LD];
+    __shared__ cpx<T> sB[MB][CT_LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ntn = (nb + CT_NB - 1) / CT_NB, ntm = (nb + CT_MB - 1) / CT_MB;
+    const int ntn = (nb + NB - 1) / NB, ntm = (nb + MB - 1) / MB;
     const int tm = bx % ntm;
     bx /= ntm;
     const int tn = bx % ntn;
     bx /= ntn;
-    const int v0 = bx * CT_VB, n0 = tn * CT_NB, m0 = tm * CT_MB;
+    const int v0 = bx * VB, n0 = tn * NB, m0 = tm * MB;
     const int64_t nchunks = (N + CT_KC - 1) / CT_KC;
     const int64_t c0 = (int64_t)split * chunks_per_split;
     const int64_t c1 = (c0 + chunks_per_split < nchunks) ? c0 + chunks_per_split : nchunks;
-    const int wv = wave >> 1, wm = wave & 1;
+    const int wv = wave / S::NCG, wm = wave % S::NCG;
 
     // FP32: the MFMA accumulators are added into FP64 every CT_FLUSH chunks,
     // so no FP32 summation chain is longer than CT_FLUSH * CT_KC points.
     constexpr bool flush = sizeof(T) == 4;
-    acc_t accr[2][2], acci[2][2];
-    double dr[2][2][4], di[2][2][4];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
+    acc_t accr[RB][CB], acci[RB][CB];
+    double dr[RB][CB][4], di[RB][CB][4];
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int b = 0; b < CB; ++b)
+#pragma unroll
             for (int v = 0; v < 4; ++v)
             {
                 accr[a][b][v] = 0;
@@ -124,9 +149,9 @@ __device__ inline void contract_tile(const cpx<T>* __restrict__ psik,
             {
                 nfl = 1;
 #pragma unroll
-                for (int a = 0; a < 2; ++a)
+                for (int a = 0; a < RB; ++a)
 #pragma unroll
-                    for (int b = 0; b < 2; ++b)
+                    for (int b = 0; b < CB; ++b)
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
                         {
@@ -137,25 +162,29 @@ __device__ inline void contract_tile(const cpx<T>* __restrict__ psik,
                         }
             }
             const int64_t r0 = c * CT_KC;
-            for (int f = tid; f < CT_VB * CT_NB * CT_KC; f += 256)
+            for (int f = tid; f < VB * NB * CT_KC; f += 256)
             {
-                const int vv = f / (CT_NB * CT_KC);
-                const int rem = f - vv * (CT_NB * CT_KC);
+                const int vv = f / (NB * CT_KC);
+                const int rem = f - vv * (NB * CT_KC);
                 const int nn = rem / CT_KC, rr = rem - nn * CT_KC;
                 const int v = v0 + vv, n = n0 + nn;
                 const int64_t r = r0 + rr;
-                cpx<T> a = {0, 0};
-                if (v < nm && n < nb && r < N)
-                    a = load_dvpsi<T>(psik, dv, v, is, n, sp, r, nb, nspinor, nmag, N);
+                // padding reads a valid element and stores zero (no branch
+                // around the load)
+                const bool in = v < nm && n < nb && r < N;
+                cpx<T> a = load_dvpsi<T>(psik, dv, in ? v : 0, is, in ? n : 0, sp, in ? r : 0, nb,
+                                         nspinor, nmag, N);
+                if (!in) a = {0, 0};
                 sA[vv][nn][rr] = a;
             }
-            for (int f = tid; f < CT_MB * CT_KC; f += 256)
+            for (int f = tid; f < MB * CT_KC; f += 256)
             {
                 const int mm = f / CT_KC, rr = f - mm * CT_KC;
                 const int m = m0 + mm;
                 const int64_t r = r0 + rr;
-                cpx<T> b = {0, 0};
-                if (m < nb && r < N) b = psikq[((int64_t)(is * nb + m) * nspinor + sp) * N + r];
+                const bool in = m < nb && r < N;
+                cpx<T> b = psikq[((int64_t)(is * nb + (in ? m : 0)) * nspinor + sp) * N + (in ? r : 0)];
+                if (!in) b = {0, 0};
                 sB[mm][rr] = b;
             }
             __syncthreads();
@@ -163,26 +192,26 @@ __device__ inline void contract_tile(const cpx<T>* __restrict__ psik,
             for (int kk = 0; kk < CT_KC / 4; ++kk)
             {
                 const int i = lane & 15, kq = kk * 4 + (lane >> 4);
-                cpx<T> a[2], b[2];
-                a[0] = sA[wv][i][kq];
-                a[1] = sA[wv][16 + i][kq];
-                b[0] = sB[wm * 32 + i][kq];
-                b[1] = sB[wm * 32 + 16 + i][kq];
+                cpx<T> a[RB], b[CB];
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) a[rb] = sA[wv][rb * 16 + i][kq];
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb) b[cb] = sB[(wm * CB + cb) * 16 + i][kq];
                 // C += a conj(b): re = ar br + ai bi, im = ai br - ar bi; the
                 // first products of all 8 accumulators go out before the
                 // second ones, so no MFMA waits on the one just issued
 #pragma unroll
-                for (int rb = 0; rb < 2; ++rb)
+                for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                    for (int cb = 0; cb < 2; ++cb)
+                    for (int cb = 0; cb < CB; ++cb)
                     {
                         accr[rb][cb] = Mfma<T>::op(a[rb].x, b[cb].x, accr[rb][cb]);
                         acci[rb][cb] = Mfma<T>::op(a[rb].y, b[cb].x, acci[rb][cb]);
                     }
 #pragma unroll
-                for (int rb = 0; rb < 2; ++rb)
+                for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                    for (int cb = 0; cb < 2; ++cb)
+                    for (int cb = 0; cb < CB; ++cb)
                     {
                         accr[rb][cb] = Mfma<T>::op(a[rb].y, b[cb].y, accr[rb][cb]);
                         acci[rb][cb] = Mfma<T>::op(-a[rb].x, b[cb].y, acci[rb][cb]);
@@ -193,12 +222,15 @@ __device__ inline void contract_tile(const cpx<T>* __restrict__ psik,
     }
     const int v = v0 + wv;
     if (v >= nm) return;
-    for (int rb = 0; rb < 2; ++rb)
-        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
             for (int e = 0; e < 4; ++e)
             {
                 const int row = n0 + rb * 16 + Mfma<T>::row(lane, e);
-                const int col = m0 + wm * 32 + cb * 16 + (lane & 15);
+                const int col = m0 + (wm * CB + cb) * 16 + (lane & 15);
                 if (row < nb && col < nb)
                 {
                     const int64_t o = ((((int64_t)split * nm + v) * nspin + is) * nb + row) * nb + col;
@@ -215,7 +247,7 @@ __device__ inline void contract_tile(const cpx<T>* __restrict__ psik,
 // of element-wise loads.  Not for the 2x2 potential of spinors (nmag 4,
 // nspinor 2).  Used for FP32; in FP64 the products formed beside the FP64
 // MFMAs cost more than the loads saved.
-template <typename T>
+template <typename T, typename S>
 __device__ inline void contract_sep_tile(const cpx<T>* __restrict__ psik,
                                          const cpx<T>* __restrict__ psikq,
                                          const cpx<T>* __restrict__ dv, z64* __restrict__ part,
@@ -224,28 +256,32 @@ __device__ inline void contract_sep_tile(const cpx<T>* __restrict__ psik,
                                          int is)
 {
     typedef typename Mfma<T>::acc acc_t;
-    __shared__ cpx<T> sV[CT_VB][CT_LD]; // dV of the block's modes
-    __shared__ cpx<T> sP[CT_NB][CT_LD]; // psi_k of the block's rows
-    __shared__ cpx<T> sB[CT_MB][CT_LD];
+    constexpr int VB = S::VB, NB = S::NB, MB = S::MB, RB = S::RB, CB = S::CB;
+    __shared__ cpx<T> sV[VB][CT_LD]; // dV of the block's modes
+    __shared__ cpx<T> sP[NB][CT_LD]; // psi_k of the block's rows
+    __shared__ cpx<T> sB[MB][CT_LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ntn = (nb + CT_NB - 1) / CT_NB, ntm = (nb + CT_MB - 1) / CT_MB;
+    const int ntn = (nb + NB - 1) / NB, ntm = (nb + MB - 1) / MB;
     const int tm = bx % ntm;
     bx /= ntm;
     const int tn = bx % ntn;
     bx /= ntn;
-    const int v0 = bx * CT_VB, n0 = tn * CT_NB, m0 = tm * CT_MB;
+    const int v0 = bx * VB, n0 = tn * NB, m0 = tm * MB;
     const int64_t nchunks = (N + CT_KC - 1) / CT_KC;
     const int64_t c0 = (int64_t)split * chunks_per_split;
     const int64_t c1 = (c0 + chunks_per_split < nchunks) ? c0 + chunks_per_split : nchunks;
-    const int wv = wave >> 1, wm = wave & 1;
+    const int wv = wave / S::NCG, wm = wave % S::NCG;
 
     // FP32: the MFMA accumulators are added into FP64 every CT_FLUSH chunks,
     // so no FP32 summation chain is longer than CT_FLUSH * CT_KC points.
     constexpr bool flush = sizeof(T) == 4;
-    acc_t accr[2][2], acci[2][2];
-    double dr[2][2][4], di[2][2][4];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b)
+    acc_t accr[RB][CB], acci[RB][CB];
+    double dr[RB][CB][4], di[RB][CB][4];
+#pragma unroll
+    for (int a = 0; a < RB; ++a)
+#pragma unroll
+        for (int b = 0; b < CB; ++b)
+#pragma unroll
             for (int v = 0; v < 4; ++v)
             {
                 accr[a][b][v] = 0;
@@ -263,9 +299,9 @@ __device__ inline void contract_sep_tile(const cpx<T>* __restrict__ psik,
             {
                 nfl = 1;
 #pragma unroll
-                for (int a = 0; a < 2; ++a)
+                for (int a = 0; a < RB; ++a)
 #pragma unroll
-                    for (int b = 0; b < 2; ++b)
+                    for (int b = 0; b < CB; ++b)
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
                         {
@@ -276,31 +312,34 @@ __device__ inline void contract_sep_tile(const cpx<T>* __restrict__ psik,
                         }
             }
             const int64_t r0 = c * CT_KC;
-            if (tid < CT_VB * CT_KC)
+            if (tid < VB * CT_KC)
             {
                 const int vv = tid / CT_KC, rr = tid - vv * CT_KC;
                 const int v = v0 + vv;
                 const int64_t r = r0 + rr;
-                cpx<T> x = {0, 0};
-                if (v < nm && r < N) x = dv[((int64_t)v * nmag + (nmag == 2 ? is : 0)) * N + r];
+                const bool in = v < nm && r < N;
+                cpx<T> x = dv[((int64_t)(in ? v : 0) * nmag + (nmag == 2 ? is : 0)) * N + (in ? r : 0)];
+                if (!in) x = {0, 0};
                 sV[vv][rr] = x;
             }
-            for (int f = tid; f < CT_NB * CT_KC; f += 256)
+            for (int f = tid; f < NB * CT_KC; f += 256)
             {
                 const int nn = f / CT_KC, rr = f - nn * CT_KC;
                 const int n = n0 + nn;
                 const int64_t r = r0 + rr;
-                cpx<T> x = {0, 0};
-                if (n < nb && r < N) x = psik[((int64_t)(is * nb + n) * nspinor + sp) * N + r];
+                const bool in = n < nb && r < N;
+                cpx<T> x = psik[((int64_t)(is * nb + (in ? n : 0)) * nspinor + sp) * N + (in ? r : 0)];
+                if (!in) x = {0, 0};
                 sP[nn][rr] = x;
             }
-            for (int f = tid; f < CT_MB * CT_KC; f += 256)
+            for (int f = tid; f < MB * CT_KC; f += 256)
             {
                 const int mm = f / CT_KC, rr = f - mm * CT_KC;
                 const int m = m0 + mm;
                 const int64_t r = r0 + rr;
-                cpx<T> b = {0, 0};
-                if (m < nb && r < N) b = psikq[((int64_t)(is * nb + m) * nspinor + sp) * N + r];
+                const bool in = m < nb && r < N;
+                cpx<T> b = psikq[((int64_t)(is * nb + (in ? m : 0)) * nspinor + sp) * N + (in ? r : 0)];
+                if (!in) b = {0, 0};
                 sB[mm][rr] = b;
             }
             __syncthreads();
@@ -308,27 +347,27 @@ __device__ inline void contract_sep_tile(const cpx<T>* __restrict__ psik,
             for (int kk = 0; kk < CT_KC / 4; ++kk)
             {
                 const int i = lane & 15, kq = kk * 4 + (lane >> 4);
-                cpx<T> a[2], b[2];
+                cpx<T> a[RB], b[CB];
                 const cpx<T> vq = sV[wv][kq];
-                a[0] = cmul(vq, sP[i][kq]);
-                a[1] = cmul(vq, sP[16 + i][kq]);
-                b[0] = sB[wm * 32 + i][kq];
-                b[1] = sB[wm * 32 + 16 + i][kq];
+#pragma unroll
+                for (int rb = 0; rb < RB; ++rb) a[rb] = cmul(vq, sP[rb * 16 + i][kq]);
+#pragma unroll
+                for (int cb = 0; cb < CB; ++cb) b[cb] = sB[(wm * CB + cb) * 16 + i][kq];
                 // C += a conj(b): re = ar br + ai bi, im = ai br - ar bi; the
                 // first products of all 8 accumulators go out before the
                 // second ones, so no MFMA waits on the one just issued
 #pragma unroll
-                for (int rb = 0; rb < 2; ++rb)
+                for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                    for (int cb = 0; cb < 2; ++cb)
+                    for (int cb = 0; cb < CB; ++cb)
                     {
                         accr[rb][cb] = Mfma<T>::op(a[rb].x, b[cb].x, accr[rb][cb]);
                         acci[rb][cb] = Mfma<T>::op(a[rb].y, b[cb].x, acci[rb][cb]);
                     }
 #pragma unroll
-                for (int rb = 0; rb < 2; ++rb)
+                for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-                    for (int cb = 0; cb < 2; ++cb)
+                    for (int cb = 0; cb < CB; ++cb)
                     {
                         accr[rb][cb] = Mfma<T>::op(a[rb].y, b[cb].y, accr[rb][cb]);
                         acci[rb][cb] = Mfma<T>::op(-a[rb].x, b[cb].y, acci[rb][cb]);
@@ -339,12 +378,15 @@ __device__ inline void contract_sep_tile(const cpx<T>* __restrict__ psik,
     }
     const int v = v0 + wv;
     if (v >= nm) return;
-    for (int rb = 0; rb < 2; ++rb)
-        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
             for (int e = 0; e < 4; ++e)
             {
                 const int row = n0 + rb * 16 + Mfma<T>::row(lane, e);
-                const int col = m0 + wm * 32 + cb * 16 + (lane & 15);
+                const int col = m0 + (wm * CB + cb) * 16 + (lane & 15);
                 if (row < nb && col < nb)
                 {
                     const int64_t o = ((((int64_t)split * nm + v) * nspin + is) * nb + row) * nb + col;
@@ -354,7 +396,7 @@ __device__ inline void contract_sep_tile(const cpx<T>* __restrict__ psik,
             }
 }
 
-template <typename T>
+template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_contract(const cpx<T>* __restrict__ psik,
                                                   const cpx<T>* __restrict__ psikq,
                                                   const cpx<T>* __restrict__ dv,
@@ -362,11 +404,11 @@ __global__ __launch_bounds__(256) void k_contract(const cpx<T>* __restrict__ psi
                                                   int nspin, int nspinor, int nmag,
                                                   int64_t N, int64_t chunks_per_split)
 {
-    contract_tile<T>(psik, psikq, dv, part, nm, nb, nspin, nspinor, nmag, N, chunks_per_split,
-                     blockIdx.x, blockIdx.y, blockIdx.z);
+    contract_tile<T, S>(psik, psikq, dv, part, nm, nb, nspin, nspinor, nmag, N, chunks_per_split,
+                        blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-template <typename T>
+template <typename T, typename S>
 __global__ __launch_bounds__(256) void k_contract_sep(const cpx<T>* __restrict__ psik,
                                                       const cpx<T>* __restrict__ psikq,
                                                       const cpx<T>* __restrict__ dv,
@@ -374,8 +416,8 @@ __global__ __launch_bounds__(256) void k_contract_sep(const cpx<T>* __restrict__
                                                       int nspin, int nspinor, int nmag,
                                                       int64_t N, int64_t chunks_per_split)
 {
-    contract_sep_tile<T>(psik, psikq, dv, part, nm, nb, nspin, nspinor, nmag, N,
-                         chunks_per_split, blockIdx.x, blockIdx.y, blockIdx.z);
+    contract_sep_tile<T, S>(psik, psikq, dv, part, nm, nb, nspin, nspinor, nmag, N,
+                            chunks_per_split, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // A (k, k+q) pair of a work-queue launch: its boxes, potential, partials
@@ -405,8 +447,9 @@ struct QueueItem
 // tile computes exactly what k_contract / k_contract_sep computes for it,
 // so the partials, and g after the fixed-order reduction, are bit-identical
 // to one launch per pair.  SEP: the launch may contain separate-staging
-// pairs (FP32 without a 2x2 spinor potential).
-template <typename T, bool SEP>
+// pairs (FP32 without a 2x2 spinor potential).  S: the tile shape of every
+// pair of the launch.
+template <typename T, bool SEP, typename S>
 __global__ __launch_bounds__(256) void k_contract_queue(const QueuePair<T>* __restrict__ pairs,
                                                         const QueueItem* __restrict__ items,
                                                         int nitems, int* __restrict__ counter)
@@ -422,11 +465,12 @@ __global__ __launch_bounds__(256) void k_contract_queue(const QueuePair<T>* __re
         const QueueItem it = items[w];
         const QueuePair<T> p = pairs[it.pair];
         if (SEP && p.sep)
-            contract_sep_tile<T>(p.psik, p.psikq, p.dv, p.part, p.nm, p.nb, p.nspin, p.nspinor,
-                                 p.nmag, p.N, p.chunks_per_split, it.tile, it.split, it.spin);
+            contract_sep_tile<T, S>(p.psik, p.psikq, p.dv, p.part, p.nm, p.nb, p.nspin,
+                                    p.nspinor, p.nmag, p.N, p.chunks_per_split, it.tile,
+                                    it.split, it.spin);
         else
-            contract_tile<T>(p.psik, p.psikq, p.dv, p.part, p.nm, p.nb, p.nspin, p.nspinor,
-                             p.nmag, p.N, p.chunks_per_split, it.tile, it.split, it.spin);
+            contract_tile<T, S>(p.psik, p.psikq, p.dv, p.part, p.nm, p.nb, p.nspin, p.nspinor,
+                                p.nmag, p.N, p.chunks_per_split, it.tile, it.split, it.spin);
     }
 }
 

@@ -83,7 +83,7 @@ struct lelph_ctx
     int device = 0;
     hipStream_t stream = nullptr;
     int precision = LELPH_FP64;
-    lelph_tuning tune{0, 0, 0};
+    lelph_tuning tune{0, 0, 0, 0};
 
     // lattice (host copy)
     lelph_lattice lat{};
@@ -614,6 +614,88 @@ static bool contract_sep_form(const lelph_ctx* c)
            c->tune.fp32_form == 0;
 }
 
+// block tile of the FP64 / FP32 contraction (contract.hpp): 1 CtWide (2
+// modes x 32 x 64), 2 Ct32 (4 x 32 x 32), 3 Ct16x48 (4 x 16 x 48), 4 Ct16 (4 x
+// 16 x 16).  All give bit-identical g; they differ in padding.
+static const int CT_SHAPES[4][3] = {{2, 32, 64}, {4, 32, 32}, {4, 16, 48}, {4, 16, 16}};
+
+static int64_t ct_tiles(int shape, int nm, int nb)
+{
+    const int* s = CT_SHAPES[shape - 1];
+    return (int64_t)cdiv(nm, s[0]) * cdiv(nb, s[1]) * cdiv(nb, s[2]);
+}
+
+// the shape with the least padding for (nm, nb), weighed by how well its
+// wave tiles reuse their fragments (a 16 x 16 wave tile reads twice the
+// LDS per MFMA of a 32 x 32 one); ties go to the larger tiles
+static int ct_auto(int nm, int nb)
+{
+    static const double reuse[4] = {1.0, 0.97, 0.85, 0.7};
+    static const int order[4] = {1, 3, 2, 4};
+    int best = 1;
+    double bs = -1.0;
+    for (int k : order)
+    {
+        const int* s = CT_SHAPES[k - 1];
+        const double util = (double)nm / (cdiv(nm, s[0]) * s[0]) *
+                            ((double)nb / (cdiv(nb, s[1]) * s[1])) *
+                            ((double)nb / (cdiv(nb, s[2]) * s[2]));
+        const double sc = util * reuse[k - 1];
+        if (sc > bs * 1.01)
+        {
+            bs = sc;
+            best = k;
+        }
+    }
+    return best;
+}
+
+static int ct_shape(const lelph_ctx* c)
+{
+    return c->tune.ctile ? c->tune.ctile : ct_auto(c->lat.nmodes, c->lat.nbnds);
+}
+
+template <typename T, typename S>
+static void launch_contract_s(bool sep, dim3 grid, hipStream_t st, const cpx<T>* psik,
+                              const cpx<T>* psikq, const cpx<T>* dv, z64* part, int nm, int nb,
+                              int nspin, int nspinor, int nmag, int64_t N, int64_t cps)
+{
+    if constexpr (std::is_same<T, float>::value)
+        if (sep)
+        {
+            hipLaunchKernelGGL((k_contract_sep<T, S>), grid, dim3(256), 0, st, psik, psikq, dv,
+                               part, nm, nb, nspin, nspinor, nmag, N, cps);
+            return;
+        }
+    hipLaunchKernelGGL((k_contract<T, S>), grid, dim3(256), 0, st, psik, psikq, dv, part, nm, nb,
+                       nspin, nspinor, nmag, N, cps);
+}
+
+template <typename T>
+static void launch_contract(int shape, bool sep, dim3 grid, hipStream_t st, const cpx<T>* psik,
+                            const cpx<T>* psikq, const cpx<T>* dv, z64* part, int nm, int nb,
+                            int nspin, int nspinor, int nmag, int64_t N, int64_t cps)
+{
+    switch (shape)
+    {
+    case 2:
+        launch_contract_s<T, Ct32>(sep, grid, st, psik, psikq, dv, part, nm, nb, nspin, nspinor,
+                                   nmag, N, cps);
+        break;
+    case 3:
+        launch_contract_s<T, Ct16x48>(sep, grid, st, psik, psikq, dv, part, nm, nb, nspin, nspinor,
+                                   nmag, N, cps);
+        break;
+    case 4:
+        launch_contract_s<T, Ct16>(sep, grid, st, psik, psikq, dv, part, nm, nb, nspin, nspinor,
+                                   nmag, N, cps);
+        break;
+    default:
+        launch_contract_s<T, CtWide>(sep, grid, st, psik, psikq, dv, part, nm, nb, nspin,
+                                     nspinor, nmag, N, cps);
+    }
+}
+
 // local part into g (device): elphLocal
 template <typename T>
 static int run_local(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64* g,
@@ -627,7 +709,10 @@ static int run_local(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64*
     const int nm = L.nmodes, nb = L.nbnds, nspin = L.nspin;
     const int nmag_eff = (L.nspinor == 1 && L.nmag == 4) ? 1 : L.nmag;
     const int64_t nchunks = cdiv(c->nf, CT_KC);
-    const int64_t tiles = (int64_t)cdiv(nm, CT_VB) * cdiv(nb, CT_NB) * cdiv(nb, CT_MB);
+    // the K splits follow the wide tile's count whatever the shape, so every
+    // shape reduces the same K ranges (bit-identical g)
+    const int64_t tiles = ct_tiles(1, nm, nb);
+    const int shape = ct_shape(c);
     const int64_t gsz = (int64_t)nm * nspin * nb * nb;
     int64_t cps;
     int nsplit;
@@ -637,17 +722,10 @@ static int run_local(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64*
     // spinor one (87.0 -> 75.5 ms per c2 pair) or the tuning asks for the
     // element-wise form; FP64 forms A element by element (its FP64 products
     // beside the FP64 MFMAs made the separate form 21 % slower)
-    if (!contract_sep_form<T>(c))
-        hipLaunchKernelGGL(k_contract<T>, dim3((unsigned)tiles, (unsigned)nsplit, (unsigned)nspin),
-                           dim3(256), 0, c->stream, c->box_k.as<cpx<T>>(),
-                           c->box_kq.as<cpx<T>>(), dv_ptr<T>(c), c->part.as<z64>(), nm, nb,
-                           nspin, L.nspinor, nmag_eff, c->nf, cps);
-    else
-        hipLaunchKernelGGL(k_contract_sep<T>, dim3((unsigned)tiles, (unsigned)nsplit,
-                                                   (unsigned)nspin),
-                           dim3(256), 0, c->stream, c->box_k.as<cpx<T>>(),
-                           c->box_kq.as<cpx<T>>(), dv_ptr<T>(c), c->part.as<z64>(), nm, nb,
-                           nspin, L.nspinor, nmag_eff, c->nf, cps);
+    launch_contract<T>(shape, contract_sep_form<T>(c),
+                       dim3((unsigned)ct_tiles(shape, nm, nb), (unsigned)nsplit, (unsigned)nspin),
+                       c->stream, c->box_k.as<cpx<T>>(), c->box_kq.as<cpx<T>>(), dv_ptr<T>(c),
+                       c->part.as<z64>(), nm, nb, nspin, L.nspinor, nmag_eff, c->nf, cps);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[2], c->stream));
     hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)cdiv(gsz, 256)), dim3(256), 0, c->stream,
@@ -1241,6 +1319,7 @@ int lelph_set_tuning(lelph_ctx* c, const lelph_tuning* t)
     if (t->fft_lines != 0 && t->fft_lines != 8 && t->fft_lines != 16 && t->fft_lines != 32)
         return LELPH_ERR_ARG;
     if (t->fp32_form != 0 && t->fp32_form != 1) return LELPH_ERR_ARG;
+    if (t->ctile < 0 || t->ctile > 4) return LELPH_ERR_ARG;
     c->tune = *t;
     return LELPH_OK;
 }
@@ -1761,6 +1840,7 @@ struct MultiPair
     int nsplit = 1;
     int64_t cps = 1, gsz = 0, tiles = 0;
     size_t bb = 0;     // bytes of one psi box
+    int shape = 1;     // contraction tile shape (ct_shape of its context)
     int64_t link = -1; // the pair before it in its k -> k+q chain (index in pairs)
     int g0[3] = {0, 0, 0}; // umklapp vector of its k+q side
     void* kbox = nullptr;  // its psi_k box: box_k, or the linked pair's box_kq
@@ -1782,9 +1862,17 @@ struct lelph_multi
     std::vector<DBuf> gdev;    // per cell: (npairs, nmodes, nspin, nb, nb) complex128
     std::vector<void*> ghost;  // per cell: pinned copy of gdev
     std::vector<size_t> gbytes;
-    DBuf qp64, qi64, qp32, qi32, counter;
-    int n64 = 0, n32 = 0, grid64 = 0, grid32 = 0;
-    bool sep32 = false;
+    // one work queue per (precision, tile shape): pair table, items, size,
+    // grid; counter holds one int per queue
+    struct Queue
+    {
+        int prec = LELPH_FP64, shape = 1;
+        bool sep = false;
+        DBuf qp, qi;
+        int n = 0, grid = 0;
+    };
+    std::vector<Queue> queues;
+    DBuf counter;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // the non-local parts run on a second stream: forked at the start,
     // their mode rotations wait for the reductions, joined before the copy
@@ -1794,6 +1882,48 @@ struct lelph_multi
 };
 
 namespace {
+template <typename T, bool SEP, typename S>
+static const void* queue_kernel()
+{
+    return reinterpret_cast<const void*>(k_contract_queue<T, SEP, S>);
+}
+
+// the work-queue kernel of (precision, separate staging, tile shape)
+static const void* queue_fn(int prec, bool sep, int shape)
+{
+    if (prec == LELPH_FP64)
+        switch (shape)
+        {
+        case 2: return queue_kernel<double, false, Ct32>();
+        case 3: return queue_kernel<double, false, Ct16x48>();
+        case 4: return queue_kernel<double, false, Ct16>();
+        default: return queue_kernel<double, false, CtWide>();
+        }
+    if (sep)
+        switch (shape)
+        {
+        case 2: return queue_kernel<float, true, Ct32>();
+        case 3: return queue_kernel<float, true, Ct16x48>();
+        case 4: return queue_kernel<float, true, Ct16>();
+        default: return queue_kernel<float, true, CtWide>();
+        }
+    switch (shape)
+    {
+    case 2: return queue_kernel<float, false, Ct32>();
+    case 3: return queue_kernel<float, false, Ct16x48>();
+    case 4: return queue_kernel<float, false, Ct16>();
+    default: return queue_kernel<float, false, CtWide>();
+    }
+}
+
+static hipError_t launch_queue(int prec, bool sep, int shape, int grid, hipStream_t st,
+                               void* pairs, void* items, int n, int* counter)
+{
+    void* args[] = {&pairs, &items, &n, &counter};
+    return hipLaunchKernel(queue_fn(prec, sep, shape), dim3((unsigned)grid), dim3(256), args, 0,
+                           st);
+}
+
 // issues the whole batch on m->stream (eagerly with events, or under capture)
 static int multi_enqueue(lelph_multi* m, bool timed)
 {
@@ -1849,21 +1979,13 @@ static int multi_enqueue(lelph_multi* m, bool timed)
         }
         if (timed) HIP_TRY(hipEventRecord(m->ev[1], m->stream));
         // the work-queue contractions
-        LELPH_TRY(zero_async(m->counter.p, 16, m->stream));
-        if (m->n64)
-            hipLaunchKernelGGL((k_contract_queue<double, false>), dim3((unsigned)m->grid64),
-                               dim3(256), 0, m->stream, m->qp64.as<QueuePair<double>>(),
-                               m->qi64.as<QueueItem>(), m->n64, m->counter.as<int>());
-        if (m->n32)
+        LELPH_TRY(zero_async(m->counter.p, sizeof(int) * m->queues.size(), m->stream));
+        for (size_t qi = 0; qi < m->queues.size(); ++qi)
         {
-            if (m->sep32)
-                hipLaunchKernelGGL((k_contract_queue<float, true>), dim3((unsigned)m->grid32),
-                                   dim3(256), 0, m->stream, m->qp32.as<QueuePair<float>>(),
-                                   m->qi32.as<QueueItem>(), m->n32, m->counter.as<int>() + 1);
-            else
-                hipLaunchKernelGGL((k_contract_queue<float, false>), dim3((unsigned)m->grid32),
-                                   dim3(256), 0, m->stream, m->qp32.as<QueuePair<float>>(),
-                                   m->qi32.as<QueueItem>(), m->n32, m->counter.as<int>() + 1);
+            auto& qu = m->queues[qi];
+            if (qu.n)
+                HIP_TRY(launch_queue(qu.prec, qu.sep, qu.shape, qu.grid, m->stream, qu.qp.p,
+                                     qu.qi.p, qu.n, m->counter.as<int>() + qi));
         }
         HIP_TRY(hipGetLastError());
         if (timed) HIP_TRY(hipEventRecord(m->ev[2], m->stream));
@@ -1908,11 +2030,11 @@ static int multi_enqueue(lelph_multi* m, bool timed)
     return st;
 }
 
-template <typename T, bool SEP>
-static int queue_grid(int nitems, int dev, int* grid)
+static int queue_grid(int prec, bool sep, int shape, int nitems, int dev, int* grid)
 {
     int occ = 0, ncu = 0;
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_contract_queue<T, SEP>, 256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, queue_fn(prec, sep, shape), 256,
+                                                         0));
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     *grid = std::max(1, std::min(nitems, std::max(1, occ) * std::max(1, ncu)));
     return LELPH_OK;
@@ -1920,14 +2042,15 @@ static int queue_grid(int nitems, int dev, int* grid)
 
 // the pair and item lists of one precision's queue, items largest first
 template <typename T>
-static int build_queue(lelph_multi* m, int prec, DBuf& qp, DBuf& qi, int* nitems, bool* any_sep)
+static int build_queue(lelph_multi* m, int prec, int shape, DBuf& qp, DBuf& qi, int* nitems,
+                       bool* any_sep)
 {
     std::vector<QueuePair<T>> hp;
     std::vector<std::pair<int64_t, QueueItem>> items;
     *any_sep = false;
     for (auto& p : m->pairs)
     {
-        if (p.prec != prec) continue;
+        if (p.prec != prec || p.shape != shape) continue;
         lelph_ctx* c = m->ctx[p.cell];
         const lelph_lattice& L = c->lat;
         const int nmag_eff = (L.nspinor == 1 && L.nmag == 4) ? 1 : L.nmag;
@@ -1990,10 +2113,11 @@ static void multi_free(lelph_multi* m)
     for (auto& g : m->gdev) g.release();
     for (void* h : m->ghost)
         if (h) (void)hipHostFree(h);
-    m->qp64.release();
-    m->qi64.release();
-    m->qp32.release();
-    m->qi32.release();
+    for (auto& qu : m->queues)
+    {
+        qu.qp.release();
+        qu.qi.release();
+    }
     m->counter.release();
     for (auto& e : m->ev)
         if (e) (void)hipEventDestroy(e);
@@ -2068,8 +2192,8 @@ int lelph_multi_create(int ncells, lelph_ctx* const* ctxs, const double* qpts,
         const int64_t nbox = (int64_t)L.nspin * L.nbnds * L.nspinor;
         const size_t bb = (prec == LELPH_FP64 ? sizeof(z64) : sizeof(c32)) * (size_t)(nbox * c->nf);
         const int64_t nchunks = cdiv(c->nf, CT_KC);
-        const int64_t tiles =
-            (int64_t)cdiv(L.nmodes, CT_VB) * cdiv(L.nbnds, CT_NB) * cdiv(L.nbnds, CT_MB);
+        const int64_t tiles = ct_tiles(1, L.nmodes, L.nbnds); // sets the K splits
+        const int shape = ct_shape(c);
         for (int64_t j = 0; j < npairs[i]; ++j)
         {
             MultiPair p;
@@ -2079,7 +2203,8 @@ int lelph_multi_create(int ncells, lelph_ctx* const* ctxs, const double* qpts,
             p.prec = prec;
             p.sep = prec == LELPH_FP32 ? (contract_sep_form<float>(c) ? 1 : 0) : 0;
             p.gsz = gsz;
-            p.tiles = tiles;
+            p.shape = shape;
+            p.tiles = ct_tiles(shape, L.nmodes, L.nbnds);
             pick_splits(tiles * L.nspin, nchunks, gsz, &p.cps, &p.nsplit, c->tune.split_target);
             p.bb = bb;
             const int64_t es = (int64_t)L.nspin * L.nbnds * L.nbnds;
@@ -2142,14 +2267,20 @@ int lelph_multi_create(int ncells, lelph_ctx* const* ctxs, const double* qpts,
             }
         }
     }
-    bool sep64 = false;
-    int st = build_queue<double>(m, LELPH_FP64, m->qp64, m->qi64, &m->n64, &sep64);
-    if (st == LELPH_OK) st = build_queue<float>(m, LELPH_FP32, m->qp32, m->qi32, &m->n32, &m->sep32);
-    if (st == LELPH_OK && m->n64) st = queue_grid<double, false>(m->n64, dev, &m->grid64);
-    if (st == LELPH_OK && m->n32)
-        st = m->sep32 ? queue_grid<float, true>(m->n32, dev, &m->grid32)
-                      : queue_grid<float, false>(m->n32, dev, &m->grid32);
-    if (st == LELPH_OK && m->counter.ensure(16)) st = LELPH_ERR_ALLOC;
+    int st = LELPH_OK;
+    for (int prec : {LELPH_FP64, LELPH_FP32})
+        for (int shape = 1; shape <= 4 && st == LELPH_OK; ++shape)
+        {
+            m->queues.emplace_back();
+            auto& qu = m->queues.back();
+            qu.prec = prec;
+            qu.shape = shape;
+            st = prec == LELPH_FP64
+                     ? build_queue<double>(m, prec, shape, qu.qp, qu.qi, &qu.n, &qu.sep)
+                     : build_queue<float>(m, prec, shape, qu.qp, qu.qi, &qu.n, &qu.sep);
+            if (st == LELPH_OK && qu.n) st = queue_grid(prec, qu.sep, shape, qu.n, dev, &qu.grid);
+        }
+    if (st == LELPH_OK && m->counter.ensure(sizeof(int) * m->queues.size())) st = LELPH_ERR_ALLOC;
     if (st != LELPH_OK) return fail(st);
     // eager pass: every workspace of the contexts reaches its size here,
     // so the captured pass allocates nothing
@@ -2208,10 +2339,13 @@ int lelph_multi_timings(lelph_multi* m, double* ms)
 {
     if (!m || !ms) return LELPH_ERR_ARG;
     for (int k = 0; k < 4; ++k) ms[k] = m->ms[k];
-    ms[4] = (double)m->n64;
-    ms[5] = (double)m->n32;
-    ms[6] = (double)m->grid64;
-    ms[7] = (double)m->grid32;
+    for (int k = 4; k < 8; ++k) ms[k] = 0;
+    for (const auto& qu : m->queues)
+    {
+        const int f = qu.prec == LELPH_FP64 ? 0 : 1;
+        ms[4 + f] += qu.n;
+        ms[6 + f] = std::max(ms[6 + f], (double)qu.grid);
+    }
     return LELPH_OK;
 }
 

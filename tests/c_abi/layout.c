@@ -63,6 +63,7 @@ int main(void)
     F(lelph_tuning, split_target);
     F(lelph_tuning, fft_lines);
     F(lelph_tuning, fp32_form);
+    F(lelph_tuning, ctile);
     Z(lelph_longrange);
     F(lelph_longrange, epsilon);
     F(lelph_longrange, Zborn);
