@@ -625,11 +625,16 @@ static int64_t ct_tiles(int shape, int nm, int nb)
     return (int64_t)cdiv(nm, s[0]) * cdiv(nb, s[1]) * cdiv(nb, s[2]);
 }
 
-// the shape with the least padding for (nm, nb), weighed by how well its
-// wave tiles reuse their fragments (a 16 x 16 wave tile reads twice the
-// LDS per MFMA of a 32 x 32 one); ties go to the larger tiles
-static int ct_auto(int nm, int nb)
+// FP32: the 16 x 16 tile (its blocks are small enough for many waves per
+// SIMD: 57.0 against 76.5 ms per c2 pair with the wide tile, and the
+// autotuner's pick on the 8-cell workload's FP32 grids).  FP64: the shape
+// with the least padding for (nm, nb), weighed by how well its wave tiles
+// reuse their fragments (a 16 x 16 wave tile reads twice the LDS per MFMA
+// of a 32 x 32 one); ties go to the larger tiles (the wide tile at c2:
+// 129.5 ms, the others 137..141 ms)
+static int ct_auto(int nm, int nb, bool fp32)
 {
+    if (fp32) return 4;
     static const double reuse[4] = {1.0, 0.97, 0.85, 0.7};
     static const int order[4] = {1, 3, 2, 4};
     int best = 1;
@@ -650,9 +655,9 @@ static int ct_auto(int nm, int nb)
     return best;
 }
 
-static int ct_shape(const lelph_ctx* c)
+static int ct_shape(const lelph_ctx* c, bool fp32)
 {
-    return c->tune.ctile ? c->tune.ctile : ct_auto(c->lat.nmodes, c->lat.nbnds);
+    return c->tune.ctile ? c->tune.ctile : ct_auto(c->lat.nmodes, c->lat.nbnds, fp32);
 }
 
 template <typename T, typename S>
@@ -712,7 +717,7 @@ static int run_local(lelph_ctx* c, const double* qpt, const lelph_pair& pr, z64*
     // the K splits follow the wide tile's count whatever the shape, so every
     // shape reduces the same K ranges (bit-identical g)
     const int64_t tiles = ct_tiles(1, nm, nb);
-    const int shape = ct_shape(c);
+    const int shape = ct_shape(c, std::is_same<T, float>::value);
     const int64_t gsz = (int64_t)nm * nspin * nb * nb;
     int64_t cps;
     int nsplit;
@@ -2193,7 +2198,7 @@ int lelph_multi_create(int ncells, lelph_ctx* const* ctxs, const double* qpts,
         const size_t bb = (prec == LELPH_FP64 ? sizeof(z64) : sizeof(c32)) * (size_t)(nbox * c->nf);
         const int64_t nchunks = cdiv(c->nf, CT_KC);
         const int64_t tiles = ct_tiles(1, L.nmodes, L.nbnds); // sets the K splits
-        const int shape = ct_shape(c);
+        const int shape = ct_shape(c, prec == LELPH_FP32);
         for (int64_t j = 0; j < npairs[i]; ++j)
         {
             MultiPair p;
