@@ -433,23 +433,38 @@ def test_soc_nonlocal_reduces_to_scalar():
 
 
 def _dd_short_and_self(s, lr, q_crys, eta, rmax=36.0):
-    """Real-space part and self term of the Ewald sum of the dipole-dipole
-    dynamical matrix (Gonze & Lee, PRB 55, 10355 (1997), sec. IV), derived
-    here from the split 1/(sqrt(det eps) D) = [erf(L D) + erfc(L D)] /
-    (sqrt(det eps) D), D^2 = d.eps^-1.d, L^2 = eta, e^2 = 2 (Rydberg), on the
-    lattice whose reciprocal vectors are the system's blat (which carries the
-    reference's truncated pi): returns sum'_R F_short(tau_a - tau_b + R)
-    e^{-i q.R} - delta_ab F_long(0), F = -Z_a^T (Hessian of the potential) Z_b,
-    with the index order of the oracle's dynamical matrix [(a, i), (b, j)]."""
+    """Real-space part and self term of the Ewald sum of the multipole
+    (Born charge + quadrupole) dynamical matrix (Gonze & Lee, PRB 55, 10355
+    (1997), sec. IV; the quadrupole terms as in Smith's multipole Ewald sum),
+    derived here from the split 1/(sqrt(det eps) D) = [erf(L D) + erfc(L D)]
+    / (sqrt(det eps) D), D^2 = d.M.d, M = eps^-1, L^2 = eta, e^2 = 2
+    (Rydberg), on the lattice whose reciprocal vectors are the system's blat
+    (which carries the reference's truncated pi).  An atom's displacement
+    charge in reciprocal space is X(K) = Z^T K - (i/2) K.Q.K, so the pair
+    kernel conj(X_a)_i X_b_j is a polynomial in K of degree 2..4 and, with
+    K -> -i grad, the real-space kernel is
+      - Z_a,ki Z_b,lj d_kl phi + 1/2 Z_a,ki Q_b,lmj d_klm phi
+      - 1/2 Q_a,lmi Z_b,kj d_klm phi + 1/4 Q_a,lmi Q_b,npj d_lmnp phi,
+    with the derivatives of phi = g(u = D^2) / sqrt(det eps) from the B_n
+    recursion (B_0 = erfc(L D) / D, B_n = [(2n-1) B_(n-1) + (2 L^2)^n
+    e^(-L^2 D^2) / (L sqrt(pi))] / D^2, and 2 g1 = -B_1, 4 g2 = B_2,
+    8 g3 = -B_3, 16 g4 = B_4 for the u-derivatives g1 .. g4) and w = M d.
+    Returns sum'_R F_short(tau_a - tau_b + R) e^(-i q.R) minus the
+    long-range part's own term at d = 0 for a = b (its second and fourth
+    derivatives there: -(4 L^3 / (3 sqrt(pi))) M and (8 L^5 / (5 sqrt(pi)))
+    times the three pairings M M), in the index order of the oracle's
+    dynamical matrix [(a, i), (b, j)]."""
     from scipy.special import erfc
     e2 = 2.0
     B = np.asarray(s.blat, np.float64)
     A = 2 * np.pi * np.linalg.inv(B).T  # columns a_j with a_j . b_i = 2 pi delta_ij
     eps = np.asarray(lr.epsilon, np.float64)
-    ei = np.linalg.inv(eps)
+    M = np.linalg.inv(eps)
     sq = np.sqrt(np.linalg.det(eps))
     L = np.sqrt(eta)
     Z = np.asarray(lr.Zborn, np.float64)  # Z[a, k, i]: (Z^T K)_i = sum_k Z[a, k, i] K_k
+    # Q[a, i, j, k]: (K.Q.K)_k = sum_ij Q[a, i, j, k] K_i K_j
+    Q = None if lr.Qpole is None else np.asarray(lr.Qpole, np.float64)
     tau = np.asarray(s.atomic_pos, np.float64)
     na = s.natom
     nmax = int(np.ceil(rmax / np.min(np.linalg.norm(A, axis=0)))) + 1
@@ -457,29 +472,50 @@ def _dd_short_and_self(s, lr, q_crys, eta, rmax=36.0):
     n = np.stack(np.meshgrid(rng_, rng_, rng_, indexing="ij"), -1).reshape(-1, 3)
     R = n @ A.T
     phase = np.exp(-2j * np.pi * (n @ np.asarray(q_crys, np.float64)))
+
+    def sym_m_ww(w):  # the 6 placements of M among 4 indices, w on the other two
+        return (np.einsum("kn,xl,xm->xklmn", M, w, w) + np.einsum("ln,xk,xm->xklmn", M, w, w)
+                + np.einsum("mn,xk,xl->xklmn", M, w, w) + np.einsum("km,xl,xn->xklmn", M, w, w)
+                + np.einsum("lm,xk,xn->xklmn", M, w, w) + np.einsum("kl,xm,xn->xklmn", M, w, w))
+    MM = (np.einsum("km,ln->klmn", M, M) + np.einsum("lm,kn->klmn", M, M)
+          + np.einsum("kl,mn->klmn", M, M))
     out = np.zeros((na, 3, na, 3), np.complex128)
     for a in range(na):
         for b in range(na):
             d = tau[a] - tau[b] + R
             keep = np.linalg.norm(d, axis=1) > 1e-9
             d, ph = d[keep], phase[keep]
-            Dl = d @ ei  # Delta = eps^-1 d
-            D = np.sqrt(np.einsum("ni,ni->n", d, Dl))
-            ec = erfc(L * D)
-            ex = (2 * L / np.sqrt(np.pi)) * np.exp(-(L * D) ** 2)
-            f1 = -ec / D ** 2 - ex / D
-            f2 = 2 * ec / D ** 3 + ex * (2 / D ** 2 + 2 * L ** 2)
-            H = (np.einsum("nk,nl->nkl", Dl, Dl) * ((f2 - f1 / D) / D ** 2)[:, None, None]
-                 + ei[None] * (f1 / D)[:, None, None]) / sq
-            Hs = np.einsum("nkl,n->kl", H, ph)
-            out[a, :, b, :] = -np.einsum("ki,kl,lj->ij", Z[a], Hs, Z[b])
-        self_long = (4 * L ** 3 / (3 * np.sqrt(np.pi) * sq)) * np.einsum("ki,kl,lj->ij", Z[a], ei,
-                                                                         Z[a])
-        out[a, :, a, :] -= self_long
+            w = d @ M
+            D = np.sqrt(np.einsum("xi,xi->x", d, w))
+            ex = np.exp(-(L * D) ** 2) / (L * np.sqrt(np.pi))
+            Bn = [erfc(L * D) / D]
+            for k in range(1, 5):
+                Bn.append(((2 * k - 1) * Bn[-1] + (2 * L ** 2) ** k * ex) / D ** 2)
+            d2 = (np.einsum("x,xk,xl->xkl", Bn[2], w, w) - Bn[1][:, None, None] * M[None]) / sq
+            F = -np.einsum("ki,xkl,lj->xij", Z[a], d2, Z[b])
+            if Q is not None:
+                d3 = (-np.einsum("x,xk,xl,xm->xklm", Bn[3], w, w, w)
+                      + Bn[2][:, None, None, None]
+                      * (np.einsum("km,xl->xklm", M, w) + np.einsum("lm,xk->xklm", M, w)
+                         + np.einsum("kl,xm->xklm", M, w))) / sq
+                d4 = (np.einsum("x,xk,xl,xm,xn->xklmn", Bn[4], w, w, w, w)
+                      - Bn[3][:, None, None, None, None] * sym_m_ww(w)
+                      + Bn[2][:, None, None, None, None] * MM[None]) / sq
+                F = F + 0.5 * np.einsum("ki,lmj,xklm->xij", Z[a], Q[b], d3)
+                F = F - 0.5 * np.einsum("lmi,kj,xklm->xij", Q[a], Z[b], d3)
+                F = F + 0.25 * np.einsum("lmi,npj,xlmnp->xij", Q[a], Q[b], d4)
+            out[a, :, b, :] = np.einsum("xij,x->ij", F, ph)
+        s2 = -(4 * L ** 3 / (3 * np.sqrt(np.pi) * sq)) * M
+        self_ = -np.einsum("ki,kl,lj->ij", Z[a], s2, Z[a])
+        if Q is not None:
+            s4 = (8 * L ** 5 / (5 * np.sqrt(np.pi) * sq)) * MM
+            self_ = self_ + 0.25 * np.einsum("lmi,npj,lmnp->ij", Q[a], Q[a], s4)
+        out[a, :, a, :] -= self_
     return e2 * out.reshape(3 * na, 3 * na)
 
 
-def test_dyn_longrange_ewald_invariance():
+@pytest.mark.parametrize("quad", [False, True])
+def test_dyn_longrange_ewald_invariance(quad):
     """The oracle's long-range dynamical matrix (the reciprocal sum over
     q + G != 0 with the Gaussian e^{-(q+G).eps.(q+G)/(4 eta)} of
     add_ph_dyn_long_range, src/phonon/dyn_long_range.c:125-404) is the
@@ -493,7 +529,7 @@ def test_dyn_longrange_ewald_invariance():
     ELPH_PI: the exact Ewald normalisation of that reciprocal lattice is
     (pi / ELPH_PI) (Omega / Omega'), applied to the oracle's sum."""
     s = synth.tiny(with_wfc=False)
-    lr = _lr_params(s, quad=False)
+    lr = _lr_params(s, quad=quad)
     lr.epsilon = np.array([[5.6, 0.3, -0.1], [0.3, 6.2, 0.2], [-0.1, 0.2, 5.9]])
     q = np.array([0.23, -0.11, 0.17])
     nm = s.nmodes
@@ -501,10 +537,10 @@ def test_dyn_longrange_ewald_invariance():
     B = np.asarray(s.blat, np.float64)
     vol_p = abs(np.linalg.det(2 * np.pi * np.linalg.inv(B).T))
     norm = (np.pi / synth.RPI) * (s.volume / vol_p)
-    say("case: Ewald invariance of the long-range dynamical matrix")
+    say(f"case: Ewald invariance of the long-range dynamical matrix, quadrupoles {quad}")
     tot, rec = [], []
     for eta in (0.6, 1.0, 1.8):
-        od = O.OracleDynInterp(s, (1, 1, 1), masses=None, lr=lr, eta_ph=eta, ecut=200.0,
+        od = O.OracleDynInterp(s, (1, 1, 1), masses=None, lr=lr, eta_ph=eta, ecut=320.0,
                                asr="none")
         r = od.lr_term(q, z, 1, masses=False, asr=False) * norm
         rec.append(r)
@@ -518,3 +554,18 @@ def test_dyn_longrange_ewald_invariance():
     assert rec_change > 1e-1
     assert spread < 1e-10
     assert planted > 1e-4
+    if quad:
+        # quadrupoles of the wrong sign in the reciprocal sum only (the
+        # dipole-quadrupole cross terms flip) break the invariance
+        import copy
+        lr2 = copy.copy(lr)
+        lr2.Qpole = -np.asarray(lr.Qpole)
+        t2 = []
+        for eta in (0.6, 1.8):
+            od = O.OracleDynInterp(s, (1, 1, 1), masses=None, lr=lr2, eta_ph=eta, ecut=320.0,
+                                   asr="none")
+            t2.append(od.lr_term(q, z, 1, masses=False, asr=False) * norm
+                      + _dd_short_and_self(s, lr, q, eta))
+        wrong = np.abs(t2[1] - t2[0]).max() / scale
+        say(f"  quadrupole sign flipped in the reciprocal sum: spread {wrong:.2e}")
+        assert wrong > 1e-4
