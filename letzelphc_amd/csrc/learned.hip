@@ -17,10 +17,12 @@
 // padding on the FFT grid, channels NN_CIN -> NN_W -> ... -> NN_W ->
 // nout (padded to NN_W), SiLU between layers, linear output.  Operands in
 // BF16, MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation, FP32
-// master weights, Adam.  Each convolution is an explicit im2col followed by
-// one GEMM; its backward is two GEMMs (the weight gradient with split-K
-// over the points, the column gradient) and a gather col2im.  Every
-// reduction has a fixed order, so a run is deterministic.
+// master weights, Adam.  Each convolution is an implicit GEMM (k_nn_conv:
+// the A fragments read from the 27 neighbours' channel rows); its backward
+// is the input gradient as the same implicit GEMM over the mirrored taps
+// and the weight gradient as one GEMM (split over the points) on transposed
+// columns built through LDS (k_nn_im2col_t).  Every reduction has a fixed
+// order, so a run is deterministic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -92,22 +94,6 @@ __device__ inline int64_t nn_nbr(const NnGrid& g, int64_t p, int t, int sgn)
     const int dx = sgn * (t / 9 - 1), dy = sgn * ((t / 3) % 3 - 1), dz = sgn * (t % 3 - 1);
     const int jx = (ix + dx + g.nx) % g.nx, jy = (iy + dy + g.ny) % g.ny, jz = (iz + dz + g.nz) % g.nz;
     return s * g.n + ((int64_t)jx * g.ny + jy) * g.nz + jz;
-}
-
-// col[p][t * C + c] = A[nbr(p, t)][c], 8 channels (16 bytes) per thread
-__global__ void k_nn_im2col(const __bf16* __restrict__ A, int C, NnGrid g, int64_t P,
-                            __bf16* __restrict__ col)
-{
-    const int cu = C / 8;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t total = P * NN_TAPS * cu;
-    if (i >= total) return;
-    const int u = (int)(i % cu);
-    const int t = (int)((i / cu) % NN_TAPS);
-    const int64_t p = i / ((int64_t)cu * NN_TAPS);
-    const int64_t q = nn_nbr(g, p, t, 1);
-    *reinterpret_cast<uint4*>(col + p * NN_TAPS * C + (int64_t)t * C + 8 * u) =
-        *reinterpret_cast<const uint4*>(A + q * C + 8 * u);
 }
 
 // colT[t * C + c][p] = A[nbr(p, t)][c] (C = 64 or 128).  Block: 64 points
@@ -214,6 +200,64 @@ __global__ void k_nn_sum_splits(const float* __restrict__ part, int ns, int64_t 
     out[i] = a;
 }
 
+// Implicit-GEMM convolution: out[p][n] = sum_t sum_c X[nbr(p, sgn t)][c]
+// B[n * bn + t * bt + c] over the 27 taps t and CR channels c (a multiple
+// of 32), with the tiling, fragment layout and k order (t, then c) of
+// k_nn_gemm, the A fragments read straight from the neighbours' channel rows
+// (16 bytes of one row per lane) instead of from a column buffer.
+//   forward: X = the layer input (CR = C), sgn = +1, B = Wt (bn = 27 C, bt =
+//            C): the same products in the same order as k_nn_gemm on
+//            explicit columns col[p][t C + c] = X[nbr(p, t)][c] would issue;
+//   input gradient: X = dZ (CR = 64), sgn = -1, B = Wk (bn = 64, bt = 64 C):
+//            dA[p][ci] = sum_t sum_c dZ[nbr(p, -t)][c] W[t C + ci][c].
+// P a multiple of 64, N of 64 (checked by the host).
+__global__ __launch_bounds__(256) void k_nn_conv(const __bf16* __restrict__ X, int CR, int sgn,
+                                                 NnGrid g, const __bf16* __restrict__ B,
+                                                 int64_t bn, int64_t bt, int64_t N,
+                                                 float* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t ntn = N / 64;
+    const int64_t tm = blockIdx.x / ntn, tn = blockIdx.x % ntn;
+    const int64_t m0 = tm * 64 + (wave >> 1) * 32, n0 = tn * 64 + (wave & 1) * 32;
+    nn_flt4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = (nn_flt4){0.f, 0.f, 0.f, 0.f};
+    const int r = lane & 15, kq = 8 * (lane >> 4);
+    const int64_t p0 = m0 + r, p1 = m0 + 16 + r;
+    for (int t = 0; t < NN_TAPS; ++t)
+    {
+        const __bf16* xa0 = X + nn_nbr(g, p0, t, sgn) * CR + kq;
+        const __bf16* xa1 = X + nn_nbr(g, p1, t, sgn) * CR + kq;
+        const __bf16* pb0 = B + (n0 + r) * bn + (int64_t)t * bt + kq;
+        const __bf16* pb1 = B + (n0 + 16 + r) * bn + (int64_t)t * bt + kq;
+        for (int c = 0; c < CR; c += 32)
+        {
+            const nn_bf16x8 a0 = *reinterpret_cast<const nn_bf16x8*>(xa0 + c);
+            const nn_bf16x8 a1 = *reinterpret_cast<const nn_bf16x8*>(xa1 + c);
+            const nn_bf16x8 b0 = *reinterpret_cast<const nn_bf16x8*>(pb0 + c);
+            const nn_bf16x8 b1 = *reinterpret_cast<const nn_bf16x8*>(pb1 + c);
+            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+            {
+                const int64_t m = m0 + rb * 16 + 4 * (lane >> 4) + v;
+                const int64_t n = n0 + cb * 16 + r;
+                out[m * N + n] = acc[rb][cb][v];
+            }
+}
+
 __device__ inline float nn_sigmoid(float z) { return 1.f / (1.f + __expf(-z)); }
 
 // Z[p][c] += b[c]; hidden layers also write A = SiLU(Z) in BF16
@@ -272,7 +316,7 @@ __global__ void k_nn_silu_bwd(float* __restrict__ d, const float* __restrict__ Z
 // dZ (FP32, [P][NN_W]) -> BF16 [P][NN_W] and transposed [NN_W][P], through a
 // 64 x 64 LDS tile
 __global__ void k_nn_dz_pack(const float* __restrict__ dZf, int64_t P, __bf16* __restrict__ dZ,
-                             __bf16* __restrict__ dZT)
+                             __bf16* __restrict__ dZT, float* __restrict__ bpart)
 {
     __shared__ float tile[64][NN_W + 1];
     const int64_t p0 = (int64_t)blockIdx.x * 64;
@@ -284,6 +328,13 @@ __global__ void k_nn_dz_pack(const float* __restrict__ dZf, int64_t P, __bf16* _
         dZ[(p0 + pp) * NN_W + c] = (__bf16)v;
     }
     __syncthreads();
+    // the bias gradient's partial of these 64 points, in point order
+    if (threadIdx.x < NN_W)
+    {
+        float s = 0.f;
+        for (int pp = 0; pp < 64; ++pp) s += tile[pp][threadIdx.x];
+        bpart[blockIdx.x * NN_W + threadIdx.x] = s;
+    }
     for (int f = threadIdx.x; f < 64 * NN_W; f += blockDim.x)
     {
         const int c = f / 64, pp = f % 64;
@@ -291,35 +342,19 @@ __global__ void k_nn_dz_pack(const float* __restrict__ dZf, int64_t P, __bf16* _
     }
 }
 
-// gb[c] = sum_p dZ[p][c], one block per channel, fixed-order tree
-__global__ void k_nn_bias_grad(const float* __restrict__ dZ, int64_t P, float* __restrict__ gb)
+// gb[c] = sum over the 64-point partials of k_nn_dz_pack, one block of 256
+// threads: 4 groups of 64 (one channel per thread, coalesced rows), each
+// summing every 4th partial in order, then the 4 groups in order
+__global__ __launch_bounds__(256) void k_nn_bias_grad(const float* __restrict__ bpart,
+                                                      int64_t nblk, float* __restrict__ gb)
 {
-    __shared__ float red[256];
-    const int c = blockIdx.x;
+    __shared__ float red[4][NN_W];
+    const int c = threadIdx.x % NN_W, grp = threadIdx.x / NN_W;
     float a = 0.f;
-    for (int64_t p = threadIdx.x; p < P; p += blockDim.x) a += dZ[p * NN_W + c];
-    red[threadIdx.x] = a;
+    for (int64_t b = grp; b < nblk; b += 4) a += bpart[b * NN_W + c];
+    red[grp][c] = a;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1)
-    {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) gb[c] = red[0];
-}
-
-// dA[q][c] = sum_t dcol[nbr(q, -t)][t * C + c] (the transpose of im2col)
-__global__ void k_nn_col2im(const float* __restrict__ dcol, int C, NnGrid g, int64_t P,
-                            float* __restrict__ dA)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P * C) return;
-    const int64_t q = i / C;
-    const int c = (int)(i % C);
-    float a = 0.f;
-    for (int t = 0; t < NN_TAPS; ++t)
-        a += dcol[nn_nbr(g, q, t, -1) * NN_TAPS * C + (int64_t)t * C + c];
-    dA[i] = a;
+    if (threadIdx.x < NN_W) gb[c] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
 }
 
 // Adam on FP32 masters (mask: entries held at zero, the padding)
@@ -453,7 +488,7 @@ struct lelph_nn
     int64_t ntarg = 0;
     // batch work space, sized for smax samples
     int smax = 0;
-    NBuf smp, X0, base, tgt, A[NN_MAXL], Z[NN_MAXL], col, colT, dcol, dZf, dZ, dZT, dA, part, lossp;
+    NBuf smp, X0, base, tgt, A[NN_MAXL], Z[NN_MAXL], colT, dZf, dZ, dZT, dA, part, lossp, bpart;
     std::vector<float> lossh;
 };
 
@@ -492,15 +527,14 @@ static int nn_ensure_batch(lelph_nn* m, int S)
         LELPH_TRY(m->A[l].ensure(sizeof(__bf16) * P * NN_W));
         LELPH_TRY(m->Z[l].ensure(sizeof(float) * P * NN_W));
     }
-    LELPH_TRY(m->col.ensure(sizeof(__bf16) * P * Kmax));
     LELPH_TRY(m->colT.ensure(sizeof(__bf16) * P * Kmax));
-    LELPH_TRY(m->dcol.ensure(sizeof(float) * P * Kmax));
     LELPH_TRY(m->dZf.ensure(sizeof(float) * P * NN_W));
     LELPH_TRY(m->dZ.ensure(sizeof(__bf16) * P * NN_W));
     LELPH_TRY(m->dZT.ensure(sizeof(__bf16) * P * NN_W));
     LELPH_TRY(m->dA.ensure(sizeof(float) * P * NN_W));
     LELPH_TRY(m->part.ensure(sizeof(float) * 32 * Kmax * NN_W));
     LELPH_TRY(m->lossp.ensure(sizeof(float) * cdiv(P * NN_W, 256)));
+    LELPH_TRY(m->bpart.ensure(sizeof(float) * (P / 64) * NN_W));
     m->smax = S;
     return LELPH_OK;
 }
@@ -514,12 +548,10 @@ static int nn_forward(lelph_nn* m, int S)
     {
         const __bf16* in = l == 0 ? m->X0.as<__bf16>() : m->A[l - 1].as<__bf16>();
         const int C = m->cin[l];
-        const int64_t tot = P * NN_TAPS * (C / 8);
-        hipLaunchKernelGGL(k_nn_im2col, dim3((unsigned)cdiv(tot, 256)), dim3(256), 0, m->stream,
-                           in, C, m->g, P, m->col.as<__bf16>());
+        hipLaunchKernelGGL(k_nn_conv, dim3((unsigned)(P / 64)), dim3(256), 0, m->stream, in, C, 1,
+                           m->g, m->Wt[l].as<__bf16>(), m->K[l], (int64_t)C, (int64_t)NN_W,
+                           m->Z[l].as<float>());
         HIP_TRY(hipGetLastError());
-        LELPH_TRY(nn_launch_gemm(m, m->col.as<__bf16>(), m->Wt[l].as<__bf16>(), P, NN_W, m->K[l],
-                                 1, m->Z[l].as<float>(), nullptr));
         const int act = l + 1 < m->L;
         hipLaunchKernelGGL(k_nn_bias_act, dim3((unsigned)cdiv(P * NN_W, 256)), dim3(256), 0,
                            m->stream, m->Z[l].as<float>(), m->b[l].as<float>(), P, act,
@@ -550,9 +582,10 @@ static int nn_backward(lelph_nn* m, int S, double* loss)
             HIP_TRY(hipGetLastError());
         }
         hipLaunchKernelGGL(k_nn_dz_pack, dim3((unsigned)(P / 64)), dim3(256), 0, m->stream,
-                           m->dZf.as<float>(), P, m->dZ.as<__bf16>(), m->dZT.as<__bf16>());
-        hipLaunchKernelGGL(k_nn_bias_grad, dim3(NN_W), dim3(256), 0, m->stream,
-                           m->dZf.as<float>(), P, m->gb[l].as<float>());
+                           m->dZf.as<float>(), P, m->dZ.as<__bf16>(), m->dZT.as<__bf16>(),
+                           m->bpart.as<float>());
+        hipLaunchKernelGGL(k_nn_bias_grad, dim3(1), dim3(256), 0, m->stream, m->bpart.as<float>(),
+                           P / 64, m->gb[l].as<float>());
         HIP_TRY(hipGetLastError());
         const __bf16* in = l == 0 ? m->X0.as<__bf16>() : m->A[l - 1].as<__bf16>();
         const int C = m->cin[l];
@@ -565,11 +598,11 @@ static int nn_backward(lelph_nn* m, int S, double* loss)
                                  ks, m->gW[l].as<float>(), m->part.as<float>()));
         if (l > 0)
         {
-            // dcol[p][k] = sum_c dZ[p][c] Wk[k][c], then col2im into dA
-            LELPH_TRY(nn_launch_gemm(m, m->dZ.as<__bf16>(), m->Wk[l].as<__bf16>(), P, m->K[l], NN_W,
-                                     1, m->dcol.as<float>(), nullptr));
-            hipLaunchKernelGGL(k_nn_col2im, dim3((unsigned)cdiv(P * C, 256)), dim3(256), 0,
-                               m->stream, m->dcol.as<float>(), C, m->g, P, m->dZf.as<float>());
+            // dA[p][ci] = sum_t sum_c dZ[nbr(p, -t)][c] Wk[t C + ci][c]
+            hipLaunchKernelGGL(k_nn_conv, dim3((unsigned)((P / 64) * (C / 64))), dim3(256), 0,
+                               m->stream, m->dZ.as<__bf16>(), NN_W, -1, m->g,
+                               m->Wk[l].as<__bf16>(), (int64_t)NN_W, (int64_t)C * NN_W,
+                               (int64_t)C, m->dZf.as<float>());
             HIP_TRY(hipGetLastError());
         }
     }
@@ -670,8 +703,8 @@ static void nn_free(lelph_nn* m)
 {
     if (!m) return;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
-    NBuf* all[] = {&m->coarse, &m->targets, &m->smp, &m->X0, &m->base, &m->tgt, &m->col,
-                   &m->colT, &m->dcol, &m->dZf, &m->dZ, &m->dZT, &m->dA, &m->part, &m->lossp};
+    NBuf* all[] = {&m->coarse, &m->targets, &m->smp, &m->X0, &m->base, &m->tgt, &m->colT,
+                   &m->dZf, &m->dZ, &m->dZT, &m->dA, &m->part, &m->lossp, &m->bpart};
     for (NBuf* b : all) b->release();
     for (int l = 0; l < NN_MAXL; ++l)
     {
