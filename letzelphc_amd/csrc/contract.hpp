@@ -499,9 +499,23 @@ __global__ void k_reduce_parts(const z64* __restrict__ part, int nsplit, int64_t
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     double re = 0, im = 0;
-    for (int s = 0; s < nsplit; ++s)
+    // eight loads in flight, then their adds in split order (the same sum)
+    int s = 0;
+    for (; s + 8 <= nsplit; s += 8)
     {
-        z64 p = part[(int64_t)s * count + i];
+        z64 p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] = part[(int64_t)(s + u) * count + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            re += p[u].x;
+            im += p[u].y;
+        }
+    }
+    for (; s < nsplit; ++s)
+    {
+        const z64 p = part[(int64_t)s * count + i];
         re += p.x;
         im += p.y;
     }
