@@ -160,6 +160,10 @@ typedef struct lelph_tuning
                          the mode and band counts, 1 two modes x 32 x 64,
                          2 four modes x 32 x 32, 3 four modes x 16 x 48,
                          4 four modes x 16 x 16 (bit-identical g)             */
+    int nl_form;      /* non-local projector GEMMs of bf16x3 / bf16x6
+                         contexts: 0 FP32 operands on FP32 MFMA, 1 bf16
+                         pieces (3 pieces, 6 products); changes g at the
+                         1e-7 level                                           */
 } lelph_tuning;
 int lelph_set_tuning(lelph_ctx* ctx, const lelph_tuning* tuning);
 int lelph_get_tuning(lelph_ctx* ctx, lelph_tuning* tuning);

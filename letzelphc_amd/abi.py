@@ -62,9 +62,9 @@ class Pseudo(C.Structure):
 class Tuning(C.Structure):
     """lelph_tuning: launch shapes of a context (fft_lines, fp32_form and
     ctile give bit-identical g, split_target changes it at the rounding
-    level)."""
+    level) and the form of the bf16 contexts' non-local GEMMs (nl_form)."""
     _fields_ = [("split_target", C.c_int), ("fft_lines", C.c_int), ("fp32_form", C.c_int),
-                ("ctile", C.c_int)]
+                ("ctile", C.c_int), ("nl_form", C.c_int)]
 
 
 class Pair(C.Structure):

@@ -83,7 +83,7 @@ struct lelph_ctx
     int device = 0;
     hipStream_t stream = nullptr;
     int precision = LELPH_FP64;
-    lelph_tuning tune{0, 0, 0, 0};
+    lelph_tuning tune{0, 0, 0, 0, 0};
 
     // lattice (host copy)
     lelph_lattice lat{};
@@ -860,11 +860,13 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
         const z64* B = side == 0 ? c->wr_K.as<z64>() : c->wr_Kp.as<z64>();
         z64* out = side == 0 ? c->bb_K.as<z64>() : c->bb_Kp.as<z64>();
         const int conjB = side == 0 ? sK.conj_w : sKp.conj_w;
-        // both split precisions take the 3-piece, 6-product form here: the
-        // 2-piece form left 8.5e-6 of |g_NL|max on the non-local part alone
-        // (80-band cell), against 2.5e-7 for 3 pieces, at a cost that is
-        // small next to the local contraction
-        if (prec == LELPH_BF16X3 || prec == LELPH_BF16X6)
+        // bf16 contexts: FP32 operands on FP32 MFMA by default (c2: 0.16 ms
+        // per GEMM against 0.36 + 0.10 ms for the bf16 pieces, and closer:
+        // 1.3-1.9e-7 against 2.1-2.6e-7 of the non-local part alone); the
+        // non-local kernels run in the gaps between contractions, so their
+        // time counts.  nl_form 1: bf16 pieces, 3 of them and 6 products for
+        // both split precisions (the 2-piece form left 8.5e-6)
+        if ((prec == LELPH_BF16X3 || prec == LELPH_BF16X6) && c->tune.nl_form == 1)
             LELPH_TRY((nl_gemm_split<3, 6>(c, A, B, rows, ncol, w.npw, conjB, out, st)));
         else
         {
@@ -1325,6 +1327,7 @@ int lelph_set_tuning(lelph_ctx* c, const lelph_tuning* t)
         return LELPH_ERR_ARG;
     if (t->fp32_form != 0 && t->fp32_form != 1) return LELPH_ERR_ARG;
     if (t->ctile < 0 || t->ctile > 4) return LELPH_ERR_ARG;
+    if (t->nl_form != 0 && t->nl_form != 1) return LELPH_ERR_ARG;
     c->tune = *t;
     return LELPH_OK;
 }

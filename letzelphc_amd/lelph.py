@@ -210,15 +210,15 @@ class Context:
                "lelph_get_precision")
         return {v: k for k, v in PRECISIONS.items()}[eff.value]
 
-    def set_tuning(self, split_target=0, fft_lines=0, fp32_form=0, ctile=0):
-        t = Tuning(int(split_target), int(fft_lines), int(fp32_form), int(ctile))
+    def set_tuning(self, split_target=0, fft_lines=0, fp32_form=0, ctile=0, nl_form=0):
+        t = Tuning(int(split_target), int(fft_lines), int(fp32_form), int(ctile), int(nl_form))
         _check(lib().lelph_set_tuning(self.h, C.byref(t)), "lelph_set_tuning")
 
     def get_tuning(self) -> dict:
         t = Tuning()
         _check(lib().lelph_get_tuning(self.h, C.byref(t)), "lelph_get_tuning")
         return {"split_target": t.split_target, "fft_lines": t.fft_lines,
-                "fp32_form": t.fp32_form, "ctile": t.ctile}
+                "fp32_form": t.fp32_form, "ctile": t.ctile, "nl_form": t.nl_form}
 
     def upload_wfc(self, ik, wfc, gvec, Fk=None):
         wfc = np.ascontiguousarray(wfc, np.complex128)

@@ -64,6 +64,7 @@ int main(void)
     F(lelph_tuning, fft_lines);
     F(lelph_tuning, fp32_form);
     F(lelph_tuning, ctile);
+    F(lelph_tuning, nl_form);
     Z(lelph_longrange);
     F(lelph_longrange, epsilon);
     F(lelph_longrange, Zborn);
