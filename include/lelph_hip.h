@@ -227,7 +227,12 @@ int lelph_gkq(lelph_ctx* ctx, const double qpt[3], int ikq, int ik, int kqsym,
               int ksym, int non_loc, double* elph_kq);
 
 /* A batch of (k,k+q) pairs for the current q; g_out (npairs, nmodes, nspin,
- * nbnds, nbnds).  Device-resident between pairs, one copy back at the end. */
+ * nbnds, nbnds).  Device-resident between pairs, one copy back at the end.
+ * The pairs run along k -> k+q chains: a pair's psi_k box is the psi_k+q box
+ * of the pair before it, and the last pair of a closed cycle takes its
+ * psi_k+q box from the chain head's psi_k box, both with the shift theorem
+ * where an umklapp vector separates them (within rounding of transforming
+ * every side, as lelph_gkq does). */
 int lelph_gkq_batch(lelph_ctx* ctx, const double qpt[3], int64_t npairs,
                     const lelph_pair* pairs, int non_loc, double* g_out);
 
@@ -237,7 +242,8 @@ int lelph_gkq_batch(lelph_ctx* ctx, const double qpt[3], int64_t npairs,
  * cell as one unit.  lelph_multi_create plans it: every pair gets its own
  * psi_k+q box and K-split partials, and its psi_k box is the psi_k+q box of
  * the pair before it on a k -> k+q chain (shifted where that side has an
- * umklapp vector) as in lelph_gkq_batch, the contractions of all pairs of
+ * umklapp vector), the last pair of a closed cycle shifting the head's psi_k
+ * box into its psi_k+q box, as in lelph_gkq_batch, the contractions of all pairs of
  * one precision become the tile list of one persistent work-queue launch
  * (largest tiles first), and the whole sequence (boxes, contractions,
  * fixed-order reductions, non-local parts, copy-out) runs once eagerly and
@@ -578,6 +584,11 @@ int lelph_interp_timings(lelph_ctx* ctx, double* ms, int n);
  *   [4] total device time per pair (start of its local part to the end of
  *       both parts; overlapping pairs of a batch count their overlap twice)
  *   [5] number of (k,k+q) pairs
+ *   [6] number of psi sides transformed (sphere scatter + backward FFT); a
+ *       batch takes the others over along k -> k+q chains with the shift
+ *       theorem: a pair's psi_k box is the psi_k+q box of the pair before
+ *       it, and the last pair of a closed cycle (k + nq = k + G) takes its
+ *       psi_k+q box from the first pair's psi_k box
  * n: number of entries wanted (<= 8). */
 int lelph_last_timings(lelph_ctx* ctx, double* ms, int n);
 

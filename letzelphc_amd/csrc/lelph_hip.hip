@@ -181,6 +181,14 @@ struct lelph_ctx
     bool box_reuse = false, box_prev_valid = false;
     int box_prev_ik = -1, box_prev_sym = -1, box_prev_prec = -1;
     int box_prev_g0[3] = {0, 0, 0};
+    // the psi_k box of the chain's head stays put (box_head points into
+    // box_k, box_kq or the spare box_x) until the next head: the last pair
+    // of a closed k -> k+q cycle, whose k+q side is the head's k side, takes
+    // it over with the shift theorem
+    DBuf box_x;
+    const void* box_head = nullptr;
+    int box_head_ik = -1, box_head_sym = -1, box_head_prec = -1;
+    int64_t box_sides = 0; // psi sides transformed since the call began
 
     // per-pair events: [0] start, [1] boxes built, [2] contraction done,
     // [3] local done, [4] non-local done
@@ -449,11 +457,13 @@ static void chain_order(const lelph_ctx* c, const lelph_pair* pairs, int64_t npa
 // k side is the k+q box of the pair before it in a k -> k+q chain (its
 // umklapp vector k_from_g0): taken over with the shift theorem (as is, when
 // that vector is zero and k_from is c->box_k itself) instead of
-// transformed.  g0_out: this pair's k+q umklapp vector.
+// transformed.  kq_from: the k+q side is the (unshifted) psi_k box of the
+// chain's head, the pair whose k side it is; shifted by this pair's umklapp
+// vector instead of transformed.  g0_out: this pair's k+q umklapp vector.
 template <typename T>
 static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
                        const void* k_from = nullptr, const int* k_from_g0 = nullptr,
-                       int* g0_out = nullptr)
+                       int* g0_out = nullptr, const void* kq_from = nullptr)
 {
     const lelph_lattice& L = c->lat;
     if (pr.ik < 0 || pr.ikq < 0 || pr.ik >= (int)c->wf.size() || pr.ikq >= (int)c->wf.size())
@@ -489,10 +499,35 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
     if (g0_out) std::memcpy(g0_out, g0, sizeof(g0));
     c->box_prev_valid = false;
     const bool linked = k_from != nullptr && !reuse;
+    // the head's box is never handed out as a k+q box while it is kept
+    auto swap_bufs = [](DBuf& a, DBuf& b) {
+        std::swap(a.p, b.p);
+        std::swap(a.n, b.n);
+    };
     if (reuse)
     {
-        std::swap(c->box_k.p, c->box_kq.p);
-        std::swap(c->box_k.n, c->box_kq.n);
+        swap_bufs(c->box_k, c->box_kq);
+        if (c->box_head && c->box_head == c->box_kq.p)
+        {
+            LELPH_TRY(c->box_x.ensure(bb));
+            swap_bufs(c->box_kq, c->box_x);
+        }
+    }
+    else if (c->box_reuse && !linked)
+    {
+        c->box_head = c->box_k.p;
+        c->box_head_ik = pr.ik;
+        c->box_head_sym = pr.ksym;
+        c->box_head_prec = prec_tag;
+    }
+    // the k+q side closes a cycle: it is the head's k side
+    const void* kq_src = kq_from;
+    if (!kq_src && c->box_reuse && c->box_head && c->box_head_ik == pr.ikq &&
+        c->box_head_sym == pr.kqsym && c->box_head_prec == prec_tag)
+        kq_src = c->box_head;
+    const bool build_kq = kq_src == nullptr;
+    if (reuse)
+    {
         const int* pg = c->box_prev_g0;
         if (pg[0] || pg[1] || pg[2])
         {
@@ -501,7 +536,7 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
                                Ny, Nz, pg[0], pg[1], pg[2], -1);
             HIP_TRY(hipGetLastError());
         }
-        if (!pruned) LELPH_TRY(zero_async(c->box_kq.p, bb, c->stream));
+        if (!pruned && build_kq) LELPH_TRY(zero_async(c->box_kq.p, bb, c->stream));
     }
     else if (linked)
     {
@@ -515,14 +550,14 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
         }
         else if (k_from != c->box_k.p)
             HIP_TRY(hipMemcpyAsync(c->box_k.p, k_from, bb, hipMemcpyDeviceToDevice, c->stream));
-        if (!pruned) LELPH_TRY(zero_async(c->box_kq.p, bb, c->stream));
+        if (!pruned && build_kq) LELPH_TRY(zero_async(c->box_kq.p, bb, c->stream));
     }
     else
     {
         if (!pruned)
         {
             LELPH_TRY(zero_async(c->box_k.p, bb, c->stream));
-            LELPH_TRY(zero_async(c->box_kq.p, bb, c->stream));
+            if (build_kq) LELPH_TRY(zero_async(c->box_kq.p, bb, c->stream));
         }
         hipLaunchKernelGGL(k_gindex, dim3((unsigned)cdiv(wk.npw, 256)), dim3(256), 0, c->stream,
                            wk.gvec, wk.npw, gk, Nx, Ny, Nz, c->idx_k.as<int>(),
@@ -531,32 +566,50 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
     hipLaunchKernelGGL(k_gindex, dim3((unsigned)cdiv(wkq.npw, 256)), dim3(256), 0, c->stream,
                        wkq.gvec, wkq.npw, gkq, Nx, Ny, Nz, c->idx_kq.as<int>(),
                        c->ph_kq.as<z64>(), c->d_err);
+    const bool build_k = !reuse && !linked;
+    c->box_sides += (build_k ? 1 : 0) + (build_kq ? 1 : 0);
     bool done = false;
     if constexpr (std::is_same<T, float>::value)
         if (pruned)
         {
-            if (!reuse && !linked)
+            if (build_k)
                 LELPH_TRY(pruned_side(c, wk, gk, c->idx_k.as<int>(), c->ph_k.as<z64>(), 0,
                                       c->box_k.as<c32>()));
-            LELPH_TRY(pruned_side(c, wkq, gkq, c->idx_kq.as<int>(), c->ph_kq.as<z64>(), 1,
-                                  c->box_kq.as<c32>()));
+            if (build_kq)
+                LELPH_TRY(pruned_side(c, wkq, gkq, c->idx_kq.as<int>(), c->ph_kq.as<z64>(), 1,
+                                      c->box_kq.as<c32>()));
             done = true;
         }
     if (!done)
     {
-        if (!reuse && !linked)
+        if (build_k)
             hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)cdiv(nsb * wk.npw, 256)), dim3(256),
                                0, c->stream, wk.wfc, nsb, L.nspinor, wk.npw, c->idx_k.as<int>(),
                                c->ph_k.as<z64>(), gk, c->nf, c->box_k.as<cpx<T>>());
-        hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)cdiv(nsb * wkq.npw, 256)), dim3(256), 0,
-                           c->stream, wkq.wfc, nsb, L.nspinor, wkq.npw, c->idx_kq.as<int>(),
-                           c->ph_kq.as<z64>(), gkq, c->nf, c->box_kq.as<cpx<T>>());
+        if (build_kq)
+            hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)cdiv(nsb * wkq.npw, 256)), dim3(256),
+                               0, c->stream, wkq.wfc, nsb, L.nspinor, wkq.npw,
+                               c->idx_kq.as<int>(), c->ph_kq.as<z64>(), gkq, c->nf,
+                               c->box_kq.as<cpx<T>>());
         HIP_TRY(hipGetLastError());
-        if (!reuse && !linked)
+        if (build_k)
             LELPH_TRY(fft3d_batch<T>(c->ax, c->box_k.as<cpx<T>>(), nbox, +1, c->stream,
                                      c->tune.fft_lines));
-        LELPH_TRY(fft3d_batch<T>(c->ax, c->box_kq.as<cpx<T>>(), nbox, +1, c->stream,
-                                 c->tune.fft_lines));
+        if (build_kq)
+            LELPH_TRY(fft3d_batch<T>(c->ax, c->box_kq.as<cpx<T>>(), nbox, +1, c->stream,
+                                     c->tune.fft_lines));
+    }
+    if (!build_kq)
+    {
+        if (g0[0] || g0[1] || g0[2])
+        {
+            hipLaunchKernelGGL(k_box_shift<T>, dim3((unsigned)cdiv(c->nf, 256)), dim3(256), 0,
+                               c->stream, static_cast<const cpx<T>*>(kq_src),
+                               c->box_kq.as<cpx<T>>(), nbox, Nx, Ny, Nz, g0[0], g0[1], g0[2], +1);
+            HIP_TRY(hipGetLastError());
+        }
+        else if (kq_src != c->box_kq.p)
+            HIP_TRY(hipMemcpyAsync(c->box_kq.p, kq_src, bb, hipMemcpyDeviceToDevice, c->stream));
     }
     if (c->box_reuse)
     {
@@ -1130,7 +1183,7 @@ static void free_ctx(lelph_ctx* c)
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     DBuf* bufs[] = {&c->dv64, &c->dv32, &c->eig_d, &c->vloc_tmp, &c->rot_tmp, &c->idx_k,
-                    &c->idx_kq, &c->ph_k, &c->ph_kq, &c->box_k, &c->box_kq, &c->part, &c->g,
+                    &c->idx_kq, &c->ph_k, &c->ph_kq, &c->box_k, &c->box_kq, &c->box_x, &c->part, &c->g,
                     &c->gbatch, &c->presplit, &c->nlK, &c->nlKp, &c->Y_K, &c->Y_Kp, &c->wr_K, &c->wr_Kp,
                     &c->beta_K, &c->beta_Kp, &c->bb_K, &c->bb_Kp, &c->bbpart, &c->ebuf,
                     &c->pl_cnt[0], &c->pl_cnt[1], &c->pl_off[0], &c->pl_off[1], &c->pl_cur[0],
@@ -1717,6 +1770,7 @@ static int finish_timings(lelph_ctx* c, int64_t npairs)
     }
     for (int k = 0; k < 5; ++k) c->timings[k] = t[k];
     c->timings[5] = (double)npairs;
+    c->timings[6] = (double)c->box_sides;
     return LELPH_OK;
 }
 
@@ -1731,6 +1785,7 @@ static int one_pair(lelph_ctx* c, const double* qpt, int ikq, int ik, int kqsym,
     const int64_t gsz = (int64_t)c->lat.nmodes * c->lat.nspin * c->lat.nbnds * c->lat.nbnds;
     LELPH_TRY(c->g.ensure(sizeof(z64) * gsz));
     LELPH_TRY(ensure_events(c, 1));
+    c->box_sides = 0;
     if (accumulate)
         HIP_TRY(hipMemcpyAsync(c->g.p, host_g, sizeof(z64) * gsz, hipMemcpyHostToDevice,
                                c->stream));
@@ -1795,6 +1850,8 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     };
     c->box_reuse = true;
     c->box_prev_valid = false;
+    c->box_head = nullptr;
+    c->box_sides = 0;
     int st = LELPH_OK;
     int64_t landed = 0;
     for (int64_t n = 0; n < npairs && st == LELPH_OK; ++n)
@@ -1817,6 +1874,7 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     }
     c->box_reuse = false;
     c->box_prev_valid = false;
+    c->box_head = nullptr;
     if (st != LELPH_OK)
     {
         (void)hipStreamSynchronize(c->nl_stream);
@@ -1849,7 +1907,8 @@ struct MultiPair
     int64_t cps = 1, gsz = 0, tiles = 0;
     size_t bb = 0;     // bytes of one psi box
     int shape = 1;     // contraction tile shape (ct_shape of its context)
-    int64_t link = -1; // the pair before it in its k -> k+q chain (index in pairs)
+    int64_t link = -1;  // the pair before it in its k -> k+q chain (index in pairs)
+    int64_t close = -1; // the chain head whose psi_k box is its psi_k+q box
     int g0[3] = {0, 0, 0}; // umklapp vector of its k+q side
     void* kbox = nullptr;  // its psi_k box: box_k, or the linked pair's box_kq
     DBuf box_k, box_kq, part, ebuf;
@@ -1967,8 +2026,9 @@ static int multi_enqueue(lelph_multi* m, bool timed)
         // every pair's psi_k+q box into its own slot, along each cell's
         // k -> k+q chains: a linked pair's psi_k box is the k+q box of the
         // pair before it (the same box, or a shifted copy where that pair
-        // has an umklapp vector), as in lelph_gkq_batch; a chain head
-        // transforms both sides
+        // has an umklapp vector), the last pair of a closed cycle takes its
+        // psi_k+q box from the head's psi_k box, as in lelph_gkq_batch; a
+        // chain head transforms both sides
         for (int64_t bi : m->border)
         {
             MultiPair& p = m->pairs[(size_t)bi];
@@ -1980,10 +2040,11 @@ static int multi_enqueue(lelph_multi* m, bool timed)
             const MultiPair* lp = p.link >= 0 ? &m->pairs[(size_t)p.link] : nullptr;
             const void* kf = lp ? lp->box_kq.p : nullptr;
             const int* kg = lp ? lp->g0 : nullptr;
+            const void* kqf = p.close >= 0 ? m->pairs[(size_t)p.close].kbox : nullptr;
             if (p.prec == LELPH_FP64)
-                LELPH_TRY(build_boxes<double>(c, q, p.pr, kf, kg));
+                LELPH_TRY(build_boxes<double>(c, q, p.pr, kf, kg, nullptr, kqf));
             else
-                LELPH_TRY(build_boxes<float>(c, q, p.pr, kf, kg));
+                LELPH_TRY(build_boxes<float>(c, q, p.pr, kf, kg, nullptr, kqf));
         }
         if (timed) HIP_TRY(hipEventRecord(m->ev[1], m->stream));
         // the work-queue contractions
@@ -2242,6 +2303,7 @@ int lelph_multi_create(int ncells, lelph_ctx* const* ctxs, const double* qpts,
             const int64_t first = (int64_t)m->pairs.size() - npairs[i];
             std::vector<int64_t> order;
             chain_order(c, pairs[i], npairs[i], order);
+            int64_t head = first;
             for (size_t n = 0; n < order.size(); ++n)
             {
                 MultiPair& p = m->pairs[(size_t)(first + order[n])];
@@ -2260,6 +2322,11 @@ int lelph_multi_create(int ncells, lelph_ctx* const* ctxs, const double* qpts,
                     if (p.box_k.ensure(bb)) return fail(LELPH_ERR_ALLOC);
                     p.kbox = p.box_k.p;
                 }
+                // a pair whose k+q side is the k side of its chain's head
+                // closes a cycle: that box, shifted, is its k+q box
+                if (!lp) head = first + order[n];
+                const MultiPair& h = m->pairs[(size_t)head];
+                if (p.pr.ikq == h.pr.ik && p.pr.kqsym == h.pr.ksym) p.close = head;
             }
         }
         m->gdev.emplace_back();

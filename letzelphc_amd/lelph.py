@@ -491,10 +491,11 @@ class Context:
 
     def last_timings(self) -> dict:
         """Device milliseconds of the last call, summed over its pairs."""
-        t = np.zeros(6)
-        _check(lib().lelph_last_timings(self.h, dptr(t), 6), "lelph_last_timings")
+        t = np.zeros(7)
+        _check(lib().lelph_last_timings(self.h, dptr(t), 7), "lelph_last_timings")
         return {"boxes_ms": t[0], "contract_ms": t[1], "reduce_ms": t[2],
-                "nonlocal_ms": t[3], "total_ms": t[4], "pairs": int(t[5])}
+                "nonlocal_ms": t[3], "total_ms": t[4], "pairs": int(t[5]),
+                "box_sides": int(t[6])}
 
 
 class MultiBatch:

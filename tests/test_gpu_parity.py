@@ -2,6 +2,8 @@
 
 Bars: FP64 path |dg|max/|g|max <= 1e-10; FP32 path <= 1e-6.
 """
+import bisect
+
 import numpy as np
 import pytest
 
@@ -310,24 +312,77 @@ def _chain_links(pairs):
                for (ikq, ik, kqsym, ksym) in pairs)
 
 
+def _chain_sides(pairs):
+    """psi sides a batch transforms: the chain order of the library (a pair
+    follows the first pair, in (iBZ k, symmetry) order, whose k side is its
+    k+q side and that has no predecessor yet; open chains first, then what
+    is left of cycles), one transform for a chain head's k side and one for
+    every k+q side except where it is the head's k side (a closed cycle)."""
+    n = len(pairs)
+    keys = sorted(((ik, ks), i) for i, (_, ik, _, ks) in enumerate(pairs))
+    nxt, has_prev = [-1] * n, [False] * n
+    for i, (ikq, _, kqs, _) in enumerate(pairs):
+        j = bisect.bisect_left(keys, ((ikq, kqs), -1))
+        if j < n and keys[j][0] == (ikq, kqs) and keys[j][1] != i and not has_prev[keys[j][1]]:
+            nxt[i] = keys[j][1]
+            has_prev[keys[j][1]] = True
+    order, used = [], [False] * n
+    for first_pass in (True, False):
+        for i in range(n):
+            if used[i] or (first_pass and has_prev[i]):
+                continue
+            j = i
+            while j >= 0 and not used[j]:
+                used[j] = True
+                order.append(j)
+                j = nxt[j]
+    sides, prev, head = 0, None, None
+    for i in order:
+        ikq, ik, kqs, ks = pairs[i]
+        if prev != (ik, ks):
+            sides, head = sides + 1, (ik, ks)
+        if (ikq, kqs) != head:
+            sides += 1
+        prev = (ikq, kqs)
+    return sides
+
+
+def test_chain_sides_counts():
+    """A chain a -> b -> c -> a (a closed cycle) needs three transforms, an
+    open chain a -> b -> c four, a self-loop one, two disjoint pairs four."""
+    cyc = [(1, 0, 0, 0), (2, 1, 0, 0), (0, 2, 0, 0)]
+    assert _chain_sides(cyc) == 3
+    assert _chain_sides(cyc[::-1]) == 3
+    assert _chain_sides([(1, 0, 0, 0), (2, 1, 0, 0), (3, 2, 0, 0)]) == 4
+    assert _chain_sides([(0, 0, 0, 0)]) == 1
+    assert _chain_sides([(1, 0, 0, 0), (3, 2, 0, 0)]) == 4
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", TOL32), ("bf16x3", TOL32)])
 @pytest.mark.parametrize("kminusq", [False, True])
 def test_batch_box_reuse(c0, c0ctx, prec, tol, kminusq):
     """A batch walks k -> k+q chains and takes each pair's k box from the
-    previous pair's k+q box, shifted back by its umklapp vector; pair by pair
-    (every box transformed) gives the same g.  The pairs go in reversed
-    order so the batch has to rebuild the chains itself."""
+    previous pair's k+q box, shifted back by its umklapp vector, and the
+    last pair of a closed cycle its k+q box from the head's k box, shifted
+    by its own; pair by pair (every box transformed) gives the same g.  The
+    pairs go in reversed order so the batch has to rebuild the chains
+    itself; it transforms exactly the sides the chains leave."""
     s, o, dv = c0
     pairs = synth.kq_pairs(s, 0, kminusq=kminusq)[::-1]
     links = _chain_links(pairs)
-    say(f"case: box reuse {prec} kminusq={kminusq} pairs={len(pairs)} links={links}")
+    want_sides = _chain_sides(pairs)
+    say(f"case: box reuse {prec} kminusq={kminusq} pairs={len(pairs)} links={links} "
+        f"sides={want_sides}")
     assert links >= len(pairs) // 2
     c0ctx.set_precision(prec)
     try:
         gb = c0ctx.gkq_batch(pairs, non_loc=True, qpt=s.qpts[0])
+        sides = c0ctx.last_timings()["box_sides"]
         one = np.stack([c0ctx.gkq(*p, non_loc=True, qpt=s.qpts[0]) for p in pairs])
+        assert c0ctx.last_timings()["box_sides"] == 2
     finally:
         c0ctx.set_precision("fp64")
+    assert sides == want_sides
     err = rel_err(gb, one)
     say(f"  batch vs pair by pair: {err:.3e}")
     assert err < tol
