@@ -49,6 +49,37 @@ __host__ __device__ inline cpx<T> cscale(cpx<T> a, T s)
     return {a.x * s, a.y * s};
 }
 
+// e^{sgn 2 pi i G0.r} at grid point p = (ix, iy, iz) (r = (ix/Nx, iy/Ny,
+// iz/Nz) in crystal units): G0.r reduced exactly in integers, the fraction
+// in FP64, rounded once to T
+template <typename T>
+__device__ inline cpx<T> box_phase(int64_t p, int Nx, int Ny, int Nz, int g0, int g1, int g2,
+                                   int sgn)
+{
+    const int ix = (int)(p / ((int64_t)Ny * Nz)), iy = (int)((p / Nz) % Ny), iz = (int)(p % Nz);
+    auto fr = [](int g, int i, int n) {
+        int64_t m = ((int64_t)g * i) % n;
+        if (m < 0) m += n;
+        return (double)m / n;
+    };
+    double f = fr(g0, ix, Nx) + fr(g1, iy, Ny) + fr(g2, iz, Nz);
+    f -= floor(f);
+    double s, c;
+    sincospi(2.0 * f * sgn, &s, &c);
+    return {(T)c, (T)s};
+}
+
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS
+// accesses (lgkmcnt) and synchronises, but unlike __syncthreads it does not
+// wait for outstanding global loads, so register prefetches stay in flight
+// across it.  Global -> LDS copies are waited for explicitly (vmcnt) before
+// it.  The memory clobber keeps the compiler from moving LDS accesses
+// across the barrier.
+__device__ inline void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 #define HIP_TRY(expr)                                                   \
     do                                                                  \
     {                                                                   \

@@ -13,6 +13,7 @@
 #pragma once
 
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "common.hpp"
@@ -314,8 +315,9 @@ constexpr int fft_ns_c(int n, int s) // product of the radices before stage s
 }
 
 // Lines sit LS elements apart in LDS (LS = N + 1 keeps the transposing
-// loads and stores of k_fft_lines_c free of bank conflicts).
-template <typename T, int SIGN, int N, int TL, int S, int LS = N>
+// loads and stores of k_fft_lines_c free of bank conflicts).  LB: the stage
+// barriers order LDS only (global loads in flight stay outstanding).
+template <typename T, int SIGN, int N, int TL, int S, int LS = N, bool LB = false>
 struct FftStagesC
 {
     static __device__ inline void run(cpx<T>*& in, cpx<T>*& out, const cpx<T>* W, int tid)
@@ -356,11 +358,14 @@ struct FftStagesC
 #pragma unroll
                 for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
             }
-            __syncthreads();
+            if constexpr (LB)
+                lds_barrier();
+            else
+                __syncthreads();
             cpx<T>* tmp = in;
             in = out;
             out = tmp;
-            FftStagesC<T, SIGN, N, TL, S + 1, LS>::run(in, out, W, tid);
+            FftStagesC<T, SIGN, N, TL, S + 1, LS, LB>::run(in, out, W, tid);
         }
     }
 };
@@ -436,6 +441,96 @@ __global__ __launch_bounds__(256) void k_fft_lines_c(cpx<T>* __restrict__ data, 
     }
 }
 
+// persistent line kernels for lstride 1 (LELPH_FFT_PERSIST=0 turns them off)
+static inline bool fft_persistent_lines()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("LELPH_FFT_PERSIST");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
+// The same line transform where adjacent lines are adjacent in memory
+// (lstride 1: the x and y passes over boxes), as a persistent kernel: a
+// block walks tiles blockIdx.x, + gridDim.x, ... and keeps the elements of
+// its next tile in flight (registers) while it transforms and stores the
+// current one; its barriers order LDS only, so those loads stay outstanding
+// across them.  Per line the arithmetic is that of k_fft_lines_c.
+template <typename T, int SIGN, int N, int TL>
+__global__ __launch_bounds__(256) void k_fft_lines_cp(cpx<T>* __restrict__ data, int64_t n_inner,
+                                                      int64_t ostride, int64_t estride,
+                                                      int64_t ntile,
+                                                      const cpx<T>* __restrict__ Wg,
+                                                      const int* __restrict__ ocnt, int omod,
+                                                      const int* __restrict__ emask)
+{
+    constexpr int LS = N + 1;
+    constexpr int PER = (TL * N + 255) / 256;
+    __shared__ cpx<T> W[N];
+    __shared__ cpx<T> b0[TL * LS];
+    __shared__ cpx<T> b1[TL * LS];
+    __shared__ int em[N];
+    const int tid = threadIdx.x;
+    const int64_t tiles = (n_inner + TL - 1) / TL;
+    const int64_t step = gridDim.x;
+    for (int t = tid; t < N; t += 256)
+    {
+        W[t] = Wg[t];
+        em[t] = emask ? emask[t] : 1;
+    }
+    lds_barrier();
+    // first tile at or after s whose outer index is not pruned
+    auto next_tile = [&](int64_t s) {
+        if (ocnt)
+            while (s < ntile && ocnt[(s / tiles) % omod] == 0) s += step;
+        return s;
+    };
+    cpx<T> nx[PER];
+    auto fetch = [&](int64_t tile) {
+        const int64_t o = tile / tiles;
+        const int64_t i0 = (tile - o * tiles) * TL;
+        const int nl = (int)((n_inner - i0) < TL ? (n_inner - i0) : TL);
+        const cpx<T>* base = data + o * ostride + i0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+        {
+            const int f = tid + k * 256, j = f / TL, t = f - j * TL;
+            nx[k] = (f < TL * N && t < nl && em[j]) ? base[(int64_t)j * estride + t]
+                                                    : cpx<T>{0, 0};
+        }
+    };
+    int64_t tile = next_tile(blockIdx.x);
+    if (tile < ntile) fetch(tile);
+    while (tile < ntile)
+    {
+        const int64_t nxt = next_tile(tile + step);
+        // the previous tile's stores have read their LDS buffer
+        lds_barrier();
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+        {
+            const int f = tid + k * 256, j = f / TL, t = f - j * TL;
+            if (f < TL * N) b0[t * LS + j] = nx[k];
+        }
+        lds_barrier();
+        if (nxt < ntile) fetch(nxt);
+        cpx<T>* in = b0;
+        cpx<T>* out = b1;
+        FftStagesC<T, SIGN, N, TL, 0, LS, true>::run(in, out, W, tid);
+        const int64_t o = tile / tiles;
+        const int64_t i0 = (tile - o * tiles) * TL;
+        const int nl = (int)((n_inner - i0) < TL ? (n_inner - i0) : TL);
+        cpx<T>* base = data + o * ostride + i0;
+        for (int f = tid; f < TL * N; f += 256)
+        {
+            const int j = f / TL, t = f - j * TL;
+            if (t < nl) base[(int64_t)j * estride + t] = in[t * LS + j];
+        }
+        tile = nxt;
+    }
+}
+
 // lines per block: two buffers of TL*(N+1) within an LDS budget of 56 KB
 // (FP64) or 28 KB (FP32: more blocks per CU; at N = 96, 16 lines of 8-byte
 // elements per block make the pass faster than 32)
@@ -467,6 +562,28 @@ static bool fft_launch_tl(cpx<T>* data, int64_t n_outer, int64_t n_inner, int64_
     {
         const int64_t nblk = cdiv(n_inner, TL) * n_outer;
         if (nblk > 0x7fffffffLL) return false;
+        if (lstride == 1 && fft_persistent_lines())
+        {
+            // persistent: as many blocks as stay resident
+            auto fn = sign > 0 ? (const void*)k_fft_lines_cp<T, 1, N, TL>
+                               : (const void*)k_fft_lines_cp<T, -1, N, TL>;
+            int occ = 0, dev = 0, ncu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, 0) != hipSuccess ||
+                hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) !=
+                    hipSuccess)
+                return false;
+            const int64_t grid = std::min<int64_t>(nblk, (int64_t)std::max(1, occ) * ncu);
+            if (sign > 0)
+                hipLaunchKernelGGL((k_fft_lines_cp<T, 1, N, TL>), dim3((unsigned)grid), dim3(256),
+                                   0, stream, data, n_inner, ostride, estride, nblk, W, ocnt,
+                                   omod, emask);
+            else
+                hipLaunchKernelGGL((k_fft_lines_cp<T, -1, N, TL>), dim3((unsigned)grid),
+                                   dim3(256), 0, stream, data, n_inner, ostride, estride, nblk, W,
+                                   ocnt, omod, emask);
+            return true;
+        }
         if (sign > 0)
             hipLaunchKernelGGL((k_fft_lines_c<T, 1, N, TL>), dim3((unsigned)nblk), dim3(256), 0,
                                stream, data, n_inner, ostride, lstride, estride, W, ocnt, omod,

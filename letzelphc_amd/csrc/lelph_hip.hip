@@ -189,6 +189,12 @@ struct lelph_ctx
     const void* box_head = nullptr;
     int box_head_ik = -1, box_head_sym = -1, box_head_prec = -1;
     int64_t box_sides = 0; // psi sides transformed since the call began
+    // k+q box in k-side form, its umklapp phase left to the presplit (set
+    // around build_boxes by the bf16 split paths); where build_boxes left
+    // the k+q box and the umklapp vector its consumer applies
+    bool box_defer = false;
+    const void* kq_view = nullptr;
+    int kq_g0[3] = {0, 0, 0};
 
     // per-pair events: [0] start, [1] boxes built, [2] contraction done,
     // [3] local done, [4] non-local done
@@ -460,6 +466,12 @@ static void chain_order(const lelph_ctx* c, const lelph_pair* pairs, int64_t npa
 // transformed.  kq_from: the k+q side is the (unshifted) psi_k box of the
 // chain's head, the pair whose k side it is; shifted by this pair's umklapp
 // vector instead of transformed.  g0_out: this pair's k+q umklapp vector.
+// With c->box_defer (FP32 boxes whose k+q side is consumed only through the
+// bf16 presplit) the k+q box is built in k-side form, the box of (iBZ k+q,
+// symmetry) with no umklapp vector, and the consumer applies e^{+2 pi i
+// G0.r} (c->kq_g0) itself: every box is then a plain transform of its key,
+// taken over along chains and at a cycle's close without any shift.
+// c->kq_view: where the k+q box is (box_kq, or a kept head's box).
 template <typename T>
 static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
                        const void* k_from = nullptr, const int* k_from_g0 = nullptr,
@@ -474,6 +486,16 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
     if (!wk.wfc || !wkq.wfc) return LELPH_ERR_STATE;
     SideGeom gkq, gk;
     LELPH_TRY(local_geometry(c, qpt, pr.ikq, pr.ik, pr.kqsym, pr.ksym, &gkq, &gk));
+    const bool defer = c->box_defer && std::is_same<T, float>::value;
+    // k-side geometry of (ikq, kqsym): the same box the next pair of a chain
+    // builds for its k side (the k+q side's translation phase uses
+    // S_kq k_kq + R G and does not depend on the umklapp vector)
+    SideGeom gkq_k;
+    if (defer)
+    {
+        SideGeom unused;
+        LELPH_TRY(local_geometry(c, qpt, pr.ikq, pr.ikq, pr.kqsym, pr.kqsym, &unused, &gkq_k));
+    }
     const int64_t nsb = (int64_t)L.nspin * L.nbnds;
     const int64_t nbox = nsb * L.nspinor;
     const size_t bb = sizeof(cpx<T>) * (size_t)(nbox * c->nf);
@@ -491,7 +513,7 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
                         nbox <= 65535;
     // the k side of this pair is the k+q side of the previous one: swap the
     // boxes and undo that side's umklapp shift instead of transforming again
-    const int prec_tag = std::is_same<T, float>::value ? 1 : 0;
+    const int prec_tag = std::is_same<T, float>::value ? (defer ? 2 : 1) : 0;
     const bool reuse = c->box_reuse && c->box_prev_valid && c->box_prev_ik == pr.ik &&
                        c->box_prev_sym == pr.ksym && c->box_prev_prec == prec_tag;
     int g0[3];
@@ -563,9 +585,16 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
                            wk.gvec, wk.npw, gk, Nx, Ny, Nz, c->idx_k.as<int>(),
                            c->ph_k.as<z64>(), c->d_err);
     }
+    // the k+q side's G + G0 must fit the box (checked also where the side
+    // is not transformed or is built in k-side form)
     hipLaunchKernelGGL(k_gindex, dim3((unsigned)cdiv(wkq.npw, 256)), dim3(256), 0, c->stream,
                        wkq.gvec, wkq.npw, gkq, Nx, Ny, Nz, c->idx_kq.as<int>(),
                        c->ph_kq.as<z64>(), c->d_err);
+    const SideGeom& gkq_b = defer ? gkq_k : gkq;
+    if (defer && build_kq)
+        hipLaunchKernelGGL(k_gindex, dim3((unsigned)cdiv(wkq.npw, 256)), dim3(256), 0, c->stream,
+                           wkq.gvec, wkq.npw, gkq_b, Nx, Ny, Nz, c->idx_kq.as<int>(),
+                           c->ph_kq.as<z64>(), c->d_err);
     const bool build_k = !reuse && !linked;
     c->box_sides += (build_k ? 1 : 0) + (build_kq ? 1 : 0);
     bool done = false;
@@ -576,7 +605,7 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
                 LELPH_TRY(pruned_side(c, wk, gk, c->idx_k.as<int>(), c->ph_k.as<z64>(), 0,
                                       c->box_k.as<c32>()));
             if (build_kq)
-                LELPH_TRY(pruned_side(c, wkq, gkq, c->idx_kq.as<int>(), c->ph_kq.as<z64>(), 1,
+                LELPH_TRY(pruned_side(c, wkq, gkq_b, c->idx_kq.as<int>(), c->ph_kq.as<z64>(), 1,
                                       c->box_kq.as<c32>()));
             done = true;
         }
@@ -589,7 +618,7 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
         if (build_kq)
             hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)cdiv(nsb * wkq.npw, 256)), dim3(256),
                                0, c->stream, wkq.wfc, nsb, L.nspinor, wkq.npw,
-                               c->idx_kq.as<int>(), c->ph_kq.as<z64>(), gkq, c->nf,
+                               c->idx_kq.as<int>(), c->ph_kq.as<z64>(), gkq_b, c->nf,
                                c->box_kq.as<cpx<T>>());
         HIP_TRY(hipGetLastError());
         if (build_k)
@@ -599,7 +628,16 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
             LELPH_TRY(fft3d_batch<T>(c->ax, c->box_kq.as<cpx<T>>(), nbox, +1, c->stream,
                                      c->tune.fft_lines));
     }
-    if (!build_kq)
+    const void* kq_view = c->box_kq.p;
+    if (!build_kq && defer)
+    {
+        // k-side form: the head's box is this pair's k+q box as it is; kept
+        // in the spare, it becomes box_kq so that a chain may go on from it
+        if (kq_src == c->box_x.p)
+            swap_bufs(c->box_kq, c->box_x);
+        kq_view = kq_src;
+    }
+    else if (!build_kq)
     {
         if (g0[0] || g0[1] || g0[2])
         {
@@ -611,13 +649,15 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
         else if (kq_src != c->box_kq.p)
             HIP_TRY(hipMemcpyAsync(c->box_kq.p, kq_src, bb, hipMemcpyDeviceToDevice, c->stream));
     }
-    if (c->box_reuse)
+    c->kq_view = kq_view;
+    for (int i = 0; i < 3; ++i) c->kq_g0[i] = defer ? g0[i] : 0;
+    if (c->box_reuse && kq_view == c->box_kq.p)
     {
         c->box_prev_valid = true;
         c->box_prev_ik = pr.ikq;
         c->box_prev_sym = pr.kqsym;
         c->box_prev_prec = prec_tag;
-        std::memcpy(c->box_prev_g0, g0, sizeof(g0));
+        for (int i = 0; i < 3; ++i) c->box_prev_g0[i] = defer ? 0 : g0[i];
     }
     return LELPH_OK;
 }
@@ -999,7 +1039,11 @@ static int run_local_split_t(lelph_ctx* c, const double* qpt, const lelph_pair& 
     const lelph_lattice& L = c->lat;
     if (!c->dv_set) return LELPH_ERR_STATE;
     if (L.nmag == 2 && (L.nspin != 2 || L.nspinor != 1)) return LELPH_ERR_ARG;
-    LELPH_TRY(build_boxes<float>(c, qpt, pr));
+    // psi_k+q in k-side form; its umklapp phase goes into the presplit
+    c->box_defer = true;
+    const int bst = build_boxes<float>(c, qpt, pr);
+    c->box_defer = false;
+    LELPH_TRY(bst);
     const int nm = L.nmodes, nb = L.nbnds, nspin = L.nspin;
     const int nmag_eff = (L.nspinor == 1 && L.nmag == 4) ? 1 : L.nmag;
     const int64_t nchunks = cdiv(c->nf, SP_KC);
@@ -1010,8 +1054,10 @@ static int run_local_split_t(lelph_ctx* c, const double* qpt, const lelph_pair& 
     hipLaunchKernelGGL((k_presplit_B<NP>),
                        dim3((unsigned)nchunks, (unsigned)(nbp / PS_ROWS),
                             (unsigned)(nspin * L.nspinor)),
-                       dim3(256), 0, c->stream, c->box_kq.as<c32>(), nb, nbp, L.nspinor, c->nf,
-                       nchunks, c->presplit.as<__bf16>());
+                       dim3(256), 0, c->stream, static_cast<const c32*>(c->kq_view), nb, nbp,
+                       L.nspinor, c->nf, nchunks, c->presplit.as<__bf16>(),
+                       (int)L.fft_dims[0], (int)L.fft_dims[1], (int)L.fft_dims[2], c->kq_g0[0],
+                       c->kq_g0[1], c->kq_g0[2]);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], c->stream));
     SplitArgs sa{};

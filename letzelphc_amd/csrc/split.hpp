@@ -84,26 +84,39 @@ static inline int64_t presplit_elems(int nspin, int nspinor, int64_t nchunks, in
 // then thread t takes row m = t % 64 and k group kq = t / 64, splits points
 // 8 kq .. 8 kq + 7 and writes one 16-byte unit per piece plane, consecutive
 // threads on consecutive rows (1 KB contiguous per wave and plane).  Rows
-// m >= nb and points >= N are zero.
+// m >= nb and points >= N are zero.  G0 = (g0x, g0y, g0z) not zero: the box
+// holds psi_k+q without its umklapp vector, and every point is multiplied by
+// e^{+2 pi i G0.r} (the shift theorem, as k_box_shift) before the split.
 #define PS_ROWS 64
 template <int NP>
 __global__ __launch_bounds__(256) void k_presplit_B(const c32* __restrict__ box, int nb, int nbp,
                                                     int nspinor, int64_t N, int64_t nchunks,
-                                                    __bf16* __restrict__ Bp)
+                                                    __bf16* __restrict__ Bp, int Nx, int Ny,
+                                                    int Nz, int g0x, int g0y, int g0z)
 {
     __shared__ c32 tile[PS_ROWS * (SP_KC + 1)];
+    __shared__ c32 eph[SP_KC];
     const int tid = threadIdx.x;
     const int64_t c = blockIdx.x;
     const int m0 = blockIdx.y * PS_ROWS;
     const int ss = blockIdx.z; // spin * nspinor + spinor
     const int is = ss / nspinor, sp = ss % nspinor;
     const int64_t r0 = c * SP_KC;
+    const bool shift = (g0x | g0y | g0z) != 0;
+    if (shift)
+    {
+        if (tid < SP_KC)
+            eph[tid] = r0 + tid < N ? box_phase<float>(r0 + tid, Nx, Ny, Nz, g0x, g0y, g0z, 1)
+                                    : c32{0, 0};
+        __syncthreads();
+    }
     for (int f = tid; f < PS_ROWS * SP_KC; f += 256)
     {
         const int m = f / SP_KC, j = f % SP_KC;
         c32 v = {0, 0};
         if (m0 + m < nb && r0 + j < N)
             v = box[((int64_t)(is * nb + m0 + m) * nspinor + sp) * N + r0 + j];
+        if (shift) v = cmul(v, eph[j]);
         tile[m * (SP_KC + 1) + j] = v;
     }
     __syncthreads();

@@ -139,16 +139,6 @@ __device__ inline void pc_stage_B(const __bf16* __restrict__ Bp, const SplitArgs
     }
 }
 
-// Workgroup barrier for LDS traffic only: waits for this wave's LDS
-// accesses (lgkmcnt) and synchronises, but unlike __syncthreads it does not
-// wait for outstanding global loads, so register prefetches stay in flight
-// across it.  Global -> LDS copies are waited for explicitly (vmcnt) before
-// it.  The memory clobber keeps the compiler from moving LDS accesses
-// across the barrier.
-__device__ inline void lds_barrier()
-{
-    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
 
 // one chunk of this lane's two A row blocks: two psi_k rows and one dV
 // (PC_MG = 1), or one psi_k row and the dV of two modes (PC_MG = 4)

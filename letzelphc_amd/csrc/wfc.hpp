@@ -117,16 +117,6 @@ __global__ __launch_bounds__(256) void k_box_shift(const cpx<T>* src, cpx<T>* ds
     const int64_t nf = (int64_t)Nx * Ny * Nz;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nf) return;
-    const int ix = (int)(p / ((int64_t)Ny * Nz)), iy = (int)((p / Nz) % Ny), iz = (int)(p % Nz);
-    auto fr = [](int g, int i, int n) {
-        int64_t m = ((int64_t)g * i) % n;
-        if (m < 0) m += n;
-        return (double)m / n;
-    };
-    double f = fr(g0, ix, Nx) + fr(g1, iy, Ny) + fr(g2, iz, Nz);
-    f -= floor(f);
-    double s, c;
-    sincospi(2.0 * f * sgn, &s, &c);
-    const cpx<T> e = {(T)c, (T)s};
+    const cpx<T> e = box_phase<T>(p, Nx, Ny, Nz, g0, g1, g2, sgn);
     for (int64_t b = 0; b < nbox; ++b) dst[b * nf + p] = cmul(src[b * nf + p], e);
 }
