@@ -130,6 +130,79 @@ struct FftStagesP
     }
 };
 
+// In-place Stockham stages on TLC lines of length N in one LDS buffer,
+// element j of line t at t * LS + j * ES: each thread loads its
+// butterflies' inputs and computes them in registers, a barrier, then it
+// stores the outputs over the same buffer.  TF: consecutive threads take
+// consecutive lines (for lines adjacent in LDS, ES > 1), else consecutive
+// butterflies of one line.  Same butterflies as FftStagesP.
+template <typename T, int SIGN, int N, int S, int NT, int LS, int ES, int TLC, bool TF>
+struct FftStagesIP
+{
+    static __device__ inline void run(cpx<T>* buf, const cpx<T>* W, int tid)
+    {
+        constexpr int R = fft_radix_c(N, S);
+        if constexpr (R != 0)
+        {
+            constexpr int Ns = fft_ns_c(N, S);
+            constexpr int NBF = N / R;
+            constexpr int TWS = N / (Ns * R);
+            constexpr int NW = TLC * NBF;
+            constexpr int PER = (NW + NT - 1) / NT;
+            cpx<T> v[PER][R];
+#pragma unroll
+            for (int u = 0; u < PER; ++u)
+            {
+                const int w = tid + u * NT;
+                if (NW % NT == 0 || w < NW)
+                {
+                    const int t = TF ? w % TLC : w / NBF, j = TF ? w / TLC : w % NBF;
+                    const int k = j % Ns;
+                    const cpx<T>* src = buf + t * LS + j * ES;
+                    v[u][0] = src[0];
+#pragma unroll
+                    for (int r = 1; r < R; ++r)
+                        v[u][r] = cmul(src[r * NBF * ES], tw<T, SIGN>(W, r * k * TWS));
+                    if constexpr (R <= 5)
+                        dft_small<T, SIGN>(v[u], R, W, N);
+                    else
+                    {
+                        cpx<T> o[R];
+#pragma unroll
+                        for (int s = 0; s < R; ++s)
+                        {
+                            cpx<T> acc = {0, 0};
+#pragma unroll
+                            for (int r = 0; r < R; ++r)
+                                acc = cadd(acc,
+                                           cmul(v[u][r], tw<T, SIGN>(W, ((r * s) % R) * (N / R))));
+                            o[s] = acc;
+                        }
+#pragma unroll
+                        for (int s = 0; s < R; ++s) v[u][s] = o[s];
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < PER; ++u)
+            {
+                const int w = tid + u * NT;
+                if (NW % NT == 0 || w < NW)
+                {
+                    const int t = TF ? w % TLC : w / NBF, j = TF ? w / TLC : w % NBF;
+                    const int k = j % Ns;
+                    cpx<T>* dst = buf + t * LS + ((j / Ns) * Ns * R + k) * ES;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) dst[r * Ns * ES] = v[u][r];
+                }
+            }
+            __syncthreads();
+            FftStagesIP<T, SIGN, N, S + 1, NT, LS, ES, TLC, TF>::run(buf, W, tid);
+        }
+    }
+};
+
 // LDS bytes of k_plane_z for a (ny, nz) plane (z twiddles; with the y pass
 // fused, the y twiddles and the [z][y] layout's padded line stride ny + 1)
 static inline size_t plane_lds_bytes(int ny, int nz, bool fuse_y = false)
@@ -212,6 +285,73 @@ __global__ __launch_bounds__(PZ_THREADS) void k_plane_z(
         for (int t = tid; t < nyz; t += PZ_THREADS) dst[t] = in[t];
 }
 
+// LDS bytes of k_plane_zy: one [y][z] plane and the twiddles of both axes
+static inline size_t plane_zy_lds_bytes(int n)
+{
+    return sizeof(c32) * ((size_t)n * n + 2 * (size_t)n);
+}
+
+// k_plane_z with the fused y pass on square y-z planes (N x N) in one LDS
+// buffer, [y][z]: gather, z stages in place (lines contiguous), y stages in
+// place on the same layout (lines N apart in y, consecutive threads on
+// consecutive z), the plane written as it lies.  About 74 KB of LDS per
+// block instead of 147 KB and no transposes, so two blocks share a CU and
+// one block's gather overlaps the other's transforms.  Per line the
+// arithmetic is that of k_plane_z<N, N>.
+template <int N>
+__global__ __launch_bounds__(PZ_THREADS, 8) void k_plane_zy(
+    const z64* __restrict__ wfc, int nspinor, int64_t npw, const int* __restrict__ idx,
+    const z64* __restrict__ ph, SideGeom geo, const int* __restrict__ perm,
+    const int* __restrict__ off, const int* __restrict__ cnt, int64_t nf,
+    const c32* __restrict__ Wg, const c32* __restrict__ Wgy, c32* __restrict__ box)
+{
+    extern __shared__ __attribute__((aligned(16))) char pz_smem[];
+    const int x = blockIdx.x;
+    const int n = cnt[x];
+    if (n == 0) return;
+    const int b = blockIdx.y;
+    const int64_t s = b / nspinor;
+    const int sig = b - (int)s * nspinor;
+    constexpr int NYZ = N * N;
+    c32* buf = reinterpret_cast<c32*>(pz_smem);
+    c32* W = buf + NYZ;
+    c32* Wy = W + N;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < N; t += PZ_THREADS)
+    {
+        W[t] = Wg[t];
+        Wy[t] = Wgy[t];
+    }
+    for (int t = tid; t < NYZ; t += PZ_THREADS) buf[t] = c32{0.f, 0.f};
+    __syncthreads();
+    const z64* w = wfc + s * nspinor * npw;
+    const int o = off[x];
+    for (int k = tid; k < n; k += PZ_THREADS)
+    {
+        const int g = perm[o + k];
+        const z64 p = ph[g];
+        z64 c;
+        if (nspinor == 1)
+            c = cmul(w[g], p);
+        else
+        {
+            const z64 u = w[g], d = w[npw + g];
+            c = sig == 0 ? cadd(cmul(u, geo.su2[0]), cmul(d, geo.su2[1]))
+                         : cadd(cmul(u, geo.su2[2]), cmul(d, geo.su2[3]));
+            c = cmul(c, p);
+        }
+        if (geo.conj) c.y = -c.y;
+        buf[idx[g] - x * NYZ] = c32{(float)c.x, (float)c.y};
+    }
+    __syncthreads();
+    // z: line y at y * N, elements contiguous
+    FftStagesIP<float, 1, N, 0, PZ_THREADS, N, 1, N, false>::run(buf, W, tid);
+    // y: line z at z, elements N apart
+    FftStagesIP<float, 1, N, 0, PZ_THREADS, 1, N, N, true>::run(buf, Wy, tid);
+    c32* dst = box + (int64_t)b * nf + (int64_t)x * NYZ;
+    for (int t = tid; t < NYZ; t += PZ_THREADS) dst[t] = buf[t];
+}
+
 // launch k_plane_z for a compile-time NZ; false if NZ is not compiled.
 // Wy not NULL asks for the fused y pass (cubic y-z planes, ny == nz); false
 // if that form is not compiled for this size or does not fit in LDS.
@@ -223,6 +363,17 @@ static bool plane_z_launch(int nz, int nx, int nbox, size_t lds, const z64* wfc,
     if (Wy && (ny != nz || lds > 160 * 1024)) return false;
 #define PZ_CASE(NZV)                                                                              \
     case NZV:                                                                                     \
+        if (Wy && plane_zy_lds_bytes(NZV) <= 80 * 1024)                                           \
+        {                                                                                         \
+            const int lz = (int)plane_zy_lds_bytes(NZV);                                          \
+            if (hipFuncSetAttribute((const void*)k_plane_zy<NZV>,                                 \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lz) != hipSuccess) \
+                return false;                                                                     \
+            hipLaunchKernelGGL((k_plane_zy<NZV>), dim3((unsigned)nx, (unsigned)nbox),             \
+                               dim3(PZ_THREADS), lz, st, wfc, nspinor, npw, idx, ph, geo, perm,   \
+                               off, cnt, nf, W, Wy, box);                                         \
+            return true;                                                                          \
+        }                                                                                         \
         if (Wy)                                                                                   \
         {                                                                                         \
             if (hipFuncSetAttribute((const void*)k_plane_z<NZV, NZV>,                             \
