@@ -172,6 +172,11 @@ struct lelph_ctx
     hipEvent_t ip_ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double ip_ms[5] = {0, 0, 0, 0, 0};
     DBuf nlK, nlKp, Y_K, Y_Kp, wr_K, wr_Kp, beta_K, beta_Kp, bb_K, bb_Kp, bbpart, ebuf;
+    // non-local part along a batch's chains: <beta_K|psi> of the previous
+    // pair (key: its iBZ k+q and symmetry) serves as this pair's K' side
+    DBuf bb_Kprev;
+    bool nl_prev_valid = false;
+    int nl_prev_ik = -1, nl_prev_sym = -1;
     // reduced-precision projector operands: FP32 copies, or bf16 piece planes
     DBuf beta32_K, beta32_Kp, wr32_K, wr32_Kp, nlsplitA, nlsplitB;
 
@@ -913,6 +918,11 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
     LELPH_TRY(c->bb_Kp.ensure(sizeof(z64) * (size_t)rows * ncol));
     const int64_t es = (int64_t)L.nspin * L.nbnds * L.nbnds;
     LELPH_TRY(c->ebuf.ensure(sizeof(z64) * (size_t)(3 * natom * es)));
+    // in a batch, the K' side of a pair whose k side is the previous pair's
+    // k+q side comes from that pair's K-side projections (k_nl_mirror)
+    const bool mirror = c->box_reuse && c->nl_prev_valid && c->nl_prev_ik == pr.ik &&
+                        c->nl_prev_sym == pr.ksym;
+    c->nl_prev_valid = false;
 
     const lelph_symmetry& SK = c->syms[pr.kqsym];
     const lelph_symmetry& SKp = c->syms[pr.ksym];
@@ -928,25 +938,52 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
     LELPH_TRY(su2mat(SK.Rmat, L.nspinor, SK.time_rev, sK.su2));
     LELPH_TRY(su2mat(SKp.Rmat, L.nspinor, SKp.time_rev, sKp.su2));
 
-    hipLaunchKernelGGL(k_nl_geom, dim3((unsigned)cdiv(wK.npw, 128)), dim3(128), 0, st,
-                       wK.gvec, wK.npw, sK, c->lmax, wK.wfc, nsb, L.nspinor, c->nlK.as<double>(),
-                       c->Y_K.as<double>(), c->wr_K.as<z64>());
-    hipLaunchKernelGGL(k_nl_geom, dim3((unsigned)cdiv(wKp.npw, 128)), dim3(128), 0, st,
-                       wKp.gvec, wKp.npw, sKp, c->lmax, wKp.wfc, nsb, L.nspinor,
-                       c->nlKp.as<double>(), c->Y_Kp.as<double>(), c->wr_Kp.as<z64>());
-    NlProj pj{natom, c->pmax, c->ntype};
-    hipLaunchKernelGGL(k_nl_beta<double>, dim3((unsigned)cdiv(wK.npw, 256), natom * c->pmax),
-                       dim3(256), 0, st, c->nlK.as<double>(), c->Y_K.as<double>(), wK.Fk,
-                       wK.npw, pj, c->d_plist, c->d_atype, c->d_apos, c->d_Fsign, -1,
-                       c->beta_K.as<z64>());
-    hipLaunchKernelGGL(k_nl_beta<double>, dim3((unsigned)cdiv(wKp.npw, 256), natom * c->pmax),
-                       dim3(256), 0, st, c->nlKp.as<double>(), c->Y_Kp.as<double>(),
-                       wKp.Fk, wKp.npw, pj, c->d_plist, c->d_atype, c->d_apos, c->d_Fsign, +1,
-                       c->beta_Kp.as<z64>());
-    HIP_TRY(hipGetLastError());
-    // projector GEMMs
+    // projector GEMMs in FP32 (FP32 contexts; bf16 ones unless nl_form 1):
+    // the rotated psi is written in FP32 and beta built in FP32 only
     const int prec = effective_precision(c);
-    for (int side = 0; side < 2 && prec != LELPH_FP64; ++side)
+    const bool nl32 = prec != LELPH_FP64 &&
+                      !((prec == LELPH_BF16X3 || prec == LELPH_BF16X6) && c->tune.nl_form == 1);
+    // K, Y_lm and the rotated psi of both sides; the rotation in groups of
+    // NL_SPY sets per block row
+    constexpr int NL_SPY = 16;
+    const unsigned gy = (unsigned)std::max<int64_t>(1, cdiv(nsb, NL_SPY));
+    const int nsides = mirror ? 1 : 2;
+    for (int side = 0; side < nsides; ++side)
+    {
+        const DevWfc& w = side == 0 ? wK : wKp;
+        const NlSide& sd = side == 0 ? sK : sKp;
+        double* kc = side == 0 ? c->nlK.as<double>() : c->nlKp.as<double>();
+        double* y = side == 0 ? c->Y_K.as<double>() : c->Y_Kp.as<double>();
+        if (nl32)
+        {
+            DBuf& w32 = side == 0 ? c->wr32_K : c->wr32_Kp;
+            LELPH_TRY(w32.ensure(sizeof(c32) * (size_t)ncol * npm));
+            hipLaunchKernelGGL(k_nl_geom<float>, dim3((unsigned)cdiv(w.npw, 128), gy), dim3(128),
+                               0, st, w.gvec, w.npw, sd, c->lmax, w.wfc, nsb, L.nspinor, kc, y,
+                               w32.as<c32>(), NL_SPY);
+        }
+        else
+            hipLaunchKernelGGL(k_nl_geom<double>, dim3((unsigned)cdiv(w.npw, 128), gy),
+                               dim3(128), 0, st, w.gvec, w.npw, sd, c->lmax, w.wfc, nsb,
+                               L.nspinor, kc, y,
+                               side == 0 ? c->wr_K.as<z64>() : c->wr_Kp.as<z64>(), NL_SPY);
+    }
+    NlProj pj{natom, c->pmax, c->ntype};
+    if (!nl32)
+    {
+        hipLaunchKernelGGL(k_nl_beta<double>, dim3((unsigned)cdiv(wK.npw, 256), natom * c->pmax),
+                           dim3(256), 0, st, c->nlK.as<double>(), c->Y_K.as<double>(), wK.Fk,
+                           wK.npw, pj, c->d_plist, c->d_atype, c->d_apos, c->d_Fsign, -1,
+                           c->beta_K.as<z64>());
+        if (!mirror)
+            hipLaunchKernelGGL(k_nl_beta<double>,
+                               dim3((unsigned)cdiv(wKp.npw, 256), natom * c->pmax), dim3(256), 0,
+                               st, c->nlKp.as<double>(), c->Y_Kp.as<double>(), wKp.Fk, wKp.npw,
+                               pj, c->d_plist, c->d_atype, c->d_apos, c->d_Fsign, +1,
+                               c->beta_Kp.as<z64>());
+    }
+    HIP_TRY(hipGetLastError());
+    for (int side = 0; side < nsides && prec != LELPH_FP64; ++side)
     {
         const DevWfc& w = side == 0 ? wK : wKp;
         const z64* A = side == 0 ? c->beta_K.as<z64>() : c->beta_Kp.as<z64>();
@@ -959,23 +996,21 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
         // non-local kernels run in the gaps between contractions, so their
         // time counts.  nl_form 1: bf16 pieces, 3 of them and 6 products for
         // both split precisions (the 2-piece form left 8.5e-6)
-        if ((prec == LELPH_BF16X3 || prec == LELPH_BF16X6) && c->tune.nl_form == 1)
+        if (!nl32)
             LELPH_TRY((nl_gemm_split<3, 6>(c, A, B, rows, ncol, w.npw, conjB, out, st)));
         else
         {
-            // FP32: beta again from the FP64 tables, stored in FP32; psi in FP32
+            // FP32: beta from the FP64 tables, stored in FP32; psi rotated
+            // into FP32 by k_nl_geom
             DBuf& b32 = side == 0 ? c->beta32_K : c->beta32_Kp;
             DBuf& w32 = side == 0 ? c->wr32_K : c->wr32_Kp;
             LELPH_TRY(b32.ensure(sizeof(c32) * (size_t)rows * npm));
-            LELPH_TRY(w32.ensure(sizeof(c32) * (size_t)ncol * npm));
             hipLaunchKernelGGL(k_nl_beta<float>,
                                dim3((unsigned)cdiv(w.npw, 256), natom * c->pmax), dim3(256), 0, st,
                                side == 0 ? c->nlK.as<double>() : c->nlKp.as<double>(),
                                side == 0 ? c->Y_K.as<double>() : c->Y_Kp.as<double>(), w.Fk,
                                w.npw, pj, c->d_plist, c->d_atype, c->d_apos, c->d_Fsign,
                                side == 0 ? -1 : +1, b32.as<c32>());
-            hipLaunchKernelGGL(k_z64_to_c32, dim3((unsigned)cdiv(ncol * w.npw, 256)), dim3(256), 0,
-                               st, B, w32.as<c32>(), ncol * w.npw);
             HIP_TRY(hipGetLastError());
             const int64_t nchunks = cdiv(w.npw, CT_KC);
             const int64_t tiles = cdiv(rows, GZ_MB) * cdiv(ncol, GZ_NB);
@@ -994,7 +1029,7 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
             HIP_TRY(hipGetLastError());
         }
     }
-    for (int side = 0; side < 2 && prec == LELPH_FP64; ++side)
+    for (int side = 0; side < nsides && prec == LELPH_FP64; ++side)
     {
         const DevWfc& w = side == 0 ? wK : wKp;
         const z64* A = side == 0 ? c->beta_K.as<z64>() : c->beta_Kp.as<z64>();
@@ -1019,6 +1054,13 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
                            (int64_t)rows * ncol, 1.0, out);
         HIP_TRY(hipGetLastError());
     }
+    if (mirror)
+    {
+        hipLaunchKernelGGL(k_nl_mirror, dim3((unsigned)cdiv((int64_t)rows * ncol, 256)),
+                           dim3(256), 0, st, c->bb_Kprev.as<z64>(), (int64_t)rows, ncol,
+                           c->bb_Kp.as<z64>());
+        HIP_TRY(hipGetLastError());
+    }
     const int64_t ne = (int64_t)natom * 3 * es;
     z64* eb = ebuf_out ? ebuf_out : c->ebuf.as<z64>();
     hipLaunchKernelGGL(k_nl_kkp, dim3((unsigned)cdiv(ne, 256)), dim3(256), 0, st,
@@ -1026,6 +1068,17 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
                        c->d_socfac, c->d_fco, c->has_fco ? c->d_fco_off : nullptr, L.nspin,
                        L.nbnds, L.nspinor, eb);
     HIP_TRY(hipGetLastError());
+    if (c->box_reuse)
+    {
+        // this pair's K-side projections stay for the next pair of the chain
+        // (the kernels above are ordered before any later write on st)
+        LELPH_TRY(c->bb_Kprev.ensure(c->bb_K.n));
+        std::swap(c->bb_K.p, c->bb_Kprev.p);
+        std::swap(c->bb_K.n, c->bb_Kprev.n);
+        c->nl_prev_valid = true;
+        c->nl_prev_ik = pr.ikq;
+        c->nl_prev_sym = pr.kqsym;
+    }
     if (ebuf_out) return LELPH_OK;
     if (mode_after) HIP_TRY(hipStreamWaitEvent(st, mode_after, 0));
     return nl_mode(c, eb, g, st);
@@ -1231,7 +1284,7 @@ static void free_ctx(lelph_ctx* c)
     DBuf* bufs[] = {&c->dv64, &c->dv32, &c->eig_d, &c->vloc_tmp, &c->rot_tmp, &c->idx_k,
                     &c->idx_kq, &c->ph_k, &c->ph_kq, &c->box_k, &c->box_kq, &c->box_x, &c->part, &c->g,
                     &c->gbatch, &c->presplit, &c->nlK, &c->nlKp, &c->Y_K, &c->Y_Kp, &c->wr_K, &c->wr_Kp,
-                    &c->beta_K, &c->beta_Kp, &c->bb_K, &c->bb_Kp, &c->bbpart, &c->ebuf,
+                    &c->beta_K, &c->beta_Kp, &c->bb_K, &c->bb_Kp, &c->bb_Kprev, &c->bbpart, &c->ebuf,
                     &c->pl_cnt[0], &c->pl_cnt[1], &c->pl_off[0], &c->pl_off[1], &c->pl_cur[0],
                     &c->pl_cur[1], &c->pl_perm[0], &c->pl_perm[1], &c->dm_map, &c->dm_ph,
                     &c->dm_pos, &c->dm_wRS, &c->dm_wS2, &c->dm_part, &c->dm_out, &c->qr_dl,
@@ -1898,6 +1951,7 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     c->box_prev_valid = false;
     c->box_head = nullptr;
     c->box_sides = 0;
+    c->nl_prev_valid = false;
     int st = LELPH_OK;
     int64_t landed = 0;
     for (int64_t n = 0; n < npairs && st == LELPH_OK; ++n)
@@ -1921,6 +1975,7 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     c->box_reuse = false;
     c->box_prev_valid = false;
     c->box_head = nullptr;
+    c->nl_prev_valid = false;
     if (st != LELPH_OK)
     {
         (void)hipStreamSynchronize(c->nl_stream);

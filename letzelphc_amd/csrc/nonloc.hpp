@@ -78,10 +78,14 @@ struct NlSide
 };
 
 // K, Y_lm(K) (nlm rows), rotated coefficients wr[(s,b)][sigma][G]
+// blockIdx.y: sets spy * y .. spy * y + spy - 1 of the rotation (K and Y_lm
+// written by y = 0).  TW: the rotated coefficients rounded once to the
+// precision of the projector GEMM that reads them.
+template <typename TW>
 __global__ void k_nl_geom(const double* __restrict__ gvec, int64_t npw, NlSide sd, int lmax,
                           const z64* __restrict__ wfc, int64_t nsets, int nspinor,
                           double* __restrict__ Kc, double* __restrict__ Y,
-                          z64* __restrict__ wr)
+                          cpx<TW>* __restrict__ wr, int spy)
 {
     int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= npw) return;
@@ -89,27 +93,54 @@ __global__ void k_nl_geom(const double* __restrict__ gvec, int64_t npw, NlSide s
                          sd.k[2] + gvec[3 * g + 2]};
     double K[3];
     for (int i = 0; i < 3; ++i) K[i] = sd.S[3 * i] * t[0] + sd.S[3 * i + 1] * t[1] + sd.S[3 * i + 2] * t[2];
-    Kc[3 * g] = K[0];
-    Kc[3 * g + 1] = K[1];
-    Kc[3 * g + 2] = K[2];
-    for (int l = 0; l <= lmax; ++l)
-        for (int im = 0; im <= 2 * l; ++im) Y[(int64_t)(l * l + im) * npw + g] = d_Ylm(l, im - l, K);
+    if (blockIdx.y == 0)
+    {
+        Kc[3 * g] = K[0];
+        Kc[3 * g + 1] = K[1];
+        Kc[3 * g + 2] = K[2];
+        for (int l = 0; l <= lmax; ++l)
+            for (int im = 0; im <= 2 * l; ++im)
+                Y[(int64_t)(l * l + im) * npw + g] = d_Ylm(l, im - l, K);
+    }
     double s, c;
     sincos(-2.0 * LELPH_RPI * (K[0] * sd.tau[0] + K[1] * sd.tau[1] + K[2] * sd.tau[2]), &s, &c);
     const z64 ph = {c, s};
-    for (int64_t st = 0; st < nsets; ++st)
+    const int64_t s0 = (int64_t)blockIdx.y * spy;
+    const int64_t s1 = s0 + spy < nsets ? s0 + spy : nsets;
+    for (int64_t st = s0; st < s1; ++st)
     {
         const z64* w = wfc + st * nspinor * npw;
-        z64* o = wr + st * nspinor * npw;
+        cpx<TW>* o = wr + st * nspinor * npw;
         if (nspinor == 1)
-            o[g] = cmul(w[g], ph);
+        {
+            const z64 v = cmul(w[g], ph);
+            o[g] = {(TW)v.x, (TW)v.y};
+        }
         else
         {
             z64 u = w[g], d = w[npw + g];
-            o[g] = cmul(cadd(cmul(u, sd.su2[0]), cmul(d, sd.su2[1])), ph);
-            o[npw + g] = cmul(cadd(cmul(u, sd.su2[2]), cmul(d, sd.su2[3])), ph);
+            const z64 a = cmul(cadd(cmul(u, sd.su2[0]), cmul(d, sd.su2[1])), ph);
+            const z64 b = cmul(cadd(cmul(u, sd.su2[2]), cmul(d, sd.su2[3])), ph);
+            o[g] = {(TW)a.x, (TW)a.y};
+            o[npw + g] = {(TW)b.x, (TW)b.y};
         }
     }
+}
+
+// <beta_K'|psi> of a (k point, symmetry) key from its <beta_K|psi>: the
+// K' projectors are the complex conjugates of the K ones (phase
+// e^{+2 pi i K.tau} for e^{-2 pi i K.tau}), the three K-weighted rows with
+// the opposite sign, and psi enters the other side's product conjugated
+// the other way ('T' for 'C'), so bbKp = diag(1, -1, -1, -1) conj(bbK) row
+// by row of four; exact (sign flips and conjugation only).
+__global__ void k_nl_mirror(const z64* __restrict__ src, int64_t rows, int64_t ncol,
+                            z64* __restrict__ dst)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * ncol) return;
+    const int64_t r = t / ncol;
+    const z64 v = src[t];
+    dst[t] = (r % 4 == 0) ? z64{v.x, -v.y} : z64{-v.x, v.y};
 }
 
 // projector list of an atom: entry p -> (lidx, l, m index)
