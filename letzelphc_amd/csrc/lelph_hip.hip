@@ -175,6 +175,7 @@ struct lelph_ctx
     // non-local part along a batch's chains: <beta_K|psi> of the previous
     // pair (key: its iBZ k+q and symmetry) serves as this pair's K' side
     DBuf bb_Kprev;
+    bool nl_chain = false; // set while a batch issues its pairs along chains
     bool nl_prev_valid = false;
     int nl_prev_ik = -1, nl_prev_sym = -1;
     // reduced-precision projector operands: FP32 copies, or bf16 piece planes
@@ -920,7 +921,7 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
     LELPH_TRY(c->ebuf.ensure(sizeof(z64) * (size_t)(3 * natom * es)));
     // in a batch, the K' side of a pair whose k side is the previous pair's
     // k+q side comes from that pair's K-side projections (k_nl_mirror)
-    const bool mirror = c->box_reuse && c->nl_prev_valid && c->nl_prev_ik == pr.ik &&
+    const bool mirror = c->nl_chain && c->nl_prev_valid && c->nl_prev_ik == pr.ik &&
                         c->nl_prev_sym == pr.ksym;
     c->nl_prev_valid = false;
 
@@ -1068,7 +1069,7 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
                        c->d_socfac, c->d_fco, c->has_fco ? c->d_fco_off : nullptr, L.nspin,
                        L.nbnds, L.nspinor, eb);
     HIP_TRY(hipGetLastError());
-    if (c->box_reuse)
+    if (c->nl_chain)
     {
         // this pair's K-side projections stay for the next pair of the chain
         // (the kernels above are ordered before any later write on st)
@@ -1951,6 +1952,7 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     c->box_prev_valid = false;
     c->box_head = nullptr;
     c->box_sides = 0;
+    c->nl_chain = true;
     c->nl_prev_valid = false;
     int st = LELPH_OK;
     int64_t landed = 0;
@@ -1975,6 +1977,7 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     c->box_reuse = false;
     c->box_prev_valid = false;
     c->box_head = nullptr;
+    c->nl_chain = false;
     c->nl_prev_valid = false;
     if (st != LELPH_OK)
     {
@@ -2120,9 +2123,27 @@ static int multi_enqueue(lelph_multi* m, bool timed)
         {
             HIP_TRY(hipEventRecord(m->fork, m->stream));
             HIP_TRY(hipStreamWaitEvent(m->nl, m->fork, 0));
-            for (auto& p : m->pairs)
-                LELPH_TRY(run_nonlocal(m->ctx[p.cell], p.pr, nullptr, m->nl, nullptr,
-                                       p.ebuf.as<z64>()));
+            // along each cell's chains, as lelph_gkq_batch: a pair's K'
+            // projections are the previous pair's K ones, mirrored
+            for (lelph_ctx* c : m->ctx)
+            {
+                c->nl_chain = true;
+                c->nl_prev_valid = false;
+            }
+            int nst = LELPH_OK;
+            for (int64_t bi : m->border)
+            {
+                MultiPair& p = m->pairs[(size_t)bi];
+                nst = run_nonlocal(m->ctx[p.cell], p.pr, nullptr, m->nl, nullptr,
+                                   p.ebuf.as<z64>());
+                if (nst != LELPH_OK) break;
+            }
+            for (lelph_ctx* c : m->ctx)
+            {
+                c->nl_chain = false;
+                c->nl_prev_valid = false;
+            }
+            LELPH_TRY(nst);
         }
         // every pair's psi_k+q box into its own slot, along each cell's
         // k -> k+q chains: a linked pair's psi_k box is the k+q box of the
