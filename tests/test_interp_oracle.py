@@ -362,6 +362,46 @@ def test_dyn_longrange_nonanalytic_limit():
     assert rel_err(Ds[0] - Ds[1], NA[0] - NA[1]) < 2e-4  # O(|q|) from the analytic rest
 
 
+def test_dyn_longrange_nonanalytic_limit_2d():
+    """2-D cells: near q = 0 the in-plane G = 0 term is the published 2-D
+    form (2 pi e^2 / A) (Z^T q)(Z^T q)^T / (|q| (1 + r_eff |q|)) (Sohier et
+    al., PRB 94, 085415 (2016)), linear in |q| and direction dependent; A is
+    the in-plane area |a1 x a2|.  The even part in q removes the odd
+    analytic terms, the difference of two in-plane directions the q = 0
+    limit and the direction-independent out-of-plane |q| term, and a
+    Richardson step the O(|q|^2) rest, leaving (2 pi e^2 / A) [(Z^T d1)
+    (Z^T d1)^T - (Z^T d2)(Z^T d2)^T] per unit |q|.  The 3-D kernel (an O(1)
+    direction dependence), a transposed Z, or a prefactor missing the slab
+    height's c / 2 fail it; the out-of-plane term and r_eff are not pinned."""
+    s = _small_system("2")
+    lr = _lr_params(s, quad=False)
+    od = _dyn_lr(s, lr)
+    nm = s.nmodes
+    z = np.zeros((nm, nm), np.complex128)
+    area = np.linalg.norm(np.cross(s.alat[0], s.alat[1]))
+    dirs = [np.array([1.0, 0, 0]), np.array([0.6, -0.8, 0.0])]
+
+    def even(h, d):
+        dcrys = np.linalg.solve(s.blat, h * d)
+        return 0.5 * (od.lr_term(dcrys, z, 1, masses=False, asr=False) +
+                      od.lr_term(-dcrys, z, 1, masses=False, asr=False))
+
+    h = 2e-5  # |q| above the reference's q + G = 0 threshold (1e-5, cartesian)
+    slope = {}
+    for hh in (h, 2 * h):
+        slope[hh] = (even(hh, dirs[0]) - even(hh, dirs[1])) / hh
+    got = 2 * slope[h] - slope[2 * h]
+    want = np.zeros((nm, nm))
+    for d, sg in zip(dirs, (1, -1)):
+        zq = np.einsum("aij,i->aj", lr.Zborn, d).reshape(-1)
+        want += sg * 2 * synth.RPI * 2.0 / area * np.outer(zq, zq)
+    say(f"case: 2-D non-analytic slope, |got - want|/|want| = {rel_err(got, want):.2e}")
+    assert rel_err(got.real, want) < 1e-5
+    assert np.abs(got.imag).max() < 1e-5 * np.abs(want).max()
+    # without the Richardson step the O(|q|) rest shows (a check on h)
+    assert rel_err(slope[h].real, want) > 10 * rel_err(got.real, want)
+
+
 @pytest.mark.parametrize("asr", ["none", "simple"])
 def test_dyn_interpolation_coarse_and_asr(asr):
     """coarse -> R -> q returns each coarse dynamical matrix (asr none, to
