@@ -178,6 +178,11 @@ struct lelph_ctx
     bool nl_chain = false; // set while a batch issues its pairs along chains
     bool nl_prev_valid = false;
     int nl_prev_ik = -1, nl_prev_sym = -1;
+    // <beta_K'|psi> of the chain's head (key: its iBZ k and symmetry): the
+    // K side of a closed cycle's last pair
+    DBuf bb_Khead;
+    bool nl_head_valid = false;
+    int nl_head_ik = -1, nl_head_sym = -1;
     // reduced-precision projector operands: FP32 copies, or bf16 piece planes
     DBuf beta32_K, beta32_Kp, wr32_K, wr32_Kp, nlsplitA, nlsplitB;
 
@@ -923,7 +928,12 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
     // k+q side comes from that pair's K-side projections (k_nl_mirror)
     const bool mirror = c->nl_chain && c->nl_prev_valid && c->nl_prev_ik == pr.ik &&
                         c->nl_prev_sym == pr.ksym;
+    // and the K side of a pair whose k+q side is the k side of the chain's
+    // head (a closed cycle) from the head's K' projections
+    const bool close_k = c->nl_chain && c->nl_head_valid && c->nl_head_ik == pr.ikq &&
+                         c->nl_head_sym == pr.kqsym;
     c->nl_prev_valid = false;
+    auto skip_side = [&](int side) { return side == 0 ? close_k : mirror; };
 
     const lelph_symmetry& SK = c->syms[pr.kqsym];
     const lelph_symmetry& SKp = c->syms[pr.ksym];
@@ -948,9 +958,9 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
     // NL_SPY sets per block row
     constexpr int NL_SPY = 16;
     const unsigned gy = (unsigned)std::max<int64_t>(1, cdiv(nsb, NL_SPY));
-    const int nsides = mirror ? 1 : 2;
-    for (int side = 0; side < nsides; ++side)
+    for (int side = 0; side < 2; ++side)
     {
+        if (skip_side(side)) continue;
         const DevWfc& w = side == 0 ? wK : wKp;
         const NlSide& sd = side == 0 ? sK : sKp;
         double* kc = side == 0 ? c->nlK.as<double>() : c->nlKp.as<double>();
@@ -972,10 +982,12 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
     NlProj pj{natom, c->pmax, c->ntype};
     if (!nl32)
     {
-        hipLaunchKernelGGL(k_nl_beta<double>, dim3((unsigned)cdiv(wK.npw, 256), natom * c->pmax),
-                           dim3(256), 0, st, c->nlK.as<double>(), c->Y_K.as<double>(), wK.Fk,
-                           wK.npw, pj, c->d_plist, c->d_atype, c->d_apos, c->d_Fsign, -1,
-                           c->beta_K.as<z64>());
+        if (!close_k)
+            hipLaunchKernelGGL(k_nl_beta<double>,
+                               dim3((unsigned)cdiv(wK.npw, 256), natom * c->pmax), dim3(256), 0,
+                               st, c->nlK.as<double>(), c->Y_K.as<double>(), wK.Fk, wK.npw, pj,
+                               c->d_plist, c->d_atype, c->d_apos, c->d_Fsign, -1,
+                               c->beta_K.as<z64>());
         if (!mirror)
             hipLaunchKernelGGL(k_nl_beta<double>,
                                dim3((unsigned)cdiv(wKp.npw, 256), natom * c->pmax), dim3(256), 0,
@@ -984,8 +996,9 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
                                c->beta_Kp.as<z64>());
     }
     HIP_TRY(hipGetLastError());
-    for (int side = 0; side < nsides && prec != LELPH_FP64; ++side)
+    for (int side = 0; side < 2 && prec != LELPH_FP64; ++side)
     {
+        if (skip_side(side)) continue;
         const DevWfc& w = side == 0 ? wK : wKp;
         const z64* A = side == 0 ? c->beta_K.as<z64>() : c->beta_Kp.as<z64>();
         const z64* B = side == 0 ? c->wr_K.as<z64>() : c->wr_Kp.as<z64>();
@@ -1030,8 +1043,9 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
             HIP_TRY(hipGetLastError());
         }
     }
-    for (int side = 0; side < nsides && prec == LELPH_FP64; ++side)
+    for (int side = 0; side < 2 && prec == LELPH_FP64; ++side)
     {
+        if (skip_side(side)) continue;
         const DevWfc& w = side == 0 ? wK : wKp;
         const z64* A = side == 0 ? c->beta_K.as<z64>() : c->beta_Kp.as<z64>();
         const z64* B = side == 0 ? c->wr_K.as<z64>() : c->wr_Kp.as<z64>();
@@ -1056,12 +1070,14 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
         HIP_TRY(hipGetLastError());
     }
     if (mirror)
-    {
         hipLaunchKernelGGL(k_nl_mirror, dim3((unsigned)cdiv((int64_t)rows * ncol, 256)),
                            dim3(256), 0, st, c->bb_Kprev.as<z64>(), (int64_t)rows, ncol,
                            c->bb_Kp.as<z64>());
-        HIP_TRY(hipGetLastError());
-    }
+    if (close_k)
+        hipLaunchKernelGGL(k_nl_mirror, dim3((unsigned)cdiv((int64_t)rows * ncol, 256)),
+                           dim3(256), 0, st, c->bb_Khead.as<z64>(), (int64_t)rows, ncol,
+                           c->bb_K.as<z64>());
+    HIP_TRY(hipGetLastError());
     const int64_t ne = (int64_t)natom * 3 * es;
     z64* eb = ebuf_out ? ebuf_out : c->ebuf.as<z64>();
     hipLaunchKernelGGL(k_nl_kkp, dim3((unsigned)cdiv(ne, 256)), dim3(256), 0, st,
@@ -1079,6 +1095,16 @@ static int run_nonlocal(lelph_ctx* c, const lelph_pair& pr, z64* g, hipStream_t 
         c->nl_prev_valid = true;
         c->nl_prev_ik = pr.ikq;
         c->nl_prev_sym = pr.kqsym;
+        // a chain's head keeps its K' projections for the cycle's close
+        if (!mirror)
+        {
+            LELPH_TRY(c->bb_Khead.ensure(c->bb_Kp.n));
+            std::swap(c->bb_Kp.p, c->bb_Khead.p);
+            std::swap(c->bb_Kp.n, c->bb_Khead.n);
+            c->nl_head_valid = true;
+            c->nl_head_ik = pr.ik;
+            c->nl_head_sym = pr.ksym;
+        }
     }
     if (ebuf_out) return LELPH_OK;
     if (mode_after) HIP_TRY(hipStreamWaitEvent(st, mode_after, 0));
@@ -1285,7 +1311,7 @@ static void free_ctx(lelph_ctx* c)
     DBuf* bufs[] = {&c->dv64, &c->dv32, &c->eig_d, &c->vloc_tmp, &c->rot_tmp, &c->idx_k,
                     &c->idx_kq, &c->ph_k, &c->ph_kq, &c->box_k, &c->box_kq, &c->box_x, &c->part, &c->g,
                     &c->gbatch, &c->presplit, &c->nlK, &c->nlKp, &c->Y_K, &c->Y_Kp, &c->wr_K, &c->wr_Kp,
-                    &c->beta_K, &c->beta_Kp, &c->bb_K, &c->bb_Kp, &c->bb_Kprev, &c->bbpart, &c->ebuf,
+                    &c->beta_K, &c->beta_Kp, &c->bb_K, &c->bb_Kp, &c->bb_Kprev, &c->bb_Khead, &c->bbpart, &c->ebuf,
                     &c->pl_cnt[0], &c->pl_cnt[1], &c->pl_off[0], &c->pl_off[1], &c->pl_cur[0],
                     &c->pl_cur[1], &c->pl_perm[0], &c->pl_perm[1], &c->dm_map, &c->dm_ph,
                     &c->dm_pos, &c->dm_wRS, &c->dm_wS2, &c->dm_part, &c->dm_out, &c->qr_dl,
@@ -1954,6 +1980,7 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     c->box_sides = 0;
     c->nl_chain = true;
     c->nl_prev_valid = false;
+    c->nl_head_valid = false;
     int st = LELPH_OK;
     int64_t landed = 0;
     for (int64_t n = 0; n < npairs && st == LELPH_OK; ++n)
@@ -1979,6 +2006,7 @@ int lelph_gkq_batch(lelph_ctx* c, const double qpt[3], int64_t npairs, const lel
     c->box_head = nullptr;
     c->nl_chain = false;
     c->nl_prev_valid = false;
+    c->nl_head_valid = false;
     if (st != LELPH_OK)
     {
         (void)hipStreamSynchronize(c->nl_stream);
@@ -2129,6 +2157,7 @@ static int multi_enqueue(lelph_multi* m, bool timed)
             {
                 c->nl_chain = true;
                 c->nl_prev_valid = false;
+                c->nl_head_valid = false;
             }
             int nst = LELPH_OK;
             for (int64_t bi : m->border)
@@ -2142,6 +2171,7 @@ static int multi_enqueue(lelph_multi* m, bool timed)
             {
                 c->nl_chain = false;
                 c->nl_prev_valid = false;
+                c->nl_head_valid = false;
             }
             LELPH_TRY(nst);
         }
