@@ -13,7 +13,6 @@
 #pragma once
 
 #include <cmath>
-#include <cstdlib>
 #include <vector>
 
 #include "common.hpp"
@@ -441,16 +440,6 @@ __global__ __launch_bounds__(256) void k_fft_lines_c(cpx<T>* __restrict__ data, 
     }
 }
 
-// persistent line kernels for lstride 1 (LELPH_FFT_PERSIST=0 turns them off)
-static inline bool fft_persistent_lines()
-{
-    static const bool on = [] {
-        const char* e = std::getenv("LELPH_FFT_PERSIST");
-        return !(e && e[0] == '0');
-    }();
-    return on;
-}
-
 // The same line transform where adjacent lines are adjacent in memory
 // (lstride 1: the x and y passes over boxes), as a persistent kernel: a
 // block walks tiles blockIdx.x, + gridDim.x, ... and keeps the elements of
@@ -562,7 +551,7 @@ static bool fft_launch_tl(cpx<T>* data, int64_t n_outer, int64_t n_inner, int64_
     {
         const int64_t nblk = cdiv(n_inner, TL) * n_outer;
         if (nblk > 0x7fffffffLL) return false;
-        if (lstride == 1 && fft_persistent_lines())
+        if (lstride == 1)
         {
             // persistent: as many blocks as stay resident
             auto fn = sign > 0 ? (const void*)k_fft_lines_cp<T, 1, N, TL>
