@@ -523,7 +523,8 @@ static int build_boxes(lelph_ctx* c, const double* qpt, const lelph_pair& pr,
                         fft_has_c(Nz) && Nx <= 1024 && plane_lds_bytes(Ny, Nz) <= 160 * 1024 &&
                         nbox <= 65535;
     // the k side of this pair is the k+q side of the previous one: swap the
-    // boxes and undo that side's umklapp shift instead of transforming again
+    // boxes and undo that side's umklapp shift (none in k-side form) instead
+    // of transforming again; boxes of the two forms are never mixed
     const int prec_tag = std::is_same<T, float>::value ? (defer ? 2 : 1) : 0;
     const bool reuse = c->box_reuse && c->box_prev_valid && c->box_prev_ik == pr.ik &&
                        c->box_prev_sym == pr.ksym && c->box_prev_prec == prec_tag;
