@@ -358,6 +358,36 @@ def test_chain_sides_counts():
     assert _chain_sides([(1, 0, 0, 0), (3, 2, 0, 0)]) == 4
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3", "bf16x6"])
+def test_batch_gamma_self_loops(prec):
+    """At q = 0 every pair's k+q side is its own k side (a cycle of one
+    pair): a batch transforms one psi side per pair, takes the k+q box over
+    from the pair's own psi_k box (a copy in FP64 / FP32, read as it is by
+    the bf16 presplit), and gives the g of pair-by-pair calls bit for bit
+    and of the oracle to the precision's bar."""
+    s = synth.tiny(nbnds=4, q=(0.0, 0.0, 0.0))
+    o = O.OracleSystem(s)
+    pairs = synth.kq_pairs(s, 0)
+    assert all(ikq == ik and kqs == ks for (ikq, ik, kqs, ks) in pairs)
+    say(f"case: q = 0 batch {prec}, {len(pairs)} self-loop pairs")
+    ctx = lelph.Context(s, "fp64")
+    try:
+        ctx.set_dvscf(s.qpts[0], s.eigvec(0), s.dvscf(0), kernel="dfpt")
+        ctx.set_precision(prec)
+        gb = ctx.gkq_batch(pairs, non_loc=True, qpt=s.qpts[0])
+        sides = ctx.last_timings()["box_sides"]
+        one = np.stack([ctx.gkq(*p, non_loc=True, qpt=s.qpts[0]) for p in pairs])
+    finally:
+        ctx.close()
+    assert sides == len(pairs)
+    assert np.array_equal(gb, one)
+    dv = o.dv_total(0)
+    ref = np.stack([o.add_nonlocal(0, *p, o.elph_local(0, *p, dv)) for p in pairs[:3]])
+    err = rel_err(gb[:3], ref)
+    say(f"  vs oracle: {err:.3e}")
+    assert err < (TOL64 if prec == "fp64" else TOL32)
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", TOL32), ("bf16x3", TOL32)])
 @pytest.mark.parametrize("kminusq", [False, True])
 def test_batch_box_reuse(c0, c0ctx, prec, tol, kminusq):
