@@ -388,6 +388,35 @@ def test_batch_gamma_self_loops(prec):
     assert err < (TOL64 if prec == "fp64" else TOL32)
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("bf16x3", 0.0)])
+def test_batch_shuffled_with_repeats(c0, c0ctx, prec, tol):
+    """A batch in random order with repeated pairs: repeats compete for the
+    same chain links, so heads, cycle closes and mirrored non-local sides
+    fall in irregular places; every pair's g equals its pair-by-pair g (bit
+    for bit in the split precisions, which build every box in k-side form)
+    and the batch transforms exactly the sides its chain walk leaves."""
+    s, o, dv = c0
+    rng = np.random.default_rng(5)
+    base = synth.kq_pairs(s, 0)
+    pairs = base + [base[i] for i in rng.choice(len(base), 12, replace=False)]
+    pairs = [pairs[i] for i in rng.permutation(len(pairs))]
+    want_sides = _chain_sides(pairs)
+    say(f"case: shuffled batch with repeats {prec}, {len(pairs)} pairs, sides {want_sides}")
+    c0ctx.set_precision(prec)
+    try:
+        gb = c0ctx.gkq_batch(pairs, non_loc=True, qpt=s.qpts[0])
+        sides = c0ctx.last_timings()["box_sides"]
+        one = {p: c0ctx.gkq(*p, non_loc=True, qpt=s.qpts[0]) for p in set(pairs)}
+    finally:
+        c0ctx.set_precision("fp64")
+    assert sides == want_sides
+    ref = np.stack([one[p] for p in pairs])
+    if tol == 0.0:
+        assert np.array_equal(gb, ref)
+    else:
+        assert rel_err(gb, ref) < tol
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", TOL32), ("bf16x3", TOL32)])
 @pytest.mark.parametrize("kminusq", [False, True])
 def test_batch_box_reuse(c0, c0ctx, prec, tol, kminusq):
