@@ -10,6 +10,14 @@
  * reference puts into its kd-tree and applies the same distance test; the
  * order of the degenerate vectors may differ from the kd-tree's, which only
  * changes the summation order in the R -> q sum.
+ *
+ * Pin status of dyn_lr_internal (the long-range dynamical matrix): the 3-D
+ * form, G != 0 terms and quadrupoles included, is pinned by Ewald
+ * invariance; in the 2-D form the in-plane non-analytic term is pinned
+ * against the published 2-D expression (2 pi e^2/A)(Z^T q)(Z^T q)^T/|q|,
+ * while its 1 - tanh range separation, r_eff and the out-of-plane (Z_z,
+ * quadrupole) terms are checked by invariants only: parity unpinned for
+ * those terms.
  */
 #include <complex.h>
 #include <math.h>
